@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Throughput of the fvmic2d marker step on one MI355X.
+
+A step is one pass of the marker hot path over the convection_tdep workload
+(201x201 uniform grid, 36 markers per cell = 1.44M markers, eta = exp(-ln(1e5) T)):
+findMarkerCells, the density loop, projection of T, eta, rho, k onto all four
+staggered node families plus Cp on basic nodes (17 nodal fields),
+subgridTemperatureChanges and advectMarkersRK -- every kernel of libfvmic2d_hip.so,
+inputs resident in HBM.  Successive steps continue the simulation (markers move).
+
+Prints ONE JSON line.  ``value`` = active markers x steps / seconds, timed with
+HIP events on the library's stream between two synchronisations.  ``roofline``
+describes the kernel with the largest share of the step time, measured in a
+separate pass with per-launch HIP events; ``cpu_baseline`` times the CPU
+restatement of the same step (the oracle) on this host.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "markers/s per MIC step (proj+advect+subgrid); KSP iters to 1e-8 vs plain guess"
+HBM_PEAK_GBS = 8000.0
+
+PHASES = ["locate", "sort", "partials", "nodes", "subgrid_t1", "subgrid_t2", "advect", "subgrid_s1",
+          "marker_update", "other"]
+
+
+def algorithmic_bytes(w):
+    """Compulsory state traffic per launch of each phase, for one convection_tdep step.
+
+    Per marker (FP64 = 8 B, int32 = 4 B) and per node; sort keys, permutations and
+    partial sums are implementation traffic and are not counted."""
+    n, nodes = w.n_markers, w.NX * w.NY
+    return {
+        # read X, Y; write cellX, cellY, cpu
+        "locate": n * (16 + 12),
+        # read X, Y, cellX, cellY of every marker
+        "sort": n * 24,
+        # the 17 projections: read X, Y, T, eta, rho, Mat once (+ dTsub, X, Y in subgrid);
+        # spread over 5 partial launches per step -> reported per launch below
+        "partials": None,
+        "nodes": None,
+        # read X, Y, T, cell; 4 nodal gathers; write dTsub
+        "subgrid_t1": n * (8 * 3 + 8 + 8) + nodes * 32,
+        # read X, Y, T, cell, dTsub; write T, Tdot; read deltaT
+        "subgrid_t2": n * (8 * 4 + 8 + 16) + nodes * 8,
+        # read X, Y, cell, s(3), e(3), Eii; write X, Y, VX, VY, wxy, s(3), Eii; read vx, vy, wxy
+        "advect": n * (8 * 2 + 8 + 24 + 24 + 8 + 72) + nodes * 24,
+    }
+
+
+STEP_BYTES_PER_MARKER = 268  # P1 60 + P2 32 + P3 176
+STEP_BYTES_PER_NODE = 224
+
+
+def cpu_baseline(budget_s=12.0):
+    """The oracle's full step on the same workload, single thread, until budget_s."""
+    import oracle
+    from fvmic2d_amd import workloads
+    w = workloads.convection_tdep()
+    st = oracle.State(w, extra=[p[4] for p in w.projections])
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        st.step_thermal()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": w.n_markers * steps / el, "unit": "markers/s", "cores": 1, "kind": "port",
+            "sample": "%d full convection_tdep steps (1.44M markers each) of the C restatement, "
+                      "gcc -O3 -ffp-contract=off, one thread, %.1f s" % (steps, el),
+            "cpu": platform.processor() or platform.machine()}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
+              "and this build has no multi-GPU path (see DESIGN.md for how it would shard).",
+              file=sys.stderr)
+        return 2
+
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import MarkerPath
+    import ctypes as C
+
+    w = workloads.convection_tdep()
+    mp = MarkerPath(w)
+    n0 = w.n_markers
+    for _ in range(a.warmup):
+        mp.step_thermal()
+    mp.ctx.sync()
+    t0 = time.perf_counter()
+    mp.ctx.timer_start()
+    for _ in range(a.steps):
+        mp.step_thermal()
+    ms = mp.ctx.timer_stop()
+    wall = time.perf_counter() - t0
+    nout = mp.find_marker_cells()
+    active = n0 - nout
+    value = active * a.steps / (ms / 1e3)
+
+    # per-kernel timing pass: every launch bracketed by HIP events on the same stream
+    np_ = 10
+    msp = (C.c_double * np_)()
+    lau = (C.c_int32 * np_)()
+    prof_steps = 5
+    mp.ctx.call("fvm_profile_enable", C.c_int32(1))
+    for _ in range(prof_steps):
+        mp.step_thermal()
+    mp.ctx.call("fvm_profile_read", msp, lau)
+    mp.ctx.call("fvm_profile_enable", C.c_int32(0))
+    phase_ms = {PHASES[i]: msp[i] / prof_steps for i in range(np_) if lau[i]}
+    phase_launches = {PHASES[i]: lau[i] // prof_steps for i in range(np_) if lau[i]}
+    ab = algorithmic_bytes(w)
+    cands = {k: v for k, v in phase_ms.items() if ab.get(k)}
+    dom = max(cands, key=cands.get)
+    per_launch_ms = phase_ms[dom] / phase_launches[dom]
+    achieved = ab[dom] / (per_launch_ms / 1e3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % dom)
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    step_bytes = n0 * STEP_BYTES_PER_MARKER + w.NX * w.NY * STEP_BYTES_PER_NODE
+    out = {
+        "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "convection_tdep", "grid": "201x201 uniform", "markers": n0,
+                   "markers_per_cell": 36, "active_markers_after": active,
+                   "projections": 17, "viscosity_contrast": 1e5},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": per_launch_ms},
+        "step": {"algorithmic_bytes": step_bytes,
+                 "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
+                 "wall_ms_per_step": wall / a.steps * 1e3,
+                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
+    }
+    if not a.no_cpu:
+        cb = cpu_baseline(a.cpu_budget)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

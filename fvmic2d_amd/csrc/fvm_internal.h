@@ -1,0 +1,169 @@
+// Internal declarations shared by the HIP translation units of libfvmic2d_hip.
+// All device code is compiled with -ffp-contract=off: the position and cell
+// chain of the marker path must round exactly like the reference's gcc build
+// (no fused multiply-add), so every expression below keeps the reference's
+// operation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "fvm_abi.h"
+
+struct fvm_buf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+struct fvm_ctx {
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+  char err[512] = {0};
+  // workspace buffers, grown on demand
+  fvm_buf key, slot, perm, count, start, fill, blocksums, partials, scratch, staging, red;
+  fvm_buf pinned;  // small pinned host buffer for scalar results
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // optional per-phase timing (fvm_profile_*): event pairs around launches
+  int profiling = 0;
+  int nprof = 0;
+  hipEvent_t prof_ev[2 * 1024];
+  int prof_phase[1024];
+  // sort cache: the bucket order is valid for (grid, markers, epoch)
+  const void *sort_x = nullptr;
+  const void *sort_X = nullptr;
+  int32_t sort_n = -1;
+  uint64_t sort_epoch = 0, epoch = 1;
+  int32_t sort_nactive = 0;
+};
+
+int fvm_set_err(fvm_ctx *ctx, int code, const char *fmt, ...);
+int fvm_ws(fvm_ctx *ctx, fvm_buf &b, size_t bytes);  // ensure workspace capacity
+
+#define FVM_HIP(ctx, call)                                                               \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fvm_set_err((ctx), FVM_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,  \
+                         hipGetErrorString(e_));                                         \
+  } while (0)
+
+#define FVM_TRY(expr)        \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != FVM_OK) return r_; \
+  } while (0)
+
+#define FVM_LAUNCH_CHECK(ctx) FVM_HIP(ctx, hipGetLastError())
+
+static inline int fvm_grid_check(fvm_ctx *ctx, const fvm_grid *g) {
+  if (!g || !g->x || !g->y || !g->xc || !g->yc || g->NX < 2 || g->NY < 2)
+    return fvm_set_err(ctx, FVM_ERR_ARG, "invalid grid descriptor");
+  if (g->xperiodic) return fvm_set_err(ctx, FVM_ERR_PERIODIC, "periodic grids are not supported");
+  return FVM_OK;
+}
+
+static inline unsigned fvm_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ---- device-side grid view -------------------------------------------------
+struct DGrid {
+  int NX, NY;
+  double LX, LY;
+  const double *x, *y, *xc, *yc;
+};
+static inline DGrid dgrid(const fvm_grid *g) {
+  DGrid d;
+  d.NX = g->NX;
+  d.NY = g->NY;
+  d.LX = g->LX;
+  d.LY = g->LY;
+  d.x = g->x;
+  d.y = g->y;
+  d.xc = g->xc;
+  d.yc = g->yc;
+  return d;
+}
+
+// A marker takes part only if its cell is inside the grid: cellX == -1 marks an
+// inactive marker (fdcode.h:120); any other out-of-range index (a caller that
+// skipped findMarkerCells) is skipped too, so no kernel indexes outside its arrays.
+__device__ __forceinline__ bool cell_ok(int cx, int cy, int NX, int NY) {
+  return (unsigned)cx <= (unsigned)(NX - 2) && (unsigned)cy <= (unsigned)(NY - 2);
+}
+
+// Read of a ghosted local array: indices outside the grid read 0 (a fresh,
+// zeroed DMDA local vector; DMGlobalToLocal fills only owned nodes).
+__device__ __forceinline__ double gat(const double *__restrict__ f, int NX, int NY, int jy, int ix) {
+  return (ix >= 0 && ix < NX && jy >= 0 && jy < NY) ? f[(size_t)jy * NX + ix] : 0.0;
+}
+
+// The reference's bilinear gather written as four accumulations
+// (updateMarkerStress.c:84-88 form): v=0; v+=(1-mdx)(1-mdy)f00; v+=(1-mdx)mdy f10; ...
+__device__ __forceinline__ double bil_acc(const double *__restrict__ f, int NX, int NY, int cy, int cx,
+                                          double mdx, double mdy) {
+  double v = 0.0;
+  v += (1.0 - mdx) * (1.0 - mdy) * gat(f, NX, NY, cy, cx);
+  v += (1.0 - mdx) * mdy * gat(f, NX, NY, cy + 1, cx);
+  v += mdx * (1.0 - mdy) * gat(f, NX, NY, cy, cx + 1);
+  v += mdx * mdy * gat(f, NX, NY, cy + 1, cx + 1);
+  return v;
+}
+
+// ---- projection engine (project.hip) --------------------------------------
+enum {
+  PM_ARITH = 0,   // ARITHMETIC: four bilinear corners
+  PM_LOCAL = 1,   // ARITHMETIC_LOCAL: nearest corner, its bilinear weight
+  PM_HARM = 2,    // HARMONIC: nearest corner, weight 1, value 1/f
+  PM_TIES = 3,    // subgridStressChanges.c:134-175 (<=0.5 / >=0.5, ties hit 2 or 4 nodes)
+  PM_CENTRE = 4   // subgridStressChanges.c:223-233 (one cell-centre node)
+};
+enum { EPI_DIVIDE = 0, EPI_SUBGRID_T = 1, EPI_SUB_FROM = 2 };
+
+#define FVM_MAXF 8
+struct ProjSrc {
+  const double *col;    // device column (marker index), or null
+  double mat[FVM_MAXMAT];
+  int use_mat;
+};
+struct ProjGroup {
+  int stagx, stagy, method, nf;
+  ProjSrc src[FVM_MAXF];
+  double *out[FVM_MAXF];
+  // epilogue: EPI_DIVIDE out = sum/w (HARM: w/sum)
+  //           EPI_SUBGRID_T out = (a + -1*b) + -1*(sum/w)     (VecAXPBYPCZ)
+  //           EPI_SUB_FROM  out = out + -1*(sum/w)             (VecAXPY(-1))
+  int epilogue;
+  const double *ea, *eb;
+};
+
+// findMarkerCells without the host round trip: counters (outside, bad) are
+// left in ctx->red[0..1] (sort.hip).
+int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank);
+// Build the (cell, quadrant) bucket order of the active markers (sort.hip).
+int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk);
+// Run one projection group over the current bucket order (project.hip).
+int fvm_project_group(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg);
+// Invalidate the cached bucket order (positions or cells changed).
+static inline void fvm_markers_moved(fvm_ctx *ctx) { ctx->epoch++; }
+
+// ---- per-phase timing --------------------------------------------------------
+struct ProfScope {
+  fvm_ctx *ctx;
+  int slot;
+  ProfScope(fvm_ctx *c, int phase) : ctx(c), slot(-1) {
+    if (c->profiling && c->nprof < 1024) {
+      slot = c->nprof++;
+      c->prof_phase[slot] = phase;
+      (void)hipEventRecord(c->prof_ev[2 * slot], c->stream);
+    }
+  }
+  ~ProfScope() {
+    if (slot >= 0) (void)hipEventRecord(ctx->prof_ev[2 * slot + 1], ctx->stream);
+  }
+};
+#define FVM_PHASE(ctx, ph) ProfScope fvm_prof_scope_(ctx, ph)
+
+// deterministic double sum of n values on the device -> *out (host), ops.hip
+int fvm_reduce_sum(fvm_ctx *ctx, const double *d, int64_t n, double *out);

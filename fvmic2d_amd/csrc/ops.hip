@@ -1,0 +1,894 @@
+// Per-marker kernels of the hot path: RK4 advection, subgrid temperature and
+// stress diffusion, marker stress / strain-rate / pressure updates, plastic
+// yielding, viscosity limits, the strain-rate residual and the glue loops.
+// Each kernel keeps the reference's expression order (see the file:line
+// citations); the grid must be non-periodic (checked by the entry points).
+#include <float.h>
+
+#include "fvm_internal.h"
+
+int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
+
+// ---- A6: advectMarkersRK (markers.c:1865-2250) --------------------------------
+__global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *__restrict__ MX, double *__restrict__ MY,
+                                                    double *__restrict__ MVX, double *__restrict__ MVY,
+                                                    double *__restrict__ Mwxy, double *__restrict__ sxx,
+                                                    double *__restrict__ syy, double *__restrict__ sxy,
+                                                    const double *__restrict__ exx, const double *__restrict__ eyy,
+                                                    const double *__restrict__ exy, double *__restrict__ Eii,
+                                                    const int32_t *__restrict__ cellXa, const int32_t *__restrict__ cellYa,
+                                                    const double *__restrict__ vx, const double *__restrict__ vy,
+                                                    const double *__restrict__ wxy, BoundaryValues bv, double dt,
+                                                    int jaumann) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int c0x = cellXa[m], c0y = cellYa[m];
+  if (!cell_ok(c0x, c0y, g.NX, g.NY)) return;
+  const int NX = g.NX, NY = g.NY;
+  const int XCMAX = NX - 2, XCMIN = 0;
+  const double X0 = MX[m], Y0 = MY[m];
+  double xcur = X0, ycur = Y0;
+  double mvx[4], mvy[4], mwxy[4];
+#pragma unroll
+  for (int irk = 0; irk < 4; irk++) {
+    int cellX = c0x, cellY = c0y;
+    if (xcur > g.x[cellX + 1]) {
+      while (xcur > g.x[cellX + 1] && cellX < XCMAX) cellX++;
+    } else if (xcur < g.x[cellX]) {
+      while (xcur < g.x[cellX] && cellX > XCMIN) cellX--;
+    }
+    if (ycur > g.y[cellY + 1]) {
+      while (ycur > g.y[cellY + 1] && cellY < NY - 2) cellY++;
+    } else if (ycur < g.y[cellY]) {
+      while (ycur < g.y[cellY] && cellY > 0) cellY--;
+    }
+    int vxX = cellX, vxY = cellY;
+    if (vxY > -1 && ycur < g.yc[vxY + 1]) vxY--;
+    int vyX = cellX;
+    if (vyX > -1 && xcur < g.xc[vyX + 1]) vyX--;
+    int vyY = cellY;
+    if (vxX < 0) vxX = 0;
+    else if (vxX > NX - 2) vxX = NX - 2;
+    if (vyX < -1) vyX = -1;
+    else if (vyX > NX - 2) vyX = NX - 2;
+    if (vxY < -1) vxY = -1;
+    else if (vxY > NY - 2) vxY = NY - 2;
+    if (vyY < 0) vyY = 0;
+    else if (vyY > NY - 2) vyY = NY - 2;
+
+    double deltaxm = xcur - g.x[vxX];
+    double deltaym = ycur - g.yc[vxY + 1];
+    double dx = g.x[vxX + 1] - g.x[vxX];
+    double dy = g.yc[vxY + 2] - g.yc[vxY + 1];
+    double vxul, vxur;
+    if (vxY == -1 && bv.mechBCTop.type[0] == 0) {
+      vxul = 2.0 * bv.mechBCTop.value[0] - gat(vx, NX, NY, vxY + 1, vxX);
+      vxur = 2.0 * bv.mechBCTop.value[0] - gat(vx, NX, NY, vxY + 1, vxX + 1);
+    } else if (vxY == -1 && bv.mechBCTop.type[0] == 1) {
+      vxul = gat(vx, NX, NY, vxY + 1, vxX);
+      vxur = gat(vx, NX, NY, vxY + 1, vxX + 1);
+    } else {
+      vxul = gat(vx, NX, NY, vxY, vxX);
+      vxur = gat(vx, NX, NY, vxY, vxX + 1);
+    }
+    const double vxll = gat(vx, NX, NY, vxY + 1, vxX);
+    const double vxlr = gat(vx, NX, NY, vxY + 1, vxX + 1);
+    double a = vxul * (1.0 - deltaxm / dx) * (1.0 - deltaym / dy);
+    a += vxur * (deltaxm / dx) * (1.0 - deltaym / dy);
+    a += vxll * (1.0 - deltaxm / dx) * (deltaym / dy);
+    a += vxlr * (deltaxm * deltaym) / dx / dy;
+    mvx[irk] = a;
+
+    double vyul;
+    const double vyur = gat(vy, NX, NY, vyY, vyX + 1);
+    double vyll;
+    const double vylr = gat(vy, NX, NY, vyY + 1, vyX + 1);
+    if (vyX == -1 && bv.mechBCLeft.type[1] == 0) {
+      vyul = 2.0 * bv.mechBCLeft.value[1] - gat(vy, NX, NY, vyY, vyX + 1);
+      vyll = 2.0 * bv.mechBCLeft.value[1] - gat(vy, NX, NY, vyY + 1, vyX + 1);
+    } else if (vyX == -1 && bv.mechBCLeft.type[1] == 1) {
+      vyul = gat(vy, NX, NY, vyY, vyX + 1);
+      vyll = gat(vy, NX, NY, vyY + 1, vyX + 1);
+    } else {
+      vyul = gat(vy, NX, NY, vyY, vyX);
+      vyll = gat(vy, NX, NY, vyY + 1, vyX);
+    }
+    deltaxm = xcur - g.xc[vyX + 1];
+    deltaym = ycur - g.y[vyY];
+    dx = g.xc[vyX + 2] - g.xc[vyX + 1];
+    dy = g.y[vyY + 1] - g.y[vyY];
+    double b = vyul * (1.0 - deltaxm / dx) * (1.0 - deltaym / dy);
+    b += vyur * (deltaxm / dx) * (1.0 - deltaym / dy);
+    b += vyll * (1.0 - deltaxm / dx) * (deltaym / dy);
+    b += vylr * (deltaxm * deltaym) / dx / dy;
+    mvy[irk] = b;
+
+    const double mdx = (xcur - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
+    const double mdy = (ycur - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
+    double w = (1.0 - mdx) * (1.0 - mdy) * gat(wxy, NX, NY, cellY, cellX);
+    w += (mdx) * (1.0 - mdy) * gat(wxy, NX, NY, cellY, cellX + 1);
+    w += (1.0 - mdx) * (mdy) * gat(wxy, NX, NY, cellY + 1, cellX);
+    w += (mdx) * (mdy) * gat(wxy, NX, NY, cellY + 1, cellX + 1);
+    mwxy[irk] = w;
+
+    if (irk < 3) {
+      if (irk < 2) {
+        xcur = X0 + dt / 2 * mvx[irk];
+        ycur = Y0 + dt / 2 * mvy[irk];
+      } else {
+        xcur = X0 + dt * mvx[irk];
+        ycur = Y0 + dt * mvy[irk];
+      }
+    }
+  }
+  const double VX = (mvx[0] + 2 * mvx[1] + 2 * mvx[2] + mvx[3]) / 6;
+  const double VY = (mvy[0] + 2 * mvy[1] + 2 * mvy[2] + mvy[3]) / 6;
+  const double wm = (mwxy[0] + 2 * mwxy[1] + 2 * mwxy[2] + mwxy[3]) / 6.0;
+  MVX[m] = VX;
+  MVY[m] = VY;
+  Mwxy[m] = wm;
+  MX[m] = X0 + dt * VX;
+  MY[m] = Y0 + dt * VY;
+
+  const double msxx0 = sxx[m], msyy0 = syy[m], msxy0 = sxy[m];
+  if (jaumann) {
+    const double wxy1 = wm * dt;
+    sxx[m] = msxx0 + 2.0 * msxy0 * -wxy1;
+    syy[m] = msyy0 + 2.0 * msxy0 * wxy1;
+    sxy[m] = msxy0 + (msxx0 - msyy0) * wxy1;
+  } else {
+    const double thetaz = -wm * dt;
+    const double c = cos(thetaz), s = sin(thetaz), s2 = sin(2.0 * thetaz), c2 = cos(2.0 * thetaz);
+    sxx[m] = msxx0 * pow(c, 2.0) + msyy0 * pow(s, 2.0) - msxy0 * s2;
+    syy[m] = msxx0 * pow(s, 2.0) + msyy0 * pow(c, 2.0) + msxy0 * s2;
+    sxy[m] = 0.5 * (msxx0 - msyy0) * s2 + msxy0 * c2;
+  }
+  const double e11 = exx[m], e22 = eyy[m], e12 = exy[m];
+  Eii[m] = Eii[m] + dt * sqrt(0.5 * (e11 * e11 + e22 * e22) + e12 * e12);
+}
+
+extern "C" int fvm_advect_markers_rk(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays *nf, const fvm_grid *g,
+                                     const BoundaryValues *bv, double dt, int32_t jaumann) {
+  if (!ctx || !mk || !nf || !bv || !nf->vx || !nf->vy || !nf->wxy) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  if (mk->nMark == 0) return FVM_OK;
+  {
+    FVM_PHASE(ctx, FVM_PHASE_ADVECT);
+    k_advect_rk4<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(
+        dgrid(g), mk->nMark, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy,
+        mk->exy, mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  fvm_markers_moved(ctx);
+  return FVM_OK;
+}
+
+// ---- A7: subgridTemperatureChanges (subgridTemperatureChanges.c:5-124) -------
+__global__ void __launch_bounds__(256) k_subgrid_t1(DGrid g, int32_t n, const double *__restrict__ MX,
+                                                    const double *__restrict__ MY, const double *__restrict__ MT,
+                                                    const int32_t *__restrict__ cX, const int32_t *__restrict__ cY,
+                                                    const double *__restrict__ T, const double *__restrict__ rho,
+                                                    const double *__restrict__ Cp, const double *__restrict__ kT,
+                                                    double rho0, double dt, double dsubgrid, double *__restrict__ dTsub) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int cellX = cX[m], cellY = cY[m];
+  if (!cell_ok(cellX, cellY, g.NX, g.NY)) {
+    dTsub[m] = 0.0;
+    return;
+  }
+  const int NX = g.NX, NY = g.NY;
+  const double dx = g.x[cellX + 1] - g.x[cellX];
+  const double dy = g.y[cellY + 1] - g.y[cellY];
+  const double mdx = (MX[m] - g.x[cellX]) / dx;
+  const double mdy = (MY[m] - g.y[cellY]) / dy;
+  const double Tm = gat(T, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
+                    gat(T, NX, NY, cellY, cellX + 1) * mdx * (1.0 - mdy) +
+                    gat(T, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
+                    gat(T, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
+  const double rhom = rho0 + gat(rho, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
+                      gat(rho, NX, NY, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
+                      gat(rho, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
+                      gat(rho, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
+  const double Cpm = gat(Cp, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
+                     gat(Cp, NX, NY, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
+                     gat(Cp, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
+                     gat(Cp, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
+  const double km = gat(kT, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
+                    gat(kT, NX, NY, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
+                    gat(kT, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
+                    gat(kT, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
+  const double dTdiff = Cpm * rhom / km / (2.0 / (dx * dx) + 2.0 / (dy * dy));
+  dTsub[m] = (Tm - MT[m]) * (1.0 - exp(-dsubgrid * dt / dTdiff));
+}
+
+__global__ void __launch_bounds__(256) k_subgrid_t2(DGrid g, int32_t n, const double *__restrict__ MX,
+                                                    const double *__restrict__ MY, double *__restrict__ MT,
+                                                    double *__restrict__ MTdot, const int32_t *__restrict__ cX,
+                                                    const int32_t *__restrict__ cY, const double *__restrict__ dT,
+                                                    const double *__restrict__ dTsub, double dt) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int cellX = cX[m], cellY = cY[m];
+  if (!cell_ok(cellX, cellY, g.NX, g.NY)) return;
+  const int NX = g.NX, NY = g.NY;
+  const double dx = g.x[cellX + 1] - g.x[cellX];
+  const double dy = g.y[cellY + 1] - g.y[cellY];
+  const double mdx = (MX[m] - g.x[cellX]) / dx;
+  const double mdy = (MY[m] - g.y[cellY]) / dy;
+  const double dtm = (gat(dT, NX, NY, cellY, cellX) * (1.0 - mdy) * (1.0 - mdx) +
+                      gat(dT, NX, NY, cellY + 1, cellX) * mdy * (1.0 - mdx) +
+                      gat(dT, NX, NY, cellY, cellX + 1) * (1.0 - mdy) * mdx +
+                      gat(dT, NX, NY, cellY + 1, cellX + 1) * mdy * mdx) +
+                     dTsub[m];
+  MT[m] = MT[m] + dtm;
+  MTdot[m] = dtm / dt;
+}
+
+extern "C" int fvm_subgrid_temperature_changes(fvm_ctx *ctx, const double *thisT, const fvm_grid *g,
+                                               const NodalArrays *nf, fvm_markers *mk, const Materials *mat,
+                                               double dt, double dsubgrid) {
+  if (!ctx || !mk || !nf || !mat || !thisT || !nf->lastT || !nf->rho || !nf->Cp || !nf->kThermal)
+    return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  const int32_t n = mk->nMark;
+  const int64_t nn = (int64_t)g->NX * g->NY;
+  FVM_TRY(fvm_ws(ctx, ctx->scratch, ((size_t)n + (size_t)nn + 64) * sizeof(double)));
+  double *dTsub = (double *)ctx->scratch.ptr;
+  double *deltaT = dTsub + ((n + 31) / 32) * 32;
+  const DGrid dg = dgrid(g);
+  if (n > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
+      k_subgrid_t1<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->T, mk->cellX, mk->cellY,
+                                                                 nf->lastT, nf->rho, nf->Cp, nf->kThermal,
+                                                                 mat->materialRho[0], dt, dsubgrid, dTsub);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  ProjGroup pg;
+  memset(&pg, 0, sizeof(pg));
+  pg.stagx = pg.stagy = 0;
+  pg.method = PM_ARITH;
+  pg.nf = 1;
+  pg.src[0].col = dTsub;
+  pg.out[0] = deltaT;
+  pg.epilogue = EPI_SUBGRID_T;
+  pg.ea = thisT;
+  pg.eb = nf->lastT;
+  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  if (n > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T2);
+      k_subgrid_t2<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->T, mk->Tdot, mk->cellX,
+                                                                 mk->cellY, deltaT, dTsub, dt);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  return FVM_OK;
+}
+
+// ---- A8: subgridStressChanges (subgridStressChanges.c:5-318) ------------------
+__global__ void __launch_bounds__(256) k_subgrid_s1(DGrid g, int32_t n, const double *__restrict__ MX,
+                                                    const double *__restrict__ MY, const double *__restrict__ eta,
+                                                    const double *__restrict__ mu, double *__restrict__ sxx,
+                                                    double *__restrict__ syy, double *__restrict__ sxy,
+                                                    const int32_t *__restrict__ cXa, const int32_t *__restrict__ cYa,
+                                                    const double *__restrict__ soxx, const double *__restrict__ soyy,
+                                                    const double *__restrict__ soxy, double dt, double dsubgrid,
+                                                    double *__restrict__ dxy, double *__restrict__ dxx,
+                                                    double *__restrict__ dyy) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int c0x = cXa[m], c0y = cYa[m];
+  if (!cell_ok(c0x, c0y, g.NX, g.NY)) {
+    dxy[m] = 0.0;
+    dxx[m] = 0.0;
+    dyy[m] = 0.0;
+    return;
+  }
+  const int NX = g.NX, NY = g.NY;
+  const double X = MX[m], Y = MY[m];
+  const double sdm = eta[m] / mu[m];
+  double sdif = -dsubgrid * dt / sdm;
+  if (sdif < -30.0) sdif = -30.0;
+  sdif = 1.0 - exp(sdif);
+
+  int cellY = c0y, cellX = c0x;
+  if (cellX < 0) cellX = 0;
+  if (cellX > NX - 2) cellX = NX - 2;
+  if (cellY < 0) cellY = 0;
+  if (cellY > NY - 2) cellY = NY - 2;
+  double mdx = (X - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
+  double mdy = (Y - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
+  double sxym = 0.0;
+  sxym += (1.0 - mdx) * (1.0 - mdy) * gat(soxy, NX, NY, cellY, cellX);
+  sxym += (1.0 - mdx) * mdy * gat(soxy, NX, NY, cellY + 1, cellX);
+  sxym += mdx * (1.0 - mdy) * gat(soxy, NX, NY, cellY, cellX + 1);
+  sxym += mdx * mdy * gat(soxy, NX, NY, cellY + 1, cellX + 1);
+  double sxy0 = sxy[m];
+  double dsxym = sxym - sxy0;
+  dsxym *= sdif;
+  sxy[m] = sxy0 + dsxym;
+  dxy[m] = dsxym;
+
+  cellX = c0x;
+  cellY = c0y;
+  if (cellX < NX - 1 && X >= g.xc[cellX + 1]) cellX++;
+  if (cellY < NY - 1 && Y >= g.yc[cellY + 1]) cellY++;
+  if (cellY < 1) cellY = 1;
+  else if (cellY > NY - 2) cellY = NY - 2;
+  if (cellX < 1) cellX = 1;
+  else if (cellX > NX - 2) cellX = NX - 2;
+  mdx = (X - g.xc[cellX]) / (g.xc[cellX + 1] - g.xc[cellX]);
+  mdy = (Y - g.yc[cellY]) / (g.yc[cellY + 1] - g.yc[cellY]);
+  const double sxxm = bil_acc(soxx, NX, NY, cellY, cellX, mdx, mdy);
+  const double sxx0 = sxx[m];
+  double dsxxm = sxxm - sxx0;
+  dsxxm *= sdif;
+  sxx[m] = sxx0 + dsxxm;
+  const double syym = bil_acc(soyy, NX, NY, cellY, cellX, mdx, mdy);
+  const double syy0 = syy[m];
+  double dsyym = syym - syy0;
+  dsyym *= sdif;
+  syy[m] = syy0 + dsyym;
+  dxx[m] = dsxxm;
+  dyy[m] = dsyym;
+}
+
+extern "C" int fvm_subgrid_stress_changes(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, fvm_markers *mk,
+                                          const Materials *mat, double dt, double dsubgrid) {
+  (void)mat;
+  if (!ctx || !mk || !nf || !nf->soxx || !nf->soyy || !nf->soxy || !nf->dsxx || !nf->dsyy || !nf->dsxy)
+    return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  const int32_t n = mk->nMark;
+  const size_t cs = ((size_t)n + 31) / 32 * 32;
+  FVM_TRY(fvm_ws(ctx, ctx->scratch, (3 * cs + 64) * sizeof(double)));
+  double *dxy = (double *)ctx->scratch.ptr, *dxx = dxy + cs, *dyy = dxx + cs;
+  if (n > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_SUBGRID_S1);
+      k_subgrid_s1<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dgrid(g), n, mk->X, mk->Y, mk->eta, mk->mu, mk->sxx,
+                                                                 mk->syy, mk->sxy, mk->cellX, mk->cellY, nf->soxx,
+                                                                 nf->soyy, nf->soxy, dt, dsubgrid, dxy, dxx, dyy);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  ProjGroup pg;
+  memset(&pg, 0, sizeof(pg));
+  pg.stagx = pg.stagy = 0;
+  pg.method = PM_TIES;
+  pg.nf = 1;
+  pg.src[0].col = dxy;
+  pg.out[0] = nf->dsxy;
+  pg.epilogue = EPI_SUB_FROM;
+  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  pg.method = PM_CENTRE;
+  pg.nf = 2;
+  pg.src[0].col = dxx;
+  pg.src[1].col = dyy;
+  pg.out[0] = nf->dsxx;
+  pg.out[1] = nf->dsyy;
+  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  return FVM_OK;
+}
+
+// ---- A9: updateMarkerStress (updateMarkerStress.c:5-163) ----------------------
+__global__ void __launch_bounds__(256) k_update_stress(DGrid g, int32_t n, const double *__restrict__ MX,
+                                                       const double *__restrict__ MY, double *__restrict__ sxx,
+                                                       double *__restrict__ syy, double *__restrict__ sxy,
+                                                       const int32_t *__restrict__ cXa, const int32_t *__restrict__ cYa,
+                                                       const double *__restrict__ dsxx, const double *__restrict__ dsyy,
+                                                       const double *__restrict__ dsxy) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int c0x = cXa[m], c0y = cYa[m];
+  if (!cell_ok(c0x, c0y, g.NX, g.NY)) return;
+  const int NX = g.NX, NY = g.NY;
+  const double X = MX[m], Y = MY[m];
+  int cellX = c0x, cellY = c0y;
+  if (cellX < NX - 2 && X >= g.xc[cellX + 1]) cellX++;
+  if (cellY < NY - 1 && Y >= g.yc[cellY + 1]) cellY++;
+  if (cellY < 1) cellY = 1;
+  else if (cellY > NY - 2) cellY = NY - 2;
+  if (cellX < 1) cellX = 1;
+  else if (cellX > NX - 2) cellX = NX - 2;
+  double mdx = (X - g.xc[cellX]) / (g.xc[cellX + 1] - g.xc[cellX]);
+  double mdy = (Y - g.yc[cellY]) / (g.yc[cellY + 1] - g.yc[cellY]);
+  const double mdsxx = bil_acc(dsxx, NX, NY, cellY, cellX, mdx, mdy);
+  const double mdsyy = bil_acc(dsyy, NX, NY, cellY, cellX, mdx, mdy);
+  sxx[m] = sxx[m] + mdsxx;
+  syy[m] = syy[m] + mdsyy;
+  cellX = c0x;
+  cellY = c0y;
+  if (cellY < 0) cellY = 0;
+  else if (cellY > NY - 2) cellY = NY - 2;
+  if (cellX < 0) cellX = 0;
+  else if (cellX > NX - 2) cellX = NX - 2;
+  mdx = (X - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
+  mdy = (Y - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
+  const double mdsxy = bil_acc(dsxy, NX, NY, cellY, cellX, mdx, mdy);
+  sxy[m] = sxy[m] + mdsxy;
+}
+
+extern "C" int fvm_update_marker_stress(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, fvm_markers *mk,
+                                        const Materials *mat) {
+  (void)mat;
+  if (!ctx || !mk || !nf || !nf->dsxx || !nf->dsyy || !nf->dsxy) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  if (mk->nMark == 0) return FVM_OK;
+  {
+    FVM_PHASE(ctx, FVM_PHASE_MARKER_UPDATE);
+    k_update_stress<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(dgrid(g), mk->nMark, mk->X, mk->Y, mk->sxx,
+                                                                           mk->syy, mk->sxy, mk->cellX, mk->cellY,
+                                                                           nf->dsxx, nf->dsyy, nf->dsxy);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  return FVM_OK;
+}
+
+// ---- A10: updateMarkerStrainPressure (updateMarkerStrainPressure.c:9-399) -----
+struct StrainIn {
+  const double *dsxx, *dsyy, *dsxy, *soxx, *soyy, *soxy, *exx, *eyy, *exy, *wxy, *p;
+};
+__global__ void __launch_bounds__(256) k_strain_pressure(DGrid g, int32_t n, const double *__restrict__ MX,
+                                                         const double *__restrict__ MY, const double *__restrict__ eta,
+                                                         const double *__restrict__ mu, double *__restrict__ Mp,
+                                                         double *__restrict__ Mwxy, double *__restrict__ e11,
+                                                         double *__restrict__ e22, double *__restrict__ e12,
+                                                         double *__restrict__ srr, const int32_t *__restrict__ cXa,
+                                                         const int32_t *__restrict__ cYa, StrainIn F, double dt,
+                                                         int32_t *__restrict__ flags) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int c0x = cXa[m], c0y = cYa[m];
+  if (!cell_ok(c0x, c0y, g.NX, g.NY)) return;
+  const int NX = g.NX, NY = g.NY;
+  const double X = MX[m], Y = MY[m];
+  int cellX = c0x, cellY = c0y;
+  if (cellX < NX - 2 && X >= g.xc[cellX + 1]) cellX++;
+  if (cellY < NY - 1 && Y >= g.yc[cellY + 1]) cellY++;
+  if (cellY < 1) cellY = 1;
+  else if (cellY > NY - 2) cellY = NY - 2;
+  if (cellX < 1) cellX = 1;
+  else if (cellX > NX - 2) cellX = NX - 2;
+  double mdx = (X - g.xc[cellX]) / (g.xc[cellX + 1] - g.xc[cellX]);
+  double mdy = (Y - g.yc[cellY]) / (g.yc[cellY + 1] - g.yc[cellY]);
+  Mp[m] = bil_acc(F.p, NX, NY, cellY, cellX, mdx, mdy);
+  const double mexx = bil_acc(F.exx, NX, NY, cellY, cellX, mdx, mdy);
+  const double msxx = bil_acc(F.soxx, NX, NY, cellY, cellX, mdx, mdy);
+  const double msyy = bil_acc(F.soyy, NX, NY, cellY, cellX, mdx, mdy);
+  const double mdsxx = bil_acc(F.dsxx, NX, NY, cellY, cellX, mdx, mdy);
+  const double mdsyy = bil_acc(F.dsyy, NX, NY, cellY, cellX, mdx, mdy);
+  cellX = c0x;
+  cellY = c0y;
+  if (cellY < 0) cellY = 0;
+  else if (cellY > NY - 2) cellY = NY - 2;
+  if (cellX < 0) cellX = 0;
+  else if (cellX > NX - 2) cellX = NX - 2;
+  mdx = (X - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
+  mdy = (Y - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
+  Mwxy[m] = bil_acc(F.wxy, NX, NY, cellY, cellX, mdx, mdy);
+  const double mexy = bil_acc(F.exy, NX, NY, cellY, cellX, mdx, mdy);
+  const double msxy = bil_acc(F.soxy, NX, NY, cellY, cellX, mdx, mdy);
+  const double mdsxy = bil_acc(F.dsxy, NX, NY, cellY, cellX, mdx, mdy);
+  const double em = eta[m], um = mu[m];
+  const double t11 = msxx / 2.0 / em + mdsxx / 2.0 / dt / um;
+  const double t22 = msyy / 2.0 / em + mdsyy / 2.0 / dt / um;
+  const double t12 = msxy / 2.0 / em + mdsxy / 2.0 / dt / um;
+  e11[m] = t11;
+  e22[m] = t22;
+  e12[m] = t12;
+  const double r = sqrt(pow(t11 - mexx, 2.0) + pow(t12 - mexy, 2.0));
+  srr[m] = r;
+  if (r != r) atomicOr(&flags[0], 1);
+}
+
+extern "C" int fvm_update_marker_strain_pressure(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
+                                                 fvm_markers *mk, const Materials *mat, const Options *opt,
+                                                 double dt) {
+  (void)mat;
+  (void)opt;
+  if (!ctx || !mk || !nf) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  StrainIn F = {nf->dsxx, nf->dsyy, nf->dsxy, nf->soxx, nf->soyy, nf->soxy,
+                nf->edotxx, nf->edotyy, nf->edotxy, nf->wxy, nf->p};
+  if (!F.dsxx || !F.dsyy || !F.dsxy || !F.soxx || !F.soyy || !F.soxy || !F.exx || !F.exy || !F.wxy || !F.p)
+    return FVM_ERR_ARG;
+  FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
+  int32_t *flags = (int32_t *)ctx->red.ptr + 8;
+  FVM_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int32_t), ctx->stream));
+  if (mk->nMark > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_MARKER_UPDATE);
+      k_strain_pressure<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(
+          dgrid(g), mk->nMark, mk->X, mk->Y, mk->eta, mk->mu, mk->p, mk->wxy, mk->exx, mk->eyy, mk->exy,
+          mk->strainRateResidual, mk->cellX, mk->cellY, F, dt, flags);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  int32_t *h = (int32_t *)ctx->pinned.ptr;
+  FVM_HIP(ctx, hipMemcpyAsync(h, flags, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (h[0]) return fvm_set_err(ctx, FVM_ERR_NAN_RESIDUAL, "NaN strain rate residual on an active marker");
+  return FVM_OK;
+}
+
+// ---- A11: checkPlasticYielding + computeRheology (markers.c:1381-1505,
+//           viscosity.c:12-75) ------------------------------------------------
+struct YieldParams {
+  Materials mat;
+  double etamin, etamax, fractionalEtamin;
+  double slabAngle, decoupleThickness, minDecoupleDepth, maxDecoupleDepth, decoupleEtaScaler;
+  double dt;
+};
+
+__device__ __forceinline__ int rheology(const YieldParams &P, int mm, double T, double X, double Y, double *eta,
+                                        double *mu) {
+  const Materials &mat = P.mat;
+  const double thismu = mat.materialMu[mm];
+  const int law = mat.hasEtaT[mm];
+  double thiseta;
+  if (law == 1) {
+    thiseta = mat.materialEta[mm] * exp(mat.QonR[mm] * (1.0 / T - 1.0 / mat.Tref[mm]));
+  } else if (law == 2) {
+    const double T0 = mat.Tref[mm];
+    thiseta = mat.materialEta[mm] * exp(mat.QonR[mm] / T0 * (1.0 - (T - T0) / T0));
+  } else if (law == 3) {
+    thiseta = mat.materialEta[mm] * T;
+  } else if (law == 5) {
+    thiseta = mat.materialEta[mm] * exp(-mat.QonR[mm] * T);
+  } else if (law == 0) {
+    thiseta = mat.materialEta[mm];
+  } else if (law == 6) {
+    const double depth = Y, distance = X;
+    thiseta = mat.materialEta[mm];
+    if (distance > depth / tan(P.slabAngle)) {
+      if (distance < (depth / tan(P.slabAngle) + P.decoupleThickness)) {
+        if (depth > P.minDecoupleDepth && depth < P.maxDecoupleDepth) thiseta *= P.decoupleEtaScaler;
+      }
+    }
+  } else {
+    return 1;
+  }
+  *eta = thiseta;
+  *mu = thismu;
+  return 0;
+}
+
+__global__ void __launch_bounds__(256) k_yield(int32_t n, YieldParams P, const double *__restrict__ MX,
+                                               const double *__restrict__ MY, const double *__restrict__ MT,
+                                               const int32_t *__restrict__ Mat, double *__restrict__ eta,
+                                               double *__restrict__ mu, double *__restrict__ sxx,
+                                               double *__restrict__ syy, double *__restrict__ sxy,
+                                               const double *__restrict__ exx, const double *__restrict__ eyy,
+                                               const double *__restrict__ exy, const double *__restrict__ Mp,
+                                               const double *__restrict__ MEii, const int32_t *__restrict__ cX,
+                                               int32_t *__restrict__ flags) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  if (cX[m] == -1) return;
+  const int mm = Mat[m];
+  if ((unsigned)mm >= FVM_MAXMAT) {
+    atomicOr(&flags[1], 1);
+    return;
+  }
+  const double eta0 = eta[m], mu0 = mu[m];
+  double s11 = sxx[m], s22 = syy[m], s12 = sxy[m];
+  const double e11 = exx[m], e22 = eyy[m], e12 = exy[m];
+  double etanew = eta0;
+  const double X = eta0 / (mu0 * P.dt + eta0);
+  const double sxxnew = s11 * X + 2 * eta0 * e11 * (1.0 - X);
+  const double syynew = s22 * X + 2 * eta0 * e22 * (1.0 - X);
+  const double sxynew = s12 * X + 2 * eta0 * e12 * (1.0 - X);
+  const double siinew = sqrt(0.5 * (sxxnew * sxxnew + syynew * syynew) + sxynew * sxynew);
+  double etaL, muL;
+  if (rheology(P, mm, MT[m], MX[m], MY[m], &etaL, &muL)) {
+    atomicOr(&flags[1], 1);
+    return;
+  }
+  mu[m] = muL;
+  double etaOut = etaL;
+  const Materials &mat = P.mat;
+  if (mat.hasPlasticity[mm]) {
+    double siiyield;
+    const int plast = mat.hasPlasticity[mm];
+    const double pm = Mp[m];
+    if (plast == 0) {
+      siiyield = DBL_MAX;
+    } else if (plast == 1) {
+      siiyield = mat.materialCohesion[mm] + mat.materialFriction[mm] * pm;
+    } else if (plast == 2) {
+      const double mEii = MEii[m];
+      double c, f;
+      if (mEii < mat.gamma[mm][0]) {
+        c = mat.C[mm][0];
+        f = sin(mat.F[mm][0] / 180.0 * M_PI);
+      } else if (mEii >= mat.gamma[mm][0] && mEii <= mat.gamma[mm][1]) {
+        const double dg = (mEii - mat.gamma[mm][0]) / (mat.gamma[mm][1] - mat.gamma[mm][0]);
+        c = (mat.C[mm][1] - mat.C[mm][0]) * dg + mat.C[mm][0];
+        f = (sin(mat.F[mm][1] / 180.0 * M_PI) - sin(mat.F[mm][0] / 180.0 * M_PI)) * dg + sin(mat.F[mm][0] / 180.0 * M_PI);
+      } else {
+        c = mat.C[mm][1];
+        f = sin(mat.F[mm][1] / 180.0 * M_PI);
+      }
+      siiyield = c + f * pm;
+    } else {
+      siiyield = 0.0;
+    }
+    if (siiyield < 0.0) siiyield = 0.0;
+    if (siinew > siiyield) {
+      const double eiiold = sqrt(0.5 * (e11 * e11 + e22 * e22) + e12 * e12);
+      etanew = siiyield / 2.0 / eiiold;
+      // markers[m].eta already holds the law's value here (markers.c:1410)
+      if (etanew / etaL < P.fractionalEtamin) etanew = P.fractionalEtamin * etaL;
+      if (etanew < P.etamin) etanew = P.etamin;
+      else if (etanew > P.etamax) etanew = P.etamax;
+      const double siiold = sqrt(0.5 * (s11 * s11 + s22 * s22) + s12 * s12);
+      if (siiold != 0.0) {
+        s11 *= siiyield / siiold;
+        s22 *= siiyield / siiold;
+        s12 *= siiyield / siiold;
+        sxx[m] = s11;
+        syy[m] = s22;
+        sxy[m] = s12;
+      }
+      atomicOr(&flags[0], 1);
+    }
+    etaOut = etanew;  // markers.c:1493 (the pre-call eta when not yielding)
+  }
+  eta[m] = etaOut;
+}
+
+extern "C" int fvm_check_plastic_yielding(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const Materials *mat,
+                                          double dt, int32_t *isYielding, const Options *opt, double elapsedTime) {
+  (void)g;
+  (void)elapsedTime;
+  if (!ctx || !mk || !mat || !opt) return FVM_ERR_ARG;
+  YieldParams P;
+  P.mat = *mat;
+  P.etamin = opt->etamin;
+  P.etamax = opt->etamax;
+  P.fractionalEtamin = opt->fractionalEtamin;
+  P.slabAngle = opt->slabAngle;
+  P.decoupleThickness = opt->decoupleThickness;
+  P.minDecoupleDepth = opt->minDecoupleDepth;
+  P.maxDecoupleDepth = opt->maxDecoupleDepth;
+  P.decoupleEtaScaler = opt->decoupleEtaScaler;
+  P.dt = dt;
+  FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
+  int32_t *flags = (int32_t *)ctx->red.ptr + 16;
+  FVM_HIP(ctx, hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), ctx->stream));
+  if (mk->nMark > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_MARKER_UPDATE);
+      k_yield<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(mk->nMark, P, mk->X, mk->Y, mk->T, mk->Mat, mk->eta,
+                                                                     mk->mu, mk->sxx, mk->syy, mk->sxy, mk->exx,
+                                                                     mk->eyy, mk->exy, mk->p, mk->Eii, mk->cellX, flags);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  int32_t *h = (int32_t *)ctx->pinned.ptr;
+  FVM_HIP(ctx, hipMemcpyAsync(h, flags, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (isYielding && h[0]) *isYielding = 1;
+  if (h[1]) return fvm_set_err(ctx, FVM_ERR_VISCOSITY_LAW, "viscosity law undefined for a marker material");
+  return FVM_OK;
+}
+
+__global__ void k_limit_viscosity(int32_t n, double *__restrict__ eta, double etamin, double etamax) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double e = eta[m];
+  if (e > etamax) eta[m] = etamax;
+  else if (e < etamin) eta[m] = etamin;
+}
+
+extern "C" int fvm_limit_viscosity(fvm_ctx *ctx, fvm_markers *mk, const Options *opt) {
+  if (!ctx || !mk || !opt) return FVM_ERR_ARG;
+  if (mk->nMark == 0) return FVM_OK;
+  {
+    FVM_PHASE(ctx, FVM_PHASE_MARKER_UPDATE);
+    k_limit_viscosity<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(mk->nMark, mk->eta, opt->etamin,
+                                                                             opt->etamax);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  return FVM_OK;
+}
+
+// ---- deterministic sums ----------------------------------------------------------
+#define RED_BS 256
+__global__ void __launch_bounds__(RED_BS) k_block_sum(const double *__restrict__ a, int64_t n, int64_t per,
+                                                      double *__restrict__ out) {
+  __shared__ double s[RED_BS];
+  const int64_t b0 = (int64_t)blockIdx.x * per;
+  const int64_t b1 = b0 + per < n ? b0 + per : n;
+  double acc = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += RED_BS) acc += a[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = RED_BS / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < (unsigned)off) s[threadIdx.x] += s[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s[0];
+}
+
+int fvm_reduce_sum(fvm_ctx *ctx, const double *d, int64_t n, double *out) {
+  const int64_t nb = n < 1024 * RED_BS ? (n + RED_BS - 1) / RED_BS : 1024;
+  const int64_t per = nb > 0 ? (n + nb - 1) / nb : 0;
+  FVM_TRY(fvm_ws(ctx, ctx->red, 4096 + (size_t)(nb + 1) * sizeof(double)));
+  double *parts = (double *)((char *)ctx->red.ptr + 4096);
+  if (nb > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_OTHER);
+      k_block_sum<<<(unsigned)nb, RED_BS, 0, ctx->stream>>>(d, n, per, parts);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+    {
+      FVM_PHASE(ctx, FVM_PHASE_OTHER);
+      k_block_sum<<<1, RED_BS, 0, ctx->stream>>>(parts, nb, nb, parts + nb);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  double *h = (double *)ctx->pinned.ptr + 8;
+  if (nb > 0) {
+    FVM_HIP(ctx, hipMemcpyAsync(h, parts + nb, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *out = *h;
+  } else {
+    *out = 0.0;
+  }
+  return FVM_OK;
+}
+
+// ---- A12: updateGlobalStrainRateResidual (residual.c:12-51) --------------------
+__global__ void k_residual_area(DGrid g, double *__restrict__ srr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)g.NX * g.NY) return;
+  const int ix = (int)(i % g.NX), jy = (int)(i / g.NX);
+  if (ix > 0 && jy > 0) {
+    const double dx = g.x[ix] - g.x[ix - 1];
+    const double dy = g.y[jy] - g.y[jy - 1];
+    srr[i] = srr[i] * (dx * dy);
+  } else {
+    srr[i] = 0.0;
+  }
+}
+
+extern "C" int fvm_update_global_strain_rate_residual(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk,
+                                                      double *residual) {
+  if (!ctx || !mk || !residual) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  const int64_t nn = (int64_t)g->NX * g->NY;
+  FVM_TRY(fvm_ws(ctx, ctx->scratch, ((size_t)nn + 64) * sizeof(double)));
+  double *srr = (double *)ctx->scratch.ptr;
+  ProjGroup pg;
+  memset(&pg, 0, sizeof(pg));
+  pg.stagx = pg.stagy = -1;
+  pg.method = PM_ARITH;
+  pg.nf = 1;
+  pg.src[0].col = mk->strainRateResidual;
+  pg.out[0] = srr;
+  pg.epilogue = EPI_DIVIDE;
+  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  {
+    FVM_PHASE(ctx, FVM_PHASE_OTHER);
+    k_residual_area<<<fvm_blocks(nn, 256), 256, 0, ctx->stream>>>(dgrid(g), srr);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  double sum = 0.0;
+  FVM_TRY(fvm_reduce_sum(ctx, srr, nn, &sum));
+  sum /= (g->LX * g->LY);
+  *residual = sum;
+  return FVM_OK;
+}
+
+// ---- A13: projectNodalFieldToMarkersS (markers.c:1670-1706) --------------------
+__global__ void k_nodal_to_markers(DGrid g, int32_t n, const double *__restrict__ MX, const double *__restrict__ MY,
+                                   const int32_t *__restrict__ cX, const int32_t *__restrict__ cY,
+                                   const double *__restrict__ f, double *__restrict__ out) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int cellX = cX[m], cellY = cY[m];
+  if (!cell_ok(cellX, cellY, g.NX, g.NY)) return;
+  const int NX = g.NX, NY = g.NY;
+  const double mdx = (MX[m] - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
+  const double mdy = (MY[m] - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
+  out[m] = gat(f, NX, NY, cellY, cellX) * (1 - mdx) * (1 - mdy) + gat(f, NX, NY, cellY + 1, cellX) * (1 - mdx) * mdy +
+           gat(f, NX, NY, cellY, cellX + 1) * (mdx) * (1 - mdy) + gat(f, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
+}
+
+extern "C" int fvm_project_nodal_field_to_markers(fvm_ctx *ctx, const fvm_grid *g, const double *nodalField,
+                                                  fvm_markers *mk, double *markerField) {
+  if (!ctx || !mk || !nodalField || !markerField) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  if (mk->nMark == 0) return FVM_OK;
+  {
+    FVM_PHASE(ctx, FVM_PHASE_MARKER_UPDATE);
+    k_nodal_to_markers<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(dgrid(g), mk->nMark, mk->X, mk->Y,
+                                                                              mk->cellX, mk->cellY, nodalField,
+                                                                              markerField);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  return FVM_OK;
+}
+
+// ---- A14: glue loops of main_subduction.c ---------------------------------------
+struct MatTab {
+  double rho[FVM_MAXMAT], alpha[FVM_MAXMAT];
+};
+__global__ void k_density(int32_t n, const int32_t *__restrict__ cX, const int32_t *__restrict__ Mat,
+                          const double *__restrict__ T, double *__restrict__ rho, double *__restrict__ rhodot,
+                          MatTab t) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  if (cX[m] == -1) return;
+  const int mm = Mat[m] & (FVM_MAXMAT - 1);
+  rho[m] = -t.rho[mm] * (t.alpha[mm] * T[m]);
+  rhodot[m] = 0.0;
+}
+
+extern "C" int fvm_update_marker_density(fvm_ctx *ctx, fvm_markers *mk, const Materials *mat) {
+  if (!ctx || !mk || !mat) return FVM_ERR_ARG;
+  if (mk->nMark == 0) return FVM_OK;
+  MatTab t;
+  for (int i = 0; i < FVM_MAXMAT; i++) {
+    t.rho[i] = mat->materialRho[i];
+    t.alpha[i] = mat->materialAlpha[i];
+  }
+  {
+    FVM_PHASE(ctx, FVM_PHASE_MARKER_UPDATE);
+    k_density<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(mk->nMark, mk->cellX, mk->Mat, mk->T, mk->rho,
+                                                                     mk->rhodot, t);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  return FVM_OK;
+}
+
+// max of non-negative doubles via their ordered bit patterns (order-free, exact)
+__global__ void k_max_pos(int32_t n, const int32_t *__restrict__ cX, const double *__restrict__ v, int absval,
+                          unsigned long long *__restrict__ out) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double x = 0.0;
+  if (m < n && cX[m] != -1) {
+    x = absval ? fabs(v[m]) : v[m];
+    if (!(x > 0.0)) x = 0.0;
+  }
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  for (int off = 32; off > 0; off >>= 1) {
+    unsigned long long o = __shfl_xor(b, off);
+    b = o > b ? o : b;
+  }
+  if ((threadIdx.x & 63) == 0 && b) atomicMax(out, b);
+}
+
+static int max_pos(fvm_ctx *ctx, const fvm_markers *mk, const double *col, int absval, double *res) {
+  FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
+  unsigned long long *d = (unsigned long long *)((char *)ctx->red.ptr + 256);
+  FVM_HIP(ctx, hipMemsetAsync(d, 0, 8, ctx->stream));
+  if (mk->nMark > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_OTHER);
+      k_max_pos<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(mk->nMark, mk->cellX, col, absval, d);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  unsigned long long *h = (unsigned long long *)ctx->pinned.ptr + 16;
+  FVM_HIP(ctx, hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  double r;
+  memcpy(&r, h, 8);
+  *res = r;
+  return FVM_OK;
+}
+
+extern "C" int fvm_max_marker_T(fvm_ctx *ctx, const fvm_markers *mk, double *maxT) {
+  if (!ctx || !mk || !maxT) return FVM_ERR_ARG;
+  return max_pos(ctx, mk, mk->T, 0, maxT);
+}
+extern "C" int fvm_max_abs_sxx(fvm_ctx *ctx, const fvm_markers *mk, double *sxxmax) {
+  if (!ctx || !mk || !sxxmax) return FVM_ERR_ARG;
+  return max_pos(ctx, mk, mk->sxx, 1, sxxmax);
+}

@@ -1,0 +1,288 @@
+// Marker cell location (findMarkerCells) and the deterministic bucket order
+// used by every projection.
+//
+// Bucket key of an active marker: ((cellY*(NX-1) + cellX) << 2) | bx | (by << 1)
+// with bx = X >= xc[cellX+1], by = Y >= yc[cellY+1] -- the quadrant tests of
+// findCellStag2D (markerProjection.c:258-268).  For every staggered node family
+// all markers of one bucket fall into the same family cell, so a family cell is
+// the union of at most four buckets.  Buckets are filled with integer atomics
+// and then each bucket is put in ascending marker-index order, so the final
+// order is a pure function of the keys (identical to a stable sort by key).
+#include "fvm_internal.h"
+
+// ---- findMarkerCell (markers.c:41-83) ---------------------------------------
+__device__ __forceinline__ int bisect(const double *__restrict__ x, int N, double X) {
+  int lo = 0, hi = N - 1;
+  while (hi - lo > 1) {
+    int mid = (hi + lo) / 2;
+    if (mid == lo) mid++;
+    else if (mid == hi) mid--;
+    if (X >= x[mid]) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool LDS>
+__global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double *__restrict__ X,
+                                                const double *__restrict__ Y, int32_t *__restrict__ cellX,
+                                                int32_t *__restrict__ cellY, int32_t *__restrict__ cpu, int32_t rank,
+                                                int32_t *__restrict__ counters) {
+  extern __shared__ double sgrid[];
+  const double *gx = g.x, *gy = g.y;
+  if (LDS) {
+    for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
+    for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
+    __syncthreads();
+    gx = sgrid;
+    gy = sgrid + g.NX;
+  }
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int out = 0, bad = 0;
+  if (m < n) {
+    const double xm = X[m], ym = Y[m];
+    int cx = -1, cy = -1;
+    if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
+      cx = bisect(gx, g.NX, xm);
+      cy = bisect(gy, g.NY, ym);
+      if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) bad = 1;
+    } else {
+      out = 1;
+    }
+    cellX[m] = cx;
+    cellY[m] = cy;
+    cpu[m] = rank;
+  }
+  // block-wide counts of outside markers and bad cells (integer: order-free)
+  __shared__ int s_out, s_bad;
+  if (threadIdx.x == 0) {
+    s_out = 0;
+    s_bad = 0;
+  }
+  __syncthreads();
+  unsigned long long bo = __ballot(out), bb = __ballot(bad);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&s_out, __popcll(bo));
+    atomicAdd(&s_bad, __popcll(bb));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_out) atomicAdd(&counters[0], s_out);
+    if (s_bad) atomicAdd(&counters[1], s_bad);
+  }
+}
+
+static int launch_locate(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank, int32_t *counters) {
+  const DGrid dg = dgrid(g);
+  const int n = mk->nMark;
+  if (n == 0) return FVM_OK;
+  const size_t lds = (size_t)(g->NX + g->NY) * sizeof(double);
+  if (lds <= 48 * 1024) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_LOCATE);
+      k_locate<true><<<fvm_blocks(n, 256), 256, lds, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
+                                                                     mk->cpu, rank, counters);
+    }
+  } else {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_LOCATE);
+      k_locate<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
+                                                                    mk->cpu, rank, counters);
+    }
+  }
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank) {
+  FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
+  int32_t *counters = (int32_t *)ctx->red.ptr;
+  FVM_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(int32_t), ctx->stream));
+  FVM_TRY(launch_locate(ctx, g, mk, rank, counters));
+  fvm_markers_moved(ctx);
+  return FVM_OK;
+}
+
+extern "C" int fvm_find_marker_cells(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank,
+                                     int32_t *nOut) {
+  if (!ctx || !mk) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_locate_async(ctx, g, mk, rank));
+  int32_t *counters = (int32_t *)ctx->red.ptr;
+  int32_t *h = (int32_t *)ctx->pinned.ptr;
+  FVM_HIP(ctx, hipMemcpyAsync(h, counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nOut) *nOut = h[0];
+  if (h[1]) return fvm_set_err(ctx, FVM_ERR_CELL_OUT_OF_BOUNDS, "%d markers located outside the cell range", h[1]);
+  return FVM_OK;
+}
+
+// ---- bucket order --------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *__restrict__ X,
+                                              const double *__restrict__ Y, const int32_t *__restrict__ cellX,
+                                              const int32_t *__restrict__ cellY, int32_t *__restrict__ key,
+                                              int32_t *__restrict__ slot, int32_t *__restrict__ count) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int cx = cellX[m], cy = cellY[m];
+  if (!cell_ok(cx, cy, g.NX, g.NY)) {  // inactive (or invalid) markers take no part
+    key[m] = -1;
+    return;
+  }
+  const int bx = X[m] >= g.xc[cx + 1] ? 1 : 0;
+  const int by = Y[m] >= g.yc[cy + 1] ? 1 : 0;
+  const int k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+  key[m] = k;
+  slot[m] = atomicAdd(&count[k], 1);
+}
+
+// exclusive scan of count[0..nk) into start[0..nk], start[nk] = total
+#define SCAN_BS 256
+#define SCAN_IPT 8
+#define SCAN_TILE (SCAN_BS * SCAN_IPT)
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_tiles(const int32_t *__restrict__ in, int32_t *__restrict__ out,
+                                                        int32_t *__restrict__ tile_sums, int64_t n) {
+  __shared__ int32_t s[SCAN_BS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+  int32_t v[SCAN_IPT];
+  int32_t t = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    t += v[i];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BS; off <<= 1) {
+    int32_t a = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int32_t run = s[threadIdx.x] - t;  // exclusive prefix of this thread
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; i++) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == SCAN_BS - 1) tile_sums[blockIdx.x] = s[SCAN_BS - 1];
+}
+
+__global__ void __launch_bounds__(1024) k_scan_sums(int32_t *__restrict__ sums, int64_t n, int32_t *__restrict__ total) {
+  __shared__ int32_t s[1024];
+  __shared__ int32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < n; base += 1024) {
+    const int64_t i = base + threadIdx.x;
+    const int32_t v = i < n ? sums[i] : 0;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      int32_t a = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+      __syncthreads();
+      s[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (i < n) sums[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += s[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_add(int32_t *__restrict__ out, const int32_t *__restrict__ tile_sums,
+                                                      int64_t n) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  const int32_t add = tile_sums[blockIdx.x];
+  for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_BS)
+    if (base + i < n) out[base + i] += add;
+}
+
+__global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__restrict__ key,
+                                                 const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
+                                                 int32_t *__restrict__ perm) {
+  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int k = key[m];
+  if (k < 0) return;
+  perm[start[k] + slot[m]] = (int32_t)m;
+}
+
+// put each bucket in ascending marker order (buckets hold ~NMX*NMY/4 markers)
+__global__ void __launch_bounds__(256) k_bucket_order(int32_t nk, const int32_t *__restrict__ start,
+                                                      int32_t *__restrict__ perm) {
+  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nk) return;
+  const int s0 = start[b], s1 = start[b + 1];
+  for (int i = s0 + 1; i < s1; i++) {
+    const int32_t v = perm[i];
+    int j = i - 1;
+    while (j >= s0 && perm[j] > v) {
+      perm[j + 1] = perm[j];
+      j--;
+    }
+    perm[j + 1] = v;
+  }
+}
+
+int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
+  if (ctx->sort_epoch == ctx->epoch && ctx->sort_x == (const void *)g->x && ctx->sort_X == (const void *)mk->X &&
+      ctx->sort_n == mk->nMark)
+    return FVM_OK;
+  const int32_t n = mk->nMark;
+  const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+  const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
+  FVM_TRY(fvm_ws(ctx, ctx->key, (size_t)(n > 0 ? n : 1) * 4));
+  FVM_TRY(fvm_ws(ctx, ctx->slot, (size_t)(n > 0 ? n : 1) * 4));
+  FVM_TRY(fvm_ws(ctx, ctx->perm, (size_t)(n > 0 ? n : 1) * 4));
+  FVM_TRY(fvm_ws(ctx, ctx->count, (size_t)(nk + 1) * 4));
+  FVM_TRY(fvm_ws(ctx, ctx->start, (size_t)(nk + 1) * 4));
+  FVM_TRY(fvm_ws(ctx, ctx->blocksums, (size_t)(ntiles + 1) * 4));
+  int32_t *key = (int32_t *)ctx->key.ptr, *slot = (int32_t *)ctx->slot.ptr, *perm = (int32_t *)ctx->perm.ptr;
+  int32_t *count = (int32_t *)ctx->count.ptr, *start = (int32_t *)ctx->start.ptr;
+  int32_t *sums = (int32_t *)ctx->blocksums.ptr;
+  FVM_HIP(ctx, hipMemsetAsync(count, 0, (size_t)(nk + 1) * 4, ctx->stream));
+  const DGrid dg = dgrid(g);
+  if (n > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_SORT);
+      k_keys<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY, key, slot, count);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    k_scan_tiles<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(count, start, sums, nk);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    k_scan_sums<<<1, 1024, 0, ctx->stream>>>(sums, ntiles, start + nk);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    k_scan_add<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(start, sums, nk);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  if (n > 0) {
+    {
+      FVM_PHASE(ctx, FVM_PHASE_SORT);
+      k_scatter<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, slot, start, perm);
+      FVM_LAUNCH_CHECK(ctx);
+    }
+  }
+  {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    k_bucket_order<<<fvm_blocks(nk, 256), 256, 0, ctx->stream>>>((int32_t)nk, start, perm);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  ctx->sort_epoch = ctx->epoch;
+  ctx->sort_x = g->x;
+  ctx->sort_X = mk->X;
+  ctx->sort_n = n;
+  return FVM_OK;
+}

@@ -1,0 +1,183 @@
+"""Host-side mirror of the reference's marker-path interface over the HIP library.
+
+``MarkerPath`` holds one problem instance on the GPU (grid, SoA markers, nodal
+fields) and exposes the reference's functions under their own names
+(findMarkerCells -> find_marker_cells, ...).  Every method is one call into
+libfvmic2d_hip.so; nothing here computes on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .lib import (Context, DeviceGrid, DeviceMarkers, DeviceNodal, fvm_projection, I32, P)
+
+VEP_PROJ = ["proj_rho", "proj_etaN", "proj_etaS", "proj_kThermal", "proj_lastT", "proj_Cp",
+            "proj_soxx", "proj_soyy", "proj_soxy", "proj_rhodot", "proj_muS", "proj_muN"]
+
+
+def _mat_table(materials, name):
+    return (C.c_double * abi.MAXMAT)(*getattr(materials, "material" + name))
+
+
+class MarkerPath:
+    def __init__(self, w, ctx=None, extra=(), capacity=None):
+        self.w = w
+        self.ctx = ctx or Context()
+        self.grid = DeviceGrid(self.ctx, w)
+        self.markers = DeviceMarkers(self.ctx, capacity or max(1, w.n_markers))
+        self.markers.upload(w.markers)
+        names = list(abi.NODAL_NAMES) + list(extra)
+        if isinstance(w.projections, list):
+            names += [p[4] for p in w.projections]
+        if w.kind == "vep":
+            names += VEP_PROJ
+        seen = []
+        for n in names:
+            if n not in seen:
+                seen.append(n)
+        self.nodal = DeviceNodal(self.ctx, w.NX, w.NY, seen, init=w.nodal)
+        self._tables = []
+        self.proj_list = None
+        if isinstance(w.projections, list):
+            self.proj_list = self.projection_list(w.projections)
+
+    # helpers --------------------------------------------------------------
+    def projection_list(self, entries):
+        arr = (fvm_projection * len(entries))()
+        for i, (src, sx, sy, meth, out) in enumerate(entries):
+            e = arr[i]
+            if src.startswith("mat:"):
+                t = _mat_table(self.w.materials, src[4:])
+                self._tables.append(t)
+                e.materialProperty = C.cast(t, P)
+            else:
+                e.markerField = self.markers.col(src)
+            e.nodalField = self.nodal.ptr(out)
+            e.stagx, e.stagy, e.method = sx, sy, meth
+        return arr
+
+    def get_markers(self):
+        return self.markers.download()
+
+    def get_nodal(self, name):
+        return self.nodal.get(name)
+
+    def set_nodal(self, name, arr):
+        self.nodal[name].set(arr)
+
+    def call(self, name, *args):
+        self.ctx.call(name, *args)
+
+    # reference-named calls -----------------------------------------------
+    def find_marker_cells(self, rank=0):
+        nout = I32()
+        self.call("fvm_find_marker_cells", self.grid.ref, self.markers.ref, I32(rank), C.byref(nout))
+        return nout.value
+
+    def project_marker_fields(self, entries):
+        arr = self.projection_list(entries)
+        self.call("fvm_project_marker_fields", self.grid.ref, self.markers.ref, I32(len(entries)),
+                  C.cast(arr, P))
+
+    def project_marker_field(self, src, out, stagx, stagy, method):
+        self.project_marker_fields([(src, stagx, stagy, method, out)])
+
+    def project_markers_nodes_all2(self, nf=None):
+        self.call("fvm_project_markers_nodes_all2", self.grid.ref, self.markers.ref,
+                  C.byref(nf) if nf is not None else self.nodal.ref, C.byref(self.w.materials))
+
+    def project_from_scalar(self, field_ptr, out):
+        self.call("fvm_project_markers_nodes_from_scalar", self.grid.ref, self.markers.ref,
+                  P(field_ptr), P(self.nodal.ptr(out)))
+
+    def project_nodal_to_markers(self, name, field):
+        self.call("fvm_project_nodal_field_to_markers", self.grid.ref, P(self.nodal.ptr(name)),
+                  self.markers.ref, P(self.markers.col(field)))
+
+    def advect(self, dt, jaumann=0, nf=None):
+        self.call("fvm_advect_markers_rk", self.markers.ref,
+                  C.byref(nf) if nf is not None else self.nodal.ref, self.grid.ref,
+                  C.byref(self.w.bv), dt, I32(jaumann))
+
+    def subgrid_temperature(self, dt, dsubgrid, thisT="thisT", nf=None):
+        self.call("fvm_subgrid_temperature_changes", P(self.nodal.ptr(thisT)), self.grid.ref,
+                  C.byref(nf) if nf is not None else self.nodal.ref, self.markers.ref,
+                  C.byref(self.w.materials), dt, dsubgrid)
+
+    def subgrid_stress(self, dt, dsubgrid):
+        self.call("fvm_subgrid_stress_changes", self.grid.ref, self.nodal.ref, self.markers.ref,
+                  C.byref(self.w.materials), dt, dsubgrid)
+
+    def update_marker_stress(self):
+        self.call("fvm_update_marker_stress", self.grid.ref, self.nodal.ref, self.markers.ref,
+                  C.byref(self.w.materials))
+
+    def update_marker_strain_pressure(self, dt):
+        self.call("fvm_update_marker_strain_pressure", self.grid.ref, self.nodal.ref,
+                  self.markers.ref, C.byref(self.w.materials), C.byref(self.w.options), dt)
+
+    def check_plastic_yielding(self, dt):
+        y = I32(0)
+        self.call("fvm_check_plastic_yielding", self.grid.ref, self.markers.ref,
+                  C.byref(self.w.materials), dt, C.byref(y), C.byref(self.w.options), 0.0)
+        return y.value
+
+    def limit_viscosity(self):
+        self.call("fvm_limit_viscosity", self.markers.ref, C.byref(self.w.options))
+
+    def strain_rate_residual(self):
+        r = C.c_double()
+        self.call("fvm_update_global_strain_rate_residual", self.grid.ref, self.markers.ref,
+                  C.byref(r))
+        return r.value
+
+    def update_marker_density(self):
+        self.call("fvm_update_marker_density", self.markers.ref, C.byref(self.w.materials))
+
+    def max_marker_T(self):
+        r = C.c_double()
+        self.call("fvm_max_marker_T", self.markers.ref, C.byref(r))
+        return r.value
+
+    def max_abs_sxx(self):
+        r = C.c_double()
+        self.call("fvm_max_abs_sxx", self.markers.ref, C.byref(r))
+        return r.value
+
+    # whole steps ------------------------------------------------------------
+    def thermal_view(self):
+        if self.proj_list is None:
+            return self.nodal.desc
+        return self.nodal.view(lastT="T_b", rho="rho_b", Cp="Cp_b", kThermal="k_b")
+
+    def step_thermal(self):
+        """One marker step (locate, density, projections, subgrid T, RK4), async."""
+        w = self.w
+        nf = self.thermal_view()
+        if self.proj_list is None:
+            self.call("fvm_step_thermal", self.grid.ref, self.markers.ref, C.byref(nf),
+                      C.byref(w.materials), C.byref(w.options), C.byref(w.bv), w.dt, I32(0), None)
+        else:
+            self.call("fvm_step_thermal", self.grid.ref, self.markers.ref, C.byref(nf),
+                      C.byref(w.materials), C.byref(w.options), C.byref(w.bv), w.dt,
+                      I32(len(self.proj_list)), C.cast(self.proj_list, P))
+
+    def vep_views(self):
+        proj = self.nodal.view(rho="proj_rho", etaN="proj_etaN", etaS="proj_etaS",
+                               kThermal="proj_kThermal", lastT="proj_lastT", Cp="proj_Cp",
+                               soxx="proj_soxx", soyy="proj_soyy", soxy="proj_soxy",
+                               rhodot="proj_rhodot", muS="proj_muS", muN="proj_muN")
+        return proj, self.nodal.desc
+
+    def step_vep(self):
+        w = self.w
+        proj, host = self.vep_views()
+        y, r = I32(0), C.c_double()
+        self.call("fvm_step_vep", self.grid.ref, self.markers.ref, C.byref(proj), C.byref(host),
+                  C.byref(w.materials), C.byref(w.options), C.byref(w.bv), w.dt, C.byref(y),
+                  C.byref(r))
+        return y.value, r.value
+
+    def step(self):
+        return self.step_vep() if self.w.kind == "vep" else self.step_thermal()

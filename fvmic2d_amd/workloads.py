@@ -1,0 +1,505 @@
+"""Seeded synthetic inputs for the marker hot path.
+
+Five workloads, each named by what it is:
+
+* ``convection_small``   51x51 uniform grid, 4x4 markers per cell (~40k markers),
+  isoviscous cellular flow; step = locate, density, projectMarkersNodesAll2,
+  subgrid temperature, RK4 advection.
+* ``convection_tdep``    201x201 uniform grid, 6x6 markers per cell (1.44M),
+  eta = exp(-ln(1e5) T); T, eta, rho, k projected onto all four staggered node
+  families (+ Cp on basic nodes); same step.
+* ``vep_subduction``     801x401 tanh-stretched grid (4x refinement in a central
+  band), 5x5 markers per cell (8M), 5 materials, Drucker-Prager yield; full
+  visco-elasto-plastic step.
+* ``ensemble``           N instances of the convection_tdep generator with random
+  sizes 65..257 and 4..6 markers per cell per axis.
+
+Random numbers come from a counter-based SplitMix64 stream, so every array is a
+pure function of (seed, stream id, index) and is identical on every machine.
+Marker order follows distributeMarkersUniformInDomain (markers.c:382-395): the
+sub-lattice column index k is outer, the row index l inner.
+"""
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+from . import abi
+
+_GOLD = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _splitmix64(x):
+    with np.errstate(over="ignore"):
+        z = x + _GOLD
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+class Rng:
+    """Counter-based stream: draw(n) returns the next n values of stream ``sid``."""
+
+    def __init__(self, seed, sid):
+        with np.errstate(over="ignore"):
+            self.key = _splitmix64(np.uint64(seed) * np.uint64(1_000_003) + np.uint64(sid))
+        self.pos = 0
+
+    def u64(self, n):
+        c = np.arange(self.pos, self.pos + n, dtype=np.uint64)
+        self.pos += n
+        with np.errstate(over="ignore"):
+            return _splitmix64(c ^ self.key)
+
+    def uniform(self, n, lo=0.0, hi=1.0):
+        u = (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+        return lo + (hi - lo) * u
+
+    def normal(self, n):
+        u1 = self.uniform(n)
+        u2 = self.uniform(n)
+        return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+# ---------------------------------------------------------------- grids ----
+def spacing_uniform(N, L):
+    """gridSpacing.c:8-17: x[i] = x[i-1] + dx, x[N-1] = L."""
+    dx = L / (float(N) - 1.0)
+    x = np.empty(N)
+    x[0] = 0.0
+    acc = 0.0
+    for i in range(1, N - 1):
+        acc = acc + dx
+        x[i] = acc
+    x[N - 1] = L
+    return x
+
+
+def spacing_tanh_band(N, L, refine=4.0, lo=0.35, hi=0.65, width=0.04):
+    """Monotone spacing refined `refine`x inside [lo, hi]*L with tanh transitions."""
+    s = (np.arange(N - 1) + 0.5) / (N - 1)
+    band = 0.5 * (np.tanh((s - lo) / width) - np.tanh((s - hi) / width))
+    h = 1.0 / (1.0 + (refine - 1.0) * band)
+    # widths in a cell-count parametrisation: more cells where h is small
+    cum = np.concatenate([[0.0], np.cumsum(h)])
+    x = L * cum / cum[-1]
+    x[0] = 0.0
+    x[-1] = L
+    return x
+
+
+def pad_grid(x, L):
+    """Padded node and centre arrays (length N+4, element 2 is index 0).
+
+    Centres follow getCellCenters (gridSpacing.c:150-157); the two ghost slots on
+    each side are filled by linear extrapolation (the reference leaves them unset
+    and never reads them on a non-periodic grid)."""
+    N = len(x)
+    xp = np.empty(N + 4)
+    xp[2:N + 2] = x
+    h0, h1 = x[1] - x[0], x[N - 1] - x[N - 2]
+    xp[1], xp[0] = x[0] - h0, x[0] - 2.0 * h0
+    xp[N + 2], xp[N + 3] = x[N - 1] + h1, x[N - 1] + 2.0 * h1
+    xc = np.empty(N + 1)
+    xc[1:N] = (x[0:N - 1] + x[1:N]) / 2.0
+    xc[0] = -xc[1]
+    xc[N] = 2.0 * L - xc[N - 1]
+    xcp = np.empty(N + 4)
+    xcp[2:N + 3] = xc
+    xcp[1], xcp[0] = xc[0] - (xc[1] - xc[0]), xc[0] - 2.0 * (xc[1] - xc[0])
+    xcp[N + 3] = xc[N] + (xc[N] - xc[N - 1])
+    return xp, xcp
+
+
+# ------------------------------------------------------------ workload ----
+@dataclass
+class Workload:
+    name: str
+    NX: int
+    NY: int
+    LX: float
+    LY: float
+    xp: np.ndarray            # padded node coordinates (NX+4)
+    yp: np.ndarray
+    xcp: np.ndarray           # padded centre coordinates (NX+4)
+    ycp: np.ndarray
+    markers: np.ndarray       # MARKER_DTYPE, AoS 216 B
+    nodal: dict               # name -> (NY, NX) float64 inputs
+    materials: object
+    options: object
+    bv: object
+    dt: float
+    kind: str                 # "plain" or "vep"
+    projections: list = field(default_factory=list)  # (src, stagx, stagy, method, out)
+    seed: int = 0
+
+    @property
+    def x(self):
+        return self.xp[2:2 + self.NX]
+
+    @property
+    def y(self):
+        return self.yp[2:2 + self.NY]
+
+    @property
+    def xc(self):
+        return self.xcp[2:2 + self.NX + 1]
+
+    @property
+    def yc(self):
+        return self.ycp[2:2 + self.NY + 1]
+
+    @property
+    def n_markers(self):
+        return len(self.markers)
+
+
+def _bc_free_slip():
+    bv = abi.BoundaryValues()
+    for side in ("mechBCLeft", "mechBCRight", "mechBCTop", "mechBCBottom"):
+        b = getattr(bv, side)
+        for k in range(3):
+            b.type[k] = 1
+            b.value[k] = 0.0
+    return bv
+
+
+def _markers_in_domain(rng, x, y, NX, NY, NMX, NMY, LX, LY):
+    """distributeMarkersUniformInDomain (markers.c:342-400) with a counter RNG."""
+    mdx = LX / (NMX * (NX - 1))
+    mdy = LY / (NMY * (NY - 1))
+    nk = NMX * (NX - 1)
+    nl = NMY * (NY - 1)
+    k = np.repeat(np.arange(nk), nl)
+    l = np.tile(np.arange(nl), nk)
+    jx = rng.uniform(nk * nl, -mdx / 4.0, mdx / 4.0)
+    jy = rng.uniform(nk * nl, -mdy / 4.0, mdy / 4.0)
+    X = x[0] + mdx * (0.5 + k) + jx
+    Y = y[0] + mdy * (0.5 + l) + jy
+    return X, Y
+
+
+def _markers_in_cells(rng, x, y, NMX, NMY):
+    """NMX x NMY jittered sub-lattice in every cell of a nonuniform grid
+    (the placement of distributeMarkersUniformInCells, markers.c:404-452)."""
+    NX, NY = len(x), len(y)
+    cx = np.repeat(np.arange(NX - 1), NMX)   # sub-column -> cell
+    ix = np.tile(np.arange(NMX), NX - 1)
+    cy = np.repeat(np.arange(NY - 1), NMY)
+    iy = np.tile(np.arange(NMY), NY - 1)
+    dxs = (x[cx + 1] - x[cx]) / NMX
+    dys = (y[cy + 1] - y[cy]) / NMY
+    colX = x[cx] + dxs * (0.5 + ix)
+    rowY = y[cy] + dys * (0.5 + iy)
+    nk, nl = len(colX), len(rowY)
+    X = np.repeat(colX, nl) + rng.uniform(nk * nl, -0.25, 0.25) * np.repeat(dxs, nl)
+    Y = np.tile(rowY, nk) + rng.uniform(nk * nl, -0.25, 0.25) * np.tile(dys, nk)
+    return X, Y
+
+
+def _new_markers(n):
+    mk = np.zeros(n, dtype=abi.MARKER_DTYPE)
+    mk["cellX"] = -1
+    mk["cellY"] = -1
+    return mk
+
+
+def discrete_vorticity(vx, vy, xc, yc, bv):
+    """wxy and exy at basic nodes (nodalStressStrain.c:157-236), ghost values by BC."""
+    NY, NX = vx.shape
+    vxm = np.empty_like(vx)
+    vym = np.empty_like(vy)
+    vym[:, 1:] = vy[:, :-1]
+    if bv.mechBCLeft.type[1] == 0:
+        vym[:, 0] = 2.0 * bv.mechBCLeft.value[1] - vy[:, 0]
+    else:
+        vym[:, 0] = vy[:, 0]
+    vxm[1:, :] = vx[:-1, :]
+    if bv.mechBCTop.type[0] == 0:
+        vxm[0, :] = 2.0 * bv.mechBCTop.value[0] - vx[0, :]
+    else:
+        vxm[0, :] = vx[0, :]
+    dx = (xc[1:NX + 1] - xc[0:NX])[None, :]
+    dy = (yc[1:NY + 1] - yc[0:NY])[:, None]
+    wxy = 0.5 * ((vx - vxm) / dy - (vy - vym) / dx)
+    exy = 0.5 * ((vx - vxm) / dy + (vy - vym) / dx)
+    return wxy, exy
+
+
+def cell_strain_rates(vx, vy, x, y):
+    """exx, eyy at cell centres (nodalStressStrain.c:55-75); row/col 0 are ghosts = 0."""
+    NY, NX = vx.shape
+    exx = np.zeros_like(vx)
+    eyy = np.zeros_like(vx)
+    dx = (x[1:] - x[:-1])[None, :]
+    dy = (y[1:] - y[:-1])[:, None]
+    dvx = (vx[:-1, 1:] - vx[:-1, :-1]) / dx
+    dvy = (vy[1:, :-1] - vy[:-1, :-1]) / dy
+    divv = dvx + dvy
+    exx[1:, 1:] = dvx - divv / 3.0
+    eyy[1:, 1:] = dvy - divv / 3.0
+    return exx, eyy
+
+
+def _cellular(w, A):
+    """Velocity of the cellular flow vx = A sin(pi x) cos(pi y), vy = -A cos(pi x) sin(pi y)
+    sampled at the staggered vx nodes (x[ix], yc[jy+1]) and vy nodes (xc[ix+1], y[jy])."""
+    X = w.x[None, :] / w.LX
+    Yc = w.yc[1:w.NY + 1][:, None] / w.LY
+    vx = A * np.sin(np.pi * X) * np.cos(np.pi * Yc)
+    Xc = w.xc[1:w.NX + 1][None, :] / w.LX
+    Y = w.y[:, None] / w.LY
+    vy = -A * np.cos(np.pi * Xc) * np.sin(np.pi * Y)
+    return vx, vy
+
+
+def convection_small(seed=1):
+    """51x51, 4x4 markers/cell, T = 1-y+0.01 cos(pi x) sin(pi y), isoviscous."""
+    NX = NY = 51
+    LX = LY = 1.0
+    x = spacing_uniform(NX, LX)
+    y = spacing_uniform(NY, LY)
+    xp, xcp = pad_grid(x, LX)
+    yp, ycp = pad_grid(y, LY)
+    rng = Rng(seed, 1)
+    X, Y = _markers_in_domain(rng, x, y, NX, NY, 4, 4, LX, LY)
+    mk = _new_markers(len(X))
+    mk["X"], mk["Y"] = X, Y
+    mk["Mat"] = 0
+    mk["T"] = 1.0 - Y + 0.01 * np.cos(np.pi * X) * np.sin(np.pi * Y)
+    mk["eta"] = 1.0
+    mk["mu"] = 1e99
+    mat = abi.default_materials()
+    mat.materialEta[0] = 1.0
+    mat.materialRho[0] = 1.0
+    mat.materialAlpha[0] = 1e-3
+    mat.materialkThermal[0] = 1.0
+    mat.materialCp[0] = 1.0
+    mat.materialMu[0] = 1e99
+    opt = abi.default_options()
+    opt.NX, opt.NY, opt.LX, opt.LY, opt.NMX, opt.NMY = NX, NY, LX, LY, 4, 4
+    bv = _bc_free_slip()
+    w = Workload("convection_small", NX, NY, LX, LY, xp, yp, xcp, ycp, mk, {}, mat, opt, bv,
+                 0.0, "plain", seed=seed)
+    vx, vy = _cellular(w, 1.0)
+    wxy, _ = discrete_vorticity(vx, vy, w.xc, w.yc, bv)
+    Xn, Yn = np.meshgrid(x, y)
+    Tn = 1.0 - Yn + 0.01 * np.cos(np.pi * Xn) * np.sin(np.pi * Yn)
+    w.nodal = {"vx": vx, "vy": vy, "wxy": wxy,
+               "thisT": Tn + 1e-3 * np.sin(np.pi * Xn) * np.sin(np.pi * Yn)}
+    w.dt = 0.5 * (LX / (NX - 1)) / 1.0
+    w.projections = "all2"
+    return w
+
+
+def _erf(v):
+    try:
+        from scipy.special import erf
+        return erf(v)
+    except ImportError:  # pragma: no cover
+        return np.vectorize(math.erf)(v)
+
+
+def _bl_profile(y, delta=0.05):
+    erf = _erf
+    return 0.5 * erf(y / delta) + 0.5 * (1.0 + erf((y - 1.0) / delta))
+
+
+def _tdep_projections():
+    out = []
+    fam = {(0, 0): "b", (-1, -1): "c", (0, 1): "vx", (1, 0): "vy"}
+    for (sx, sy), tag in fam.items():
+        out.append(("T", sx, sy, abi.ARITHMETIC, "T_" + tag))
+        out.append(("eta", sx, sy, abi.ARITHMETIC, "eta_" + tag))
+        out.append(("rho", sx, sy, abi.ARITHMETIC, "rho_" + tag))
+        out.append(("mat:kThermal", sx, sy, abi.ARITHMETIC, "k_" + tag))
+    out.append(("mat:Cp", 0, 0, abi.ARITHMETIC, "Cp_b"))
+    return out
+
+
+def convection_tdep(seed=2, N=201, NM=6, contrast=1e5, name="convection_tdep"):
+    """N x N uniform grid, NM x NM markers/cell, eta = exp(-ln(contrast) T)."""
+    NX = NY = N
+    LX = LY = 1.0
+    x = spacing_uniform(NX, LX)
+    y = spacing_uniform(NY, LY)
+    xp, xcp = pad_grid(x, LX)
+    yp, ycp = pad_grid(y, LY)
+    rng = Rng(seed, 1)
+    X, Y = _markers_in_domain(rng, x, y, NX, NY, NM, NM, LX, LY)
+    mk = _new_markers(len(X))
+    mk["X"], mk["Y"] = X, Y
+    mk["Mat"] = 0
+    T = _bl_profile(Y) * (1.0 + 0.02 * Rng(seed, 2).normal(len(X)))
+    mk["T"] = T
+    qonr = math.log(contrast)
+    mk["eta"] = np.exp(-qonr * T)
+    mk["mu"] = 1e99
+    mat = abi.default_materials()
+    mat.materialEta[0] = 1.0
+    mat.materialRho[0] = 1.0
+    mat.materialAlpha[0] = 1e-3
+    mat.materialkThermal[0] = 1.0
+    mat.materialCp[0] = 1.0
+    mat.materialMu[0] = 1e99
+    mat.hasEtaT[0] = 5
+    mat.QonR[0] = qonr
+    opt = abi.default_options()
+    opt.NX, opt.NY, opt.LX, opt.LY, opt.NMX, opt.NMY = NX, NY, LX, LY, NM, NM
+    bv = _bc_free_slip()
+    A = float(Rng(seed, 3).uniform(1, 0.5, 2.0)[0])
+    w = Workload(name, NX, NY, LX, LY, xp, yp, xcp, ycp, mk, {}, mat, opt, bv, 0.0, "plain",
+                 seed=seed)
+    vx, vy = _cellular(w, A)
+    wxy, _ = discrete_vorticity(vx, vy, w.xc, w.yc, bv)
+    Xn, Yn = np.meshgrid(x, y)
+    w.nodal = {"vx": vx, "vy": vy, "wxy": wxy,
+               "thisT": _bl_profile(Yn) + 1e-3 * np.sin(np.pi * Xn) * np.sin(np.pi * Yn)}
+    w.dt = 0.5 * (LX / (NX - 1)) / A
+    w.projections = _tdep_projections()
+    return w
+
+
+def ensemble(n_instances=64, seed=4, nmin=65, nmax=257, small=False):
+    """Independent convection_tdep instances with random sizes and marker densities."""
+    r = Rng(seed, 7)
+    out = []
+    for i in range(n_instances):
+        u = r.uniform(3)
+        N = int(nmin + min(int(u[0] * (nmax - nmin + 1)), nmax - nmin))
+        NM = 4 + min(int(u[1] * 3), 2)
+        contrast = 10.0 ** (2.0 + 4.0 * u[2])
+        out.append(convection_tdep(seed=1000 * seed + i, N=N, NM=NM, contrast=contrast,
+                                   name="ensemble_%d" % i))
+    return out
+
+
+def _streamfunction_velocity(w, rng, amp, nmodes=4):
+    """Divergence-free staggered velocity from a smooth random streamfunction psi
+    sampled at basic nodes: vx = dpsi/dy on vx nodes, vy = -dpsi/dx on vy nodes
+    (discrete divergence zero on every cell, nodalStressStrain.c:67)."""
+    x, y = w.x, w.y
+    a = rng.normal(nmodes * nmodes).reshape(nmodes, nmodes)
+    Xn, Yn = np.meshgrid(x / w.LX, y / w.LY)
+    psi = np.zeros_like(Xn)
+    for i in range(nmodes):
+        for j in range(nmodes):
+            psi += a[i, j] / (1.0 + i * i + j * j) * np.sin(np.pi * (i + 1) * Xn) * np.sin(np.pi * (j + 1) * Yn)
+    psi *= amp
+    NY, NX = psi.shape
+    vx = np.zeros((NY, NX))
+    vy = np.zeros((NY, NX))
+    dy = (y[1:] - y[:-1])[:, None]
+    dx = (x[1:] - x[:-1])[None, :]
+    vx[:-1, :] = (psi[1:, :] - psi[:-1, :]) / dy
+    vy[:, :-1] = -(psi[:, 1:] - psi[:, :-1]) / dx
+    return vx, vy
+
+
+def vep_subduction(seed=3, NX=801, NY=401, NM=5):
+    """Layered 5-material slab model on a tanh-stretched grid with Drucker-Prager yield."""
+    LX, LY = 2.0e5, 1.0e5
+    x = spacing_tanh_band(NX, LX)
+    y = spacing_tanh_band(NY, LY)
+    xp, xcp = pad_grid(x, LX)
+    yp, ycp = pad_grid(y, LY)
+    rng = Rng(seed, 1)
+    X, Y = _markers_in_cells(rng, x, y, NM, NM)
+    n = len(X)
+    mk = _new_markers(n)
+    mk["X"], mk["Y"] = X, Y
+    # layered materials with a dipping slab; interfaces perturbed by a random Fourier series
+    fr = Rng(seed, 2).normal(12)
+    pert = np.zeros(n)
+    for k in range(6):
+        pert += 1.5e3 * fr[k] / (1 + k) * np.sin((k + 1) * np.pi * X / LX + fr[6 + k])
+    d = Y + pert
+    matid = np.where(d < 1.0e4, 0, np.where(d < 3.0e4, 1, np.where(d < 6.0e4, 2, 3)))
+    slab = (np.abs((Y - (X - 0.6e5) * 0.8)) < 1.2e4) & (X > 0.6e5)
+    matid = np.where(slab, 4, matid)
+    mk["Mat"] = matid
+    T = 273.0 + 1300.0 * np.clip(Y / 1.0e5, 0, 1) * (1.0 + 0.01 * Rng(seed, 3).normal(n))
+    T = np.where(slab, T - 400.0, T)
+    mk["T"] = T
+    mat = abi.default_materials()
+    mat.nMaterials = 5
+    etas = [1e23, 1e22, 1e21, 1e20, 1e24]
+    for i in range(5):
+        mat.materialEta[i] = etas[i]
+        mat.materialRho[i] = 3300.0 + 20.0 * i
+        mat.materialkThermal[i] = 3.0
+        mat.materialCp[i] = 1000.0 + 50.0 * i
+        mat.materialAlpha[i] = 3e-5
+        mat.materialMu[i] = 1e10
+        mat.materialCohesion[i] = 1e7 * (1 + i)
+        mat.materialFriction[i] = 0.3 + 0.05 * i
+        mat.hasPlasticity[i] = 1
+        mat.hasEtaT[i] = 1 if i < 4 else 0
+        mat.QonR[i] = 2.0e4
+        mat.Tref[i] = 1573.0
+    # strain-weakening law on material 2
+    mat.hasPlasticity[2] = 2
+    mat.C[2][0], mat.C[2][1] = 2e7, 5e6
+    mat.F[2][0], mat.F[2][1] = 30.0, 10.0
+    mat.gamma[2][0], mat.gamma[2][1] = 0.1, 1.0
+    opt = abi.default_options()
+    opt.NX, opt.NY, opt.LX, opt.LY, opt.NMX, opt.NMY = NX, NY, LX, LY, NM, NM
+    opt.etamin, opt.etamax, opt.fractionalEtamin = 1e18, 1e25, 1e-4
+    opt.subgridStressDiffusivity = 1.0
+    opt.subgridTemperatureDiffusivity = 1.0
+    r4 = Rng(seed, 4)
+    eta0 = np.array(etas)[matid] * np.exp(0.1 * r4.normal(n))
+    mk["eta"] = eta0
+    mk["mu"] = 1e10
+    mk["sxx"] = 1e7 * r4.normal(n)
+    mk["syy"] = -mk["sxx"] + 1e6 * r4.normal(n)
+    mk["sxy"] = 5e6 * r4.normal(n)
+    mk["exx"] = 1e-15 * r4.normal(n)
+    mk["eyy"] = -mk["exx"]
+    mk["exy"] = 1e-15 * r4.normal(n)
+    mk["p"] = 3300.0 * 9.8 * Y * (1.0 + 0.05 * r4.normal(n))
+    mk["Eii"] = np.abs(r4.uniform(n, 0.0, 1.5))
+    bv = _bc_free_slip()
+    w = Workload("vep_subduction", NX, NY, LX, LY, xp, yp, xcp, ycp, mk, {}, mat, opt, bv, 0.0,
+                 "vep", seed=seed)
+    vel = 1e-9  # m/s
+    vx, vy = _streamfunction_velocity(w, Rng(seed, 5), vel * LY)
+    vmax = max(np.abs(vx).max(), np.abs(vy).max())
+    hmin = min(np.diff(x).min(), np.diff(y).min())
+    dt = 0.5 * hmin / vmax
+    wxy, exy = discrete_vorticity(vx, vy, w.xc, w.yc, bv)
+    exx, eyy = cell_strain_rates(vx, vy, x, y)
+    # smooth viscosities on both node families and the nodal stress update of
+    # nodalStressStrain.c:90-131, 256-271 with muN = muS = 1e99 (markerProjection.c:36-37)
+    Xn, Yn = np.meshgrid(x, y)
+    Xc, Yc = np.meshgrid(w.xc[0:NX], w.yc[0:NY])
+    etaS = 1e21 * np.exp(2.0 * np.sin(np.pi * Xn / LX) * np.cos(np.pi * Yn / LY))
+    etaN = 1e21 * np.exp(2.0 * np.sin(np.pi * Xc / LX) * np.cos(np.pi * Yc / LY))
+    mu = 1e99
+    r6 = Rng(seed, 6)
+    so_old = {k: 1e7 * r6.normal(NX * NY).reshape(NY, NX) for k in ("soxx", "soyy", "soxy")}
+    nod = {"vx": vx, "vy": vy, "wxy": wxy, "edotxy": exy, "edotxx": exx, "edotyy": eyy}
+    XN = etaN / (etaN + dt * mu)
+    X1N = etaN * (-2.0 * XN + 2.0)
+    XS = etaS / (etaS + dt * mu)
+    X1S = etaS * (-2.0 * XS + 2.0)
+    for comp, X_, X1_, e_ in (("xx", XN, X1N, exx), ("yy", XN, X1N, eyy), ("xy", XS, X1S, exy)):
+        snew = X_ * so_old["so" + comp] + X1_ * e_
+        nod["ds" + comp] = snew - so_old["so" + comp]
+        nod["so" + comp] = snew
+    nod["p"] = 3300.0 * 9.8 * Yc * (1.0 + 0.01 * r6.normal(NX * NY).reshape(NY, NX))
+    Tn = 273.0 + 1300.0 * np.clip(Yn / 1.0e5, 0, 1)
+    nod["thisT"] = Tn + 5.0 * np.sin(np.pi * Xn / LX) * np.sin(np.pi * Yn / LY)
+    w.nodal = nod
+    w.dt = dt
+    w.projections = "all2"
+    return w
+
+
+WORKLOADS = {
+    "convection_small": convection_small,
+    "convection_tdep": convection_tdep,
+    "vep_subduction": vep_subduction,
+}

@@ -1,0 +1,241 @@
+/*
+ * fvm_abi.h -- C ABI of the MI355X (gfx950) marker hot path.
+ *
+ * Every entry point takes plain pointers and sizes (no PETSc, no torch), returns
+ * an int status (FVM_OK = 0) and never aborts.  Marker state lives on the GPU as
+ * structure-of-arrays columns (fvm_markers), nodal fields as dense NX*NY device
+ * arrays in the reference's DMDA natural order ([jy][ix], ix fastest).  Grid
+ * coordinate arrays are device copies of the reference's padded arrays: each
+ * pointer addresses element 0 of an array that is valid for indices -2..N+1
+ * (gridGenerator.c:123-130).
+ *
+ * Work is enqueued on the context's HIP stream; a call returns once its kernels
+ * are enqueued unless it has to hand a scalar back (nOut, isYielding, residual),
+ * in which case it synchronises the stream.
+ *
+ * Semantics kept from the reference's PETSc layer: scatters to node indices
+ * outside the grid are dropped (DMLocalToGlobal ADD on DM_BOUNDARY_GHOSTED),
+ * gathers outside the grid read 0, normalisation is IEEE division (nodes with
+ * zero weight become NaN).  Only non-periodic grids are supported
+ * (xperiodic != 0 returns FVM_ERR_PERIODIC).
+ *
+ * Projections are deterministic: markers are bucketed by (basic cell, quadrant)
+ * into a sort order that is a pure function of the marker positions, and every
+ * nodal sum is formed in a fixed order, so repeated calls give bitwise
+ * identical nodal fields.
+ */
+#ifndef FVM_ABI_H
+#define FVM_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "structures.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+enum {
+  FVM_OK = 0,
+  FVM_ERR_PERIODIC = 1,          /* xperiodic grids are not supported */
+  FVM_ERR_CELL_OUT_OF_BOUNDS = 2,/* markers.c:74-77 (reference aborts) */
+  FVM_ERR_BAD_STAG = 3,          /* markerProjection.c:137-153 (reference aborts) */
+  FVM_ERR_BAD_METHOD = 4,        /* markerProjection.c:224-225, 241-243 (aborts) */
+  FVM_ERR_NAN_RESIDUAL = 5,      /* updateMarkerStrainPressure.c:302-306 (aborts) */
+  FVM_ERR_VISCOSITY_LAW = 6,     /* viscosity.c:60-62 (aborts) */
+  FVM_ERR_NOMEM = 7,
+  FVM_ERR_HIP = 8,               /* a HIP runtime call failed; see fvm_last_error */
+  FVM_ERR_ARG = 9,               /* invalid argument (NULL pointer, size) */
+  FVM_ERR_TOO_MANY_FIELDS = 10   /* more projected fields in one group than supported */
+};
+
+/* ---- device marker store: one device column per Marker field --------------- */
+typedef struct {
+  int32_t nMark;   /* markers visited: m < nMark (MarkerSet.nMark) */
+  int32_t maxMark; /* capacity of every column */
+  double *X, *Y, *VX, *VY, *T, *Tdot;
+  double *sxx, *syy, *sxy;            /* Marker.s.T11, T22, T12 */
+  double *eta, *mu, *rho, *rhodot, *p;
+  double *exx, *eyy, *exy;            /* Marker.e */
+  double *Exx, *Eyy, *Exy;            /* Marker.E */
+  double *Eii, *wxy, *strainRateResidual;
+  int32_t *Mat, *cellX, *cellY, *cpu, *isYielding;
+} fvm_markers;
+
+/* ---- device grid -------------------------------------------------------- */
+typedef struct {
+  int32_t NX, NY;
+  double LX, LY;
+  int32_t xperiodic;
+  const double *x, *y, *xc, *yc; /* device, valid for indices -2..N+1 */
+} fvm_grid;
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct fvm_ctx fvm_ctx;
+
+/* Create a context on the current device.  stream: a hipStream_t to enqueue on,
+ * or NULL for a stream owned by the context. */
+int fvm_ctx_create(fvm_ctx **ctx, void *stream);
+int fvm_ctx_destroy(fvm_ctx *ctx);
+const char *fvm_last_error(const fvm_ctx *ctx);
+void *fvm_ctx_stream(fvm_ctx *ctx);
+int fvm_ctx_sync(fvm_ctx *ctx);
+/* Bytes of device workspace the context currently holds (sort buffers, partials). */
+size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
+
+/* ---- device memory helpers (so a host program needs no other GPU API) ---- */
+int fvm_device_count(int *n);
+int fvm_malloc(void **ptr, size_t bytes);
+int fvm_free(void *ptr);
+int fvm_memcpy_htod(fvm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fvm_memcpy_dtoh(fvm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fvm_memcpy_dtod(fvm_ctx *ctx, void *dst, const void *src, size_t bytes);
+int fvm_memset(fvm_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* Allocate every column of a marker store in one device slab (freed by
+ * fvm_markers_free).  Columns are zero-filled, cellX/cellY = -1. */
+int fvm_markers_alloc(fvm_ctx *ctx, int32_t maxMark, fvm_markers *mk);
+int fvm_markers_free(fvm_markers *mk);
+/* Host AoS block (216-byte Marker, fdcode.h:97-129) <-> device columns.
+ * Upload sets mk->nMark = nMark. */
+int fvm_markers_upload_aos(fvm_ctx *ctx, fvm_markers *mk, const Marker *aos, int32_t nMark);
+int fvm_markers_download_aos(fvm_ctx *ctx, const fvm_markers *mk, Marker *aos);
+
+/* ---- A2: findMarkerCells (markers.c:85-106, findMarkerCell 41-83) ---------- */
+/* Writes cellX, cellY, cpu = rank; *nOut (may be NULL) receives the number of
+ * markers outside the domain. */
+int fvm_find_marker_cells(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank,
+                          int32_t *nOut);
+
+/* ---- A3/A4/A5: projection -------------------------------------------------- */
+/* One projection of markerProjection.c:71 projectMarkerFieldToNodes: a marker
+ * column (markerField, device) or a material property table (materialProperty,
+ * host array of FVM_MAXMAT, indexed by Mat) onto the node family (stagx,stagy)
+ * with the given PROJECTION_METHOD, result in nodalField (device, NX*NY). */
+typedef struct {
+  const double *markerField;      /* device column, or NULL */
+  const double *materialProperty; /* host table, or NULL */
+  double *nodalField;             /* device output */
+  int32_t stagx, stagy;
+  int32_t method;
+} fvm_projection;
+
+/* markerProjection.c:71-250 projectMarkerFieldToNodes: a list of projections
+ * sharing one marker sort.  Entries with the same (stagx, stagy, method) share
+ * one pass over the markers (up to 8 fields per group). */
+int fvm_project_marker_fields(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk,
+                              int32_t n, const fvm_projection *list);
+
+/* markerProjection.c:11-68 projectMarkersNodesAll2: rho, etaN, etaS, kThermal,
+ * lastT, Cp, soxx, soyy, soxy; sets rhodot = 0 and muS = muN = 1e99. */
+int fvm_project_markers_nodes_all2(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk,
+                                   const NodalArrays *nf, const Materials *mat);
+
+/* markers.c:1507-1594 projectMarkersNodesFromScalar (dense device marker array). */
+int fvm_project_markers_nodes_from_scalar(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk,
+                                          const double *markerField, double *nodalField);
+
+/* ---- A13: markers.c:1670-1706 projectNodalFieldToMarkersS ------------------ */
+int fvm_project_nodal_field_to_markers(fvm_ctx *ctx, const fvm_grid *g, const double *nodalField,
+                                       fvm_markers *mk, double *markerField);
+
+/* ---- A6: markers.c:1865-2250 advectMarkersRK -------------------------------- */
+/* Reads nf->vx, vy, wxy.  jaumann = 0: default exact stress rotation
+ * (markers.c:2157-2164); 1: the -DJAUMANN variant (markers.c:2147-2156). */
+int fvm_advect_markers_rk(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays *nf, const fvm_grid *g,
+                          const BoundaryValues *bv, double dt, int32_t jaumann);
+
+/* ---- A7: subgridTemperatureChanges.c:5-124 --------------------------------- */
+/* Reads nf->lastT, rho, Cp, kThermal; thisT is the host thermal solution. */
+int fvm_subgrid_temperature_changes(fvm_ctx *ctx, const double *thisT, const fvm_grid *g,
+                                    const NodalArrays *nf, fvm_markers *mk, const Materials *mat,
+                                    double dt, double dsubgrid);
+
+/* ---- A8: subgridStressChanges.c:5-318 --------------------------------------- */
+/* Reads nf->soxx, soyy, soxy; updates nf->dsxx, dsyy, dsxy in place. */
+int fvm_subgrid_stress_changes(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
+                               fvm_markers *mk, const Materials *mat, double dt, double dsubgrid);
+
+/* ---- A9: updateMarkerStress.c:5-163 ----------------------------------------- */
+int fvm_update_marker_stress(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
+                             fvm_markers *mk, const Materials *mat);
+
+/* ---- A10: updateMarkerStrainPressure.c:9-399 -------------------------------- */
+/* Returns FVM_ERR_NAN_RESIDUAL where the reference aborts (a NaN strain-rate
+ * residual on any active marker); marker fields are written either way. */
+int fvm_update_marker_strain_pressure(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
+                                      fvm_markers *mk, const Materials *mat, const Options *opt,
+                                      double dt);
+
+/* ---- A11: markers.c:1381-1505 checkPlasticYielding + viscosity.c:12-75 ------- */
+/* ORs 1 into *isYielding when any marker yields. */
+int fvm_check_plastic_yielding(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk,
+                               const Materials *mat, double dt, int32_t *isYielding,
+                               const Options *opt, double elapsedTime);
+/* viscosity.c:90-104 limitViscosity (every marker m < nMark). */
+int fvm_limit_viscosity(fvm_ctx *ctx, fvm_markers *mk, const Options *opt);
+
+/* ---- A12: residual.c:12-51 updateGlobalStrainRateResidual ------------------- */
+int fvm_update_global_strain_rate_residual(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk,
+                                           double *residual);
+
+/* ---- A14: step glue loops of main_subduction.c ------------------------------ */
+/* :255-264 rho = -rho0(Mat) alpha(Mat) T, rhodot = 0 on active markers. */
+int fvm_update_marker_density(fvm_ctx *ctx, fvm_markers *mk, const Materials *mat);
+/* :271-289 max T over active markers (0 if none is positive). */
+int fvm_max_marker_T(fvm_ctx *ctx, const fvm_markers *mk, double *maxT);
+/* :437-448 max |sxx| over active markers. */
+int fvm_max_abs_sxx(fvm_ctx *ctx, const fvm_markers *mk, double *sxxmax);
+
+/* ---- whole marker step, in the reference's call order (main_subduction.c) -- */
+/* Thermal-convection step: findMarkerCells (:243), density loop (:255-264),
+ * the projection list (or projectMarkersNodesAll2 when list == NULL, :268),
+ * subgridTemperatureChanges (:415, thisT = nf->thisT), advectMarkersRK (:435).
+ * The host solves between them are outside this call: their outputs
+ * (vx, vy, wxy, thisT) are read from nf. */
+int fvm_step_thermal(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const NodalArrays *nf,
+                     const Materials *mat, const Options *opt, const BoundaryValues *bv,
+                     double dt, int32_t nproj, const fvm_projection *list);
+
+/* Visco-elasto-plastic step: findMarkerCells, density loop, checkPlasticYielding
+ * + limitViscosity (:299-300), projectMarkersNodesAll2 into nf (:303); then, with
+ * the host Stokes/nodalStressStrain outputs in host (vx, vy, p, edotxx, edotyy,
+ * edotxy, wxy, soxx, soyy, soxy, dsxx, dsyy, dsxy, thisT):
+ * updateMarkerStrainPressure (:341), updateGlobalStrainRateResidual (:345),
+ * subgridStressChanges (:365), updateMarkerStress (:368),
+ * subgridTemperatureChanges (:415), advectMarkersRK (:435).
+ * Optional outputs (NULL to skip): *isYielding, *residual. */
+int fvm_step_vep(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const NodalArrays *nf,
+                 const NodalArrays *host, const Materials *mat, const Options *opt,
+                 const BoundaryValues *bv, double dt, int32_t *isYielding, double *residual);
+
+/* ---- timing on the context's stream (HIP events) --------------------------- */
+/* Per-phase kernel timing: while enabled, every kernel launch of the library is
+ * bracketed by a pair of HIP events on the context's stream.  Phases:
+ * FVM_PHASE_* below.  fvm_profile_read synchronises and returns, per phase,
+ * the summed milliseconds and the number of launches. */
+enum {
+  FVM_PHASE_LOCATE = 0,    /* findMarkerCells */
+  FVM_PHASE_SORT = 1,      /* bucket keys, scan, scatter, in-bucket order */
+  FVM_PHASE_PARTIALS = 2,  /* projection pass 1 (per lattice cell) */
+  FVM_PHASE_NODES = 3,     /* projection pass 2 (per node) */
+  FVM_PHASE_SUBGRID_T1 = 4,
+  FVM_PHASE_SUBGRID_T2 = 5,
+  FVM_PHASE_ADVECT = 6,
+  FVM_PHASE_SUBGRID_S1 = 7,
+  FVM_PHASE_MARKER_UPDATE = 8, /* stress, strain/pressure, yield, viscosity, density */
+  FVM_PHASE_OTHER = 9,
+  FVM_NUM_PHASES = 10
+};
+int fvm_profile_enable(fvm_ctx *ctx, int32_t on);
+int fvm_profile_read(fvm_ctx *ctx, double *ms, int32_t *launches);
+
+int fvm_timer_start(fvm_ctx *ctx);
+/* Milliseconds between fvm_timer_start and now (synchronises). */
+int fvm_timer_stop(fvm_ctx *ctx, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FVM_ABI_H */

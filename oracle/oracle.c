@@ -1,0 +1,915 @@
+/*
+ * oracle.c -- CPU restatement of fvmic2d's marker hot path.  TEST INFRASTRUCTURE
+ * ONLY (see oracle.h).  Build with -ffp-contract=off: the reference was built by
+ * gcc -O3 without -march (makefile:14), i.e. no fused multiply-add, and the
+ * position / cell chain must reproduce its roundings bit for bit.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+#include "oracle.h"
+
+/* ---- ghosted local arrays (DMDA stencil width 2, nodalFields.c:19) ------- */
+typedef struct {
+  int NX, NY;
+  double *a; /* (NY+4) x (NX+4), zero-initialised */
+} Local;
+
+static int local_new(Local *l, int NX, int NY) {
+  l->NX = NX;
+  l->NY = NY;
+  l->a = (double *)calloc((size_t)(NX + 4) * (NY + 4), sizeof(double));
+  return l->a ? ORC_OK : ORC_ERR_NOMEM;
+}
+static void local_free(Local *l) { free(l->a); l->a = NULL; }
+static double *local_at(Local *l, int jy, int ix) {
+  return &l->a[(size_t)(jy + 2) * (l->NX + 4) + (ix + 2)];
+}
+/* DMGlobalToLocal: interior from the global vector, ghosts stay zero. */
+static void local_fill(Local *l, const double *global) {
+  for (int jy = 0; jy < l->NY; jy++)
+    for (int ix = 0; ix < l->NX; ix++) *local_at(l, jy, ix) = global[(size_t)jy * l->NX + ix];
+}
+/* DMLocalToGlobal ADD_VALUES on DM_BOUNDARY_GHOSTED: ghosts are dropped. */
+static void local_add_to_global(Local *l, double *global) {
+  for (int jy = 0; jy < l->NY; jy++)
+    for (int ix = 0; ix < l->NX; ix++) {
+      size_t i = (size_t)jy * l->NX + ix;
+      global[i] = global[i] + *local_at(l, jy, ix);
+    }
+}
+#define LA(l, jy, ix) (*local_at(&(l), (jy), (ix)))
+
+static double field_of(const Marker *mk, size_t off) {
+  double v;
+  memcpy(&v, (const char *)mk + off, sizeof(double));
+  return v;
+}
+static void set_field_of(Marker *mk, size_t off, double v) { memcpy((char *)mk + off, &v, sizeof(double)); }
+
+/* ---- markers.c:41-106 ---------------------------------------------------- */
+static int find_marker_cell(Marker *mk, const GridData *g) {
+  const int NX = g->NX, NY = g->NY;
+  if (mk->X >= 0.0 && mk->Y >= 0.0 && mk->X <= g->LX && mk->Y <= g->LY) {
+    int lo = 0, hi = NX - 1;
+    while (hi - lo > 1) {
+      int mid = (hi + lo) / 2;
+      if (mid == lo) mid++;
+      else if (mid == hi) mid--;
+      if (mk->X >= g->x[mid]) lo = mid;
+      else hi = mid;
+    }
+    mk->cellX = lo;
+    lo = 0;
+    hi = NY - 1;
+    while (hi - lo > 1) {
+      int mid = (hi + lo) / 2;
+      if (mid == lo) mid++;
+      else if (mid == hi) mid--;
+      if (mk->Y >= g->y[mid]) lo = mid;
+      else hi = mid;
+    }
+    mk->cellY = lo;
+    if (mk->cellX < 0 || mk->cellX > NX - 2 || mk->cellY < 0 || mk->cellY > NY - 2)
+      return ORC_ERR_CELL_OUT_OF_BOUNDS; /* markers.c:74-77 aborts */
+  } else {
+    mk->cellX = -1;
+    mk->cellY = -1;
+  }
+  return ORC_OK;
+}
+
+int orc_find_marker_cells(MarkerSet *ms, const GridData *g, int rank) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  int nout = 0, err = ORC_OK;
+  for (int m = 0; m < ms->nMark; m++) {
+    int e = find_marker_cell(&ms->markers[m], g);
+    if (e && !err) err = e;
+    ms->markers[m].cpu = rank;
+    if (ms->markers[m].cellX == -1) nout++;
+  }
+  ms->nOut = nout;
+  return err;
+}
+
+/* ---- markerProjection.c:252-269 ----------------------------------------- */
+static void find_cell_stag(const Marker *mk, const GridData *g, int stagx, int stagy, int *cx, int *cy) {
+  *cx = mk->cellX;
+  *cy = mk->cellY;
+  if (stagx == 0 && stagy == 0) return;
+  if (stagx == 1) {
+    if (mk->X < g->xc[mk->cellX + 1]) (*cx)--;
+  } else if (stagx == -1) {
+    if (mk->X >= g->xc[mk->cellX + 1]) (*cx)++;
+  }
+  if (stagy == 1) {
+    if (mk->Y < g->yc[mk->cellY + 1]) (*cy)--;
+  } else if (stagy == -1) {
+    if (mk->Y >= g->yc[mk->cellY + 1]) (*cy)++;
+  }
+}
+
+/* ---- markerProjection.c:71-250 ----------------------------------------- */
+int orc_project_marker_field(const GridData *g, const MarkerSet *ms, const double *materialProperty,
+                             size_t field_offset, double *nodalField, int stagx, int stagy,
+                             int method) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (stagx < -1 || stagx > 1 || stagy < -1 || stagy > 1) return ORC_ERR_BAD_STAG;
+  if (method != ARITHMETIC && method != ARITHMETIC_LOCAL && method != HARMONIC) return ORC_ERR_BAD_METHOD;
+  const int NX = g->NX, NY = g->NY;
+  const size_t nn = (size_t)NX * NY;
+  double *wtg = (double *)calloc(nn, sizeof(double));
+  Local wt, nf;
+  if (!wtg || local_new(&wt, NX, NY) || local_new(&nf, NX, NY)) return ORC_ERR_NOMEM;
+  memset(nodalField, 0, nn * sizeof(double));
+  const Marker *mk = ms->markers;
+  double *mf = (double *)malloc((size_t)(ms->nMark > 0 ? ms->nMark : 1) * sizeof(double));
+  if (!mf) return ORC_ERR_NOMEM;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (materialProperty == NULL) mf[m] = field_of(&mk[m], field_offset);
+    else mf[m] = (mk[m].cellX != -1) ? materialProperty[mk[m].Mat] : 0.0; /* only active markers are used */
+  }
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    int cellX, cellY;
+    find_cell_stag(&mk[m], g, stagx, stagy, &cellX, &cellY);
+    double mdx, mdy;
+    if (stagx == -1) mdx = (mk[m].X - g->xc[cellX]) / (g->xc[cellX + 1] - g->xc[cellX]);
+    else if (stagx == 0) mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    else mdx = (mk[m].X - g->xc[cellX + 1]) / (g->xc[cellX + 2] - g->xc[cellX + 1]);
+    if (stagy == -1) mdy = (mk[m].Y - g->yc[cellY]) / (g->yc[cellY + 1] - g->yc[cellY]);
+    else if (stagy == 0) mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    else mdy = (mk[m].Y - g->yc[cellY + 1]) / (g->yc[cellY + 2] - g->yc[cellY + 1]);
+    if (method == ARITHMETIC) {
+      const double w1 = (1.0 - mdx) * (1.0 - mdy);
+      LA(nf, cellY, cellX) += w1 * mf[m];
+      LA(wt, cellY, cellX) += w1;
+      const double w2 = mdx * (1.0 - mdy);
+      LA(nf, cellY, cellX + 1) += w2 * mf[m];
+      LA(wt, cellY, cellX + 1) += w2;
+      const double w3 = (1.0 - mdx) * (mdy);
+      LA(nf, cellY + 1, cellX) += w3 * mf[m];
+      LA(wt, cellY + 1, cellX) += w3;
+      const double w4 = mdx * mdy;
+      LA(nf, cellY + 1, cellX + 1) += w4 * mf[m];
+      LA(wt, cellY + 1, cellX + 1) += w4;
+    } else if (method == ARITHMETIC_LOCAL) {
+      double w = 1.0;
+      if (mdx >= 0.5) {
+        cellX++;
+        w *= mdx;
+      } else {
+        w *= (1.0 - mdx);
+      }
+      if (mdy >= 0.5) {
+        cellY++;
+        w *= mdy;
+      } else {
+        w *= (1.0 - mdy);
+      }
+      LA(nf, cellY, cellX) += w * mf[m];
+      LA(wt, cellY, cellX) += w;
+    } else { /* HARMONIC */
+      if (mdx >= 0.5) cellX++;
+      if (mdy >= 0.5) cellY++;
+      LA(nf, cellY, cellX) += 1.0 / mf[m];
+      LA(wt, cellY, cellX) += 1.0;
+    }
+  }
+  local_add_to_global(&nf, nodalField);
+  local_add_to_global(&wt, wtg);
+  for (size_t i = 0; i < nn; i++) {
+    if (method == HARMONIC) nodalField[i] = wtg[i] / nodalField[i];
+    else nodalField[i] = nodalField[i] / wtg[i];
+  }
+  free(mf);
+  free(wtg);
+  local_free(&wt);
+  local_free(&nf);
+  return ORC_OK;
+}
+
+/* ---- markerProjection.c:11-68 ------------------------------------------- */
+int orc_project_markers_nodes_all2(const MarkerSet *ms, const GridData *g, NodalArrays *nf,
+                                   const Materials *mat) {
+  const size_t nn = (size_t)g->NX * g->NY;
+  int e = 0;
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, rho), nf->rho, 0, 0, ARITHMETIC);
+  memset(nf->rhodot, 0, nn * sizeof(double));
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, eta), nf->etaN, -1, -1, ARITHMETIC);
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, eta), nf->etaS, 0, 0, ARITHMETIC);
+  for (size_t i = 0; i < nn; i++) {
+    nf->muS[i] = 1e99;
+    nf->muN[i] = 1e99;
+  }
+  e |= orc_project_marker_field(g, ms, mat->materialkThermal, 0, nf->kThermal, 0, 0, ARITHMETIC);
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, T), nf->lastT, 0, 0, ARITHMETIC);
+  e |= orc_project_marker_field(g, ms, mat->materialCp, 0, nf->Cp, 0, 0, ARITHMETIC);
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, s.T11), nf->soxx, -1, -1, ARITHMETIC_LOCAL);
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, s.T22), nf->soyy, -1, -1, ARITHMETIC_LOCAL);
+  e |= orc_project_marker_field(g, ms, NULL, offsetof(Marker, s.T12), nf->soxy, 0, 0, ARITHMETIC_LOCAL);
+  return e;
+}
+
+/* ---- markers.c:1507-1594 ------------------------------------------------ */
+int orc_project_markers_nodes_from_scalar(const MarkerSet *ms, const GridData *g,
+                                          const double *markerField, double *nodalField) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  const size_t nn = (size_t)NX * NY;
+  double *wtng = (double *)calloc(nn, sizeof(double));
+  Local wtn, nf;
+  if (!wtng || local_new(&wtn, NX, NY) || local_new(&nf, NX, NY)) return ORC_ERR_NOMEM;
+  memset(nodalField, 0, nn * sizeof(double));
+  const Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    const int cellX = mk[m].cellX, cellY = mk[m].cellY;
+    const double dx = g->x[cellX + 1] - g->x[cellX];
+    const double dy = g->y[cellY + 1] - g->y[cellY];
+    const double mdx = (mk[m].X - g->x[cellX]) / dx;
+    const double mdy = (mk[m].Y - g->y[cellY]) / dy;
+    double w = (1.0 - mdx) * (1.0 - mdy);
+    LA(wtn, cellY, cellX) += w;
+    LA(nf, cellY, cellX) += w * markerField[m];
+    w = mdx * (1.0 - mdy);
+    LA(wtn, cellY, cellX + 1) += w;
+    LA(nf, cellY, cellX + 1) += w * markerField[m];
+    w = (1.0 - mdx) * mdy;
+    LA(wtn, cellY + 1, cellX) += w;
+    LA(nf, cellY + 1, cellX) += w * markerField[m];
+    w = mdx * mdy;
+    LA(wtn, cellY + 1, cellX + 1) += w;
+    LA(nf, cellY + 1, cellX + 1) += w * markerField[m];
+  }
+  local_add_to_global(&wtn, wtng);
+  local_add_to_global(&nf, nodalField);
+  for (size_t i = 0; i < nn; i++) nodalField[i] = nodalField[i] / wtng[i];
+  free(wtng);
+  local_free(&wtn);
+  local_free(&nf);
+  return ORC_OK;
+}
+
+/* ---- markers.c:1670-1706 ------------------------------------------------ */
+int orc_project_nodal_field_to_markers(const double *nodalField, MarkerSet *ms, size_t field_offset,
+                                       const GridData *g) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  Local f;
+  if (local_new(&f, g->NX, g->NY)) return ORC_ERR_NOMEM;
+  local_fill(&f, nodalField);
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    const int cellX = mk[m].cellX, cellY = mk[m].cellY;
+    const double mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    const double mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    const double val = LA(f, cellY, cellX) * (1 - mdx) * (1 - mdy) + LA(f, cellY + 1, cellX) * (1 - mdx) * mdy +
+                       LA(f, cellY, cellX + 1) * (mdx) * (1 - mdy) + LA(f, cellY + 1, cellX + 1) * mdx * mdy;
+    set_field_of(&mk[m], field_offset, val);
+  }
+  local_free(&f);
+  return ORC_OK;
+}
+
+/* ---- markers.c:1865-2250 ------------------------------------------------ */
+int orc_advect_markers_rk(MarkerSet *ms, const NodalArrays *nfa, const GridData *g,
+                          const BoundaryValues *bv, double dt, int jaumann) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  Local vx, vy, wxy;
+  if (local_new(&vx, NX, NY) || local_new(&vy, NX, NY) || local_new(&wxy, NX, NY)) return ORC_ERR_NOMEM;
+  local_fill(&vx, nfa->vx);
+  local_fill(&vy, nfa->vy);
+  local_fill(&wxy, nfa->wxy);
+  const int XCMAX = NX - 2, XCMIN = 0;
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    double xcur = mk[m].X, ycur = mk[m].Y;
+    double mvx[4], mvy[4], mwxy[4];
+    for (int irk = 0; irk < 4; irk++) {
+      int cellX = mk[m].cellX, cellY = mk[m].cellY;
+      if (xcur > g->x[cellX + 1]) {
+        while (xcur > g->x[cellX + 1] && cellX < XCMAX) cellX++;
+      } else if (xcur < g->x[cellX]) {
+        while (xcur < g->x[cellX] && cellX > XCMIN) cellX--;
+      }
+      if (ycur > g->y[cellY + 1]) {
+        while (ycur > g->y[cellY + 1] && cellY < NY - 2) cellY++;
+      } else if (ycur < g->y[cellY]) {
+        while (ycur < g->y[cellY] && cellY > 0) cellY--;
+      }
+      int vxX = cellX, vxY = cellY;
+      if (vxY > -1 && ycur < g->yc[vxY + 1]) vxY--;
+      int vyX = cellX;
+      if (vyX > -1 && xcur < g->xc[vyX + 1]) vyX--;
+      int vyY = cellY;
+      if (vxX < 0) vxX = 0;
+      else if (vxX > NX - 2) vxX = NX - 2;
+      if (vyX < -1) vyX = -1;
+      else if (vyX > NX - 2) vyX = NX - 2;
+      if (vxY < -1) vxY = -1;
+      else if (vxY > NY - 2) vxY = NY - 2;
+      if (vyY < 0) vyY = 0;
+      else if (vyY > NY - 2) vyY = NY - 2;
+
+      double deltaxm = xcur - g->x[vxX];
+      double deltaym = ycur - g->yc[vxY + 1];
+      double dx = g->x[vxX + 1] - g->x[vxX];
+      double dy = g->yc[vxY + 2] - g->yc[vxY + 1];
+      double vxul, vxur;
+      if (vxY == -1 && bv->mechBCTop.type[0] == 0) {
+        vxul = 2.0 * bv->mechBCTop.value[0] - LA(vx, vxY + 1, vxX);
+        vxur = 2.0 * bv->mechBCTop.value[0] - LA(vx, vxY + 1, vxX + 1);
+      } else if (vxY == -1 && bv->mechBCTop.type[0] == 1) {
+        vxul = LA(vx, vxY + 1, vxX);
+        vxur = LA(vx, vxY + 1, vxX + 1);
+      } else {
+        vxul = LA(vx, vxY, vxX);
+        vxur = LA(vx, vxY, vxX + 1);
+      }
+      const double vxll = LA(vx, vxY + 1, vxX);
+      const double vxlr = LA(vx, vxY + 1, vxX + 1);
+      mvx[irk] = vxul * (1.0 - deltaxm / dx) * (1.0 - deltaym / dy);
+      mvx[irk] += vxur * (deltaxm / dx) * (1.0 - deltaym / dy);
+      mvx[irk] += vxll * (1.0 - deltaxm / dx) * (deltaym / dy);
+      mvx[irk] += vxlr * (deltaxm * deltaym) / dx / dy;
+
+      double vyul;
+      const double vyur = LA(vy, vyY, vyX + 1);
+      double vyll;
+      const double vylr = LA(vy, vyY + 1, vyX + 1);
+      if (vyX == -1 && bv->mechBCLeft.type[1] == 0) {
+        vyul = 2.0 * bv->mechBCLeft.value[1] - LA(vy, vyY, vyX + 1);
+        vyll = 2.0 * bv->mechBCLeft.value[1] - LA(vy, vyY + 1, vyX + 1);
+      } else if (vyX == -1 && bv->mechBCLeft.type[1] == 1) {
+        vyul = LA(vy, vyY, vyX + 1);
+        vyll = LA(vy, vyY + 1, vyX + 1);
+      } else {
+        vyul = LA(vy, vyY, vyX);
+        vyll = LA(vy, vyY + 1, vyX);
+      }
+      deltaxm = xcur - g->xc[vyX + 1];
+      deltaym = ycur - g->y[vyY];
+      dx = g->xc[vyX + 2] - g->xc[vyX + 1];
+      dy = g->y[vyY + 1] - g->y[vyY];
+      mvy[irk] = vyul * (1.0 - deltaxm / dx) * (1.0 - deltaym / dy);
+      mvy[irk] += vyur * (deltaxm / dx) * (1.0 - deltaym / dy);
+      mvy[irk] += vyll * (1.0 - deltaxm / dx) * (deltaym / dy);
+      mvy[irk] += vylr * (deltaxm * deltaym) / dx / dy;
+
+      const double mdx = (xcur - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+      const double mdy = (ycur - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+      mwxy[irk] = (1.0 - mdx) * (1.0 - mdy) * LA(wxy, cellY, cellX);
+      mwxy[irk] += (mdx) * (1.0 - mdy) * LA(wxy, cellY, cellX + 1);
+      mwxy[irk] += (1.0 - mdx) * (mdy) * LA(wxy, cellY + 1, cellX);
+      mwxy[irk] += (mdx) * (mdy) * LA(wxy, cellY + 1, cellX + 1);
+
+      if (irk < 3) {
+        if (irk < 2) {
+          xcur = mk[m].X + dt / 2 * mvx[irk];
+          ycur = mk[m].Y + dt / 2 * mvy[irk];
+        } else {
+          xcur = mk[m].X + dt * mvx[irk];
+          ycur = mk[m].Y + dt * mvy[irk];
+        }
+      }
+    }
+    mk[m].VX = (mvx[0] + 2 * mvx[1] + 2 * mvx[2] + mvx[3]) / 6;
+    mk[m].VY = (mvy[0] + 2 * mvy[1] + 2 * mvy[2] + mvy[3]) / 6;
+    mk[m].wxy = (mwxy[0] + 2 * mwxy[1] + 2 * mwxy[2] + mwxy[3]) / 6.0;
+    mk[m].X += dt * mk[m].VX;
+    mk[m].Y += dt * mk[m].VY;
+
+    const double msxx0 = mk[m].s.T11, msyy0 = mk[m].s.T22, msxy0 = mk[m].s.T12;
+    if (jaumann) {
+      const double wxy1 = mk[m].wxy * dt;
+      mk[m].s.T11 += 2.0 * msxy0 * -wxy1;
+      mk[m].s.T22 += 2.0 * msxy0 * wxy1;
+      mk[m].s.T12 += (msxx0 - msyy0) * wxy1;
+    } else {
+      const double thetaz = -mk[m].wxy * dt;
+      mk[m].s.T11 = msxx0 * pow(cos(thetaz), 2.0) + msyy0 * pow(sin(thetaz), 2.0) - msxy0 * sin(2.0 * thetaz);
+      mk[m].s.T22 = msxx0 * pow(sin(thetaz), 2.0) + msyy0 * pow(cos(thetaz), 2.0) + msxy0 * sin(2.0 * thetaz);
+      mk[m].s.T12 = 0.5 * (msxx0 - msyy0) * sin(2.0 * thetaz) + msxy0 * cos(2.0 * thetaz);
+    }
+    mk[m].Eii += dt * sqrt(0.5 * (mk[m].e.T11 * mk[m].e.T11 + mk[m].e.T22 * mk[m].e.T22) + mk[m].e.T12 * mk[m].e.T12);
+  }
+  local_free(&vx);
+  local_free(&vy);
+  local_free(&wxy);
+  return ORC_OK;
+}
+
+/* ---- subgridTemperatureChanges.c:5-124 --------------------------------- */
+int orc_subgrid_temperature_changes(const double *thisT, const GridData *g, const NodalArrays *nfa,
+                                    MarkerSet *ms, const Materials *mat, double dt, double dsubgrid) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  const size_t nn = (size_t)NX * NY;
+  Local T, rho, Cp, kT;
+  if (local_new(&T, NX, NY) || local_new(&rho, NX, NY) || local_new(&Cp, NX, NY) || local_new(&kT, NX, NY))
+    return ORC_ERR_NOMEM;
+  local_fill(&T, nfa->lastT);
+  local_fill(&rho, nfa->rho);
+  local_fill(&Cp, nfa->Cp);
+  local_fill(&kT, nfa->kThermal);
+  double *dTsub = (double *)malloc((size_t)(ms->nMark > 0 ? ms->nMark : 1) * sizeof(double));
+  double *deltaT = (double *)malloc(nn * sizeof(double));
+  if (!dTsub || !deltaT) return ORC_ERR_NOMEM;
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    dTsub[m] = 0.0;
+    if (mk[m].cellX == -1) continue;
+    const int cellY = mk[m].cellY, cellX = mk[m].cellX;
+    const double dx = g->x[cellX + 1] - g->x[cellX];
+    const double dy = g->y[cellY + 1] - g->y[cellY];
+    const double mdx = (mk[m].X - g->x[cellX]) / dx;
+    const double mdy = (mk[m].Y - g->y[cellY]) / dy;
+    const double Tm = LA(T, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) + LA(T, cellY, cellX + 1) * mdx * (1.0 - mdy) +
+                      LA(T, cellY + 1, cellX) * (1.0 - mdx) * (mdy) + LA(T, cellY + 1, cellX + 1) * mdx * mdy;
+    const double rhom = mat->materialRho[0] + LA(rho, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
+                        LA(rho, cellY, cellX + 1) * (mdx) * (1.0 - mdy) + LA(rho, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
+                        LA(rho, cellY + 1, cellX + 1) * mdx * mdy;
+    const double Cpm = LA(Cp, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) + LA(Cp, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
+                       LA(Cp, cellY + 1, cellX) * (1.0 - mdx) * (mdy) + LA(Cp, cellY + 1, cellX + 1) * mdx * mdy;
+    const double km = LA(kT, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) + LA(kT, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
+                      LA(kT, cellY + 1, cellX) * (1.0 - mdx) * (mdy) + LA(kT, cellY + 1, cellX + 1) * mdx * mdy;
+    const double dTdiff = Cpm * rhom / km / (2.0 / (dx * dx) + 2.0 / (dy * dy));
+    dTsub[m] = (Tm - mk[m].T) * (1.0 - exp(-dsubgrid * dt / dTdiff));
+  }
+  int e = orc_project_markers_nodes_from_scalar(ms, g, dTsub, deltaT);
+  if (e) return e;
+  /* VecAXPBYPCZ(deltaT, 1, -1, -1, thisT, lastT): z = x + (-1)y + (-1)z */
+  for (size_t i = 0; i < nn; i++) deltaT[i] = thisT[i] + -1.0 * nfa->lastT[i] + -1.0 * deltaT[i];
+  Local dT;
+  if (local_new(&dT, NX, NY)) return ORC_ERR_NOMEM;
+  local_fill(&dT, deltaT);
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    const int cellY = mk[m].cellY, cellX = mk[m].cellX;
+    const double dx = g->x[cellX + 1] - g->x[cellX];
+    const double dy = g->y[cellY + 1] - g->y[cellY];
+    const double mdx = (mk[m].X - g->x[cellX]) / dx;
+    const double mdy = (mk[m].Y - g->y[cellY]) / dy;
+    const double dtm = (LA(dT, cellY, cellX) * (1.0 - mdy) * (1.0 - mdx) + LA(dT, cellY + 1, cellX) * mdy * (1.0 - mdx) +
+                        LA(dT, cellY, cellX + 1) * (1.0 - mdy) * mdx + LA(dT, cellY + 1, cellX + 1) * mdy * mdx) +
+                       dTsub[m];
+    mk[m].T += dtm;
+    mk[m].Tdot = dtm / dt;
+  }
+  local_free(&dT);
+  free(dTsub);
+  free(deltaT);
+  local_free(&T);
+  local_free(&rho);
+  local_free(&Cp);
+  local_free(&kT);
+  return ORC_OK;
+}
+
+/* ---- subgridStressChanges.c:5-318 -------------------------------------- */
+int orc_subgrid_stress_changes(const GridData *g, NodalArrays *nfa, MarkerSet *ms, const Materials *mat,
+                               double dt, double dsubgrid) {
+  (void)mat;
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  const size_t nn = (size_t)NX * NY;
+  Local soxx, soyy, soxy, dsxxn, dsyyn, dsxyn, wtetan, wtetas;
+  if (local_new(&soxx, NX, NY) || local_new(&soyy, NX, NY) || local_new(&soxy, NX, NY) ||
+      local_new(&dsxxn, NX, NY) || local_new(&dsyyn, NX, NY) || local_new(&dsxyn, NX, NY) ||
+      local_new(&wtetan, NX, NY) || local_new(&wtetas, NX, NY))
+    return ORC_ERR_NOMEM;
+  local_fill(&soxx, nfa->soxx);
+  local_fill(&soyy, nfa->soyy);
+  local_fill(&soxy, nfa->soxy);
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    const double sdm = mk[m].eta / mk[m].mu;
+    double sdif = -dsubgrid * dt / sdm;
+    if (sdif < -30.0) sdif = -30.0;
+    sdif = 1.0 - exp(sdif);
+
+    int cellY = mk[m].cellY, cellX = mk[m].cellX;
+    if (cellX < 0) cellX = 0;
+    if (cellX > NX - 2) cellX = NX - 2;
+    if (cellY < 0) cellY = 0;
+    if (cellY > NY - 2) cellY = NY - 2;
+    double mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    double mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    double sxym = 0.0;
+    sxym += (1.0 - mdx) * (1.0 - mdy) * LA(soxy, cellY, cellX);
+    sxym += (1.0 - mdx) * mdy * LA(soxy, cellY + 1, cellX);
+    sxym += mdx * (1.0 - mdy) * LA(soxy, cellY, cellX + 1);
+    sxym += mdx * mdy * LA(soxy, cellY + 1, cellX + 1);
+    double dsxym = sxym - mk[m].s.T12;
+    dsxym *= sdif;
+    mk[m].s.T12 += dsxym;
+
+    cellY = mk[m].cellY;
+    cellX = mk[m].cellX;
+    mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    if (mdx <= 0.5 && mdy <= 0.5) {
+      LA(dsxyn, cellY, cellX) = LA(dsxyn, cellY, cellX) + (1.0 - mdx) * (1.0 - mdy) * dsxym;
+      LA(wtetas, cellY, cellX) = LA(wtetas, cellY, cellX) + (1.0 - mdx) * (1.0 - mdy);
+    }
+    if (mdx <= 0.5 && mdy >= 0.5) {
+      LA(dsxyn, cellY + 1, cellX) = LA(dsxyn, cellY + 1, cellX) + (1.0 - mdx) * (mdy) * dsxym;
+      LA(wtetas, cellY + 1, cellX) = LA(wtetas, cellY + 1, cellX) + (1.0 - mdx) * (mdy);
+    }
+    if (mdx >= 0.5 && mdy <= 0.5) {
+      LA(dsxyn, cellY, cellX + 1) = LA(dsxyn, cellY, cellX + 1) + (mdx) * (1.0 - mdy) * dsxym;
+      LA(wtetas, cellY, cellX + 1) = LA(wtetas, cellY, cellX + 1) + (mdx) * (1.0 - mdy);
+    }
+    if (mdx >= 0.5 && mdy >= 0.5) {
+      LA(dsxyn, cellY + 1, cellX + 1) = LA(dsxyn, cellY + 1, cellX + 1) + (mdx) * (mdy) * dsxym;
+      LA(wtetas, cellY + 1, cellX + 1) = LA(wtetas, cellY + 1, cellX + 1) + (mdx) * (mdy);
+    }
+
+    cellX = mk[m].cellX;
+    cellY = mk[m].cellY;
+    if (cellX < NX - 1 && mk[m].X >= g->xc[cellX + 1]) cellX++;
+    if (cellY < NY - 1 && mk[m].Y >= g->yc[cellY + 1]) cellY++;
+    if (cellY < 1) cellY = 1;
+    else if (cellY > NY - 2) cellY = NY - 2;
+    if (cellX < 1) cellX = 1;
+    else if (cellX > NX - 2) cellX = NX - 2;
+    mdx = (mk[m].X - g->xc[cellX]) / (g->xc[cellX + 1] - g->xc[cellX]);
+    mdy = (mk[m].Y - g->yc[cellY]) / (g->yc[cellY + 1] - g->yc[cellY]);
+    double sxxm = 0.0;
+    sxxm += (1.0 - mdx) * (1.0 - mdy) * LA(soxx, cellY, cellX);
+    sxxm += (1.0 - mdx) * mdy * LA(soxx, cellY + 1, cellX);
+    sxxm += mdx * (1.0 - mdy) * LA(soxx, cellY, cellX + 1);
+    sxxm += mdx * mdy * LA(soxx, cellY + 1, cellX + 1);
+    double dsxxm = sxxm - mk[m].s.T11;
+    dsxxm *= sdif;
+    mk[m].s.T11 += dsxxm;
+    double syym = 0.0;
+    syym += (1.0 - mdx) * (1.0 - mdy) * LA(soyy, cellY, cellX);
+    syym += (1.0 - mdx) * mdy * LA(soyy, cellY + 1, cellX);
+    syym += mdx * (1.0 - mdy) * LA(soyy, cellY, cellX + 1);
+    syym += mdx * mdy * LA(soyy, cellY + 1, cellX + 1);
+    double dsyym = syym - mk[m].s.T22;
+    dsyym *= sdif;
+    mk[m].s.T22 += dsyym;
+
+    cellX = mk[m].cellX;
+    cellY = mk[m].cellY;
+    mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    LA(dsxxn, cellY + 1, cellX + 1) += (1.0 - fabs(0.5 - mdx)) * (1.0 - fabs(0.5 - mdy)) * dsxxm;
+    LA(dsyyn, cellY + 1, cellX + 1) += (1.0 - fabs(0.5 - mdx)) * (1.0 - fabs(0.5 - mdy)) * dsyym;
+    LA(wtetan, cellY + 1, cellX + 1) += (1.0 - fabs(0.5 - mdx)) * (1.0 - fabs(0.5 - mdy));
+  }
+  double *wn = (double *)calloc(nn, sizeof(double)), *ws = (double *)calloc(nn, sizeof(double));
+  double *gx = (double *)calloc(nn, sizeof(double)), *gy = (double *)calloc(nn, sizeof(double));
+  double *gxy = (double *)calloc(nn, sizeof(double));
+  if (!wn || !ws || !gx || !gy || !gxy) return ORC_ERR_NOMEM;
+  local_add_to_global(&wtetan, wn);
+  local_add_to_global(&wtetas, ws);
+  local_add_to_global(&dsxxn, gx);
+  local_add_to_global(&dsyyn, gy);
+  local_add_to_global(&dsxyn, gxy);
+  for (size_t i = 0; i < nn; i++) {
+    gxy[i] = gxy[i] / ws[i];
+    gx[i] = gx[i] / wn[i];
+    gy[i] = gy[i] / wn[i];
+    nfa->dsxx[i] = nfa->dsxx[i] + -1.0 * gx[i];
+    nfa->dsyy[i] = nfa->dsyy[i] + -1.0 * gy[i];
+    nfa->dsxy[i] = nfa->dsxy[i] + -1.0 * gxy[i];
+  }
+  free(wn);
+  free(ws);
+  free(gx);
+  free(gy);
+  free(gxy);
+  local_free(&soxx);
+  local_free(&soyy);
+  local_free(&soxy);
+  local_free(&dsxxn);
+  local_free(&dsyyn);
+  local_free(&dsxyn);
+  local_free(&wtetan);
+  local_free(&wtetas);
+  return ORC_OK;
+}
+
+/* ---- updateMarkerStress.c:5-163 ---------------------------------------- */
+int orc_update_marker_stress(const GridData *g, const NodalArrays *nfa, MarkerSet *ms) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  Local dsxx, dsyy, dsxy;
+  if (local_new(&dsxx, NX, NY) || local_new(&dsyy, NX, NY) || local_new(&dsxy, NX, NY)) return ORC_ERR_NOMEM;
+  local_fill(&dsxx, nfa->dsxx);
+  local_fill(&dsyy, nfa->dsyy);
+  local_fill(&dsxy, nfa->dsxy);
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    int cellX = mk[m].cellX, cellY = mk[m].cellY;
+    int cellXMax = NX - 2;
+    if (cellX < cellXMax && mk[m].X >= g->xc[cellX + 1]) cellX++;
+    if (cellY < NY - 1 && mk[m].Y >= g->yc[cellY + 1]) cellY++;
+    if (cellY < 1) cellY = 1;
+    else if (cellY > NY - 2) cellY = NY - 2;
+    cellXMax = NX - 2;
+    if (cellX < 1) cellX = 1;
+    else if (cellX > cellXMax) cellX = cellXMax;
+    double mdx = (mk[m].X - g->xc[cellX]) / (g->xc[cellX + 1] - g->xc[cellX]);
+    double mdy = (mk[m].Y - g->yc[cellY]) / (g->yc[cellY + 1] - g->yc[cellY]);
+    double mdsxx = 0.0;
+    mdsxx += (1.0 - mdx) * (1.0 - mdy) * LA(dsxx, cellY, cellX);
+    mdsxx += (1.0 - mdx) * mdy * LA(dsxx, cellY + 1, cellX);
+    mdsxx += mdx * (1.0 - mdy) * LA(dsxx, cellY, cellX + 1);
+    mdsxx += mdx * mdy * LA(dsxx, cellY + 1, cellX + 1);
+    double mdsyy = 0.0;
+    mdsyy += (1.0 - mdx) * (1.0 - mdy) * LA(dsyy, cellY, cellX);
+    mdsyy += (1.0 - mdx) * mdy * LA(dsyy, cellY + 1, cellX);
+    mdsyy += mdx * (1.0 - mdy) * LA(dsyy, cellY, cellX + 1);
+    mdsyy += mdx * mdy * LA(dsyy, cellY + 1, cellX + 1);
+    mk[m].s.T11 += mdsxx;
+    mk[m].s.T22 += mdsyy;
+    cellX = mk[m].cellX;
+    cellY = mk[m].cellY;
+    cellXMax = NX - 2;
+    if (cellY < 0) cellY = 0;
+    else if (cellY > NY - 2) cellY = NY - 2;
+    if (cellX < 0) cellX = 0;
+    else if (cellX > cellXMax) cellX = cellXMax;
+    mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    double mdsxy = 0.0;
+    mdsxy += (1.0 - mdx) * (1.0 - mdy) * LA(dsxy, cellY, cellX);
+    mdsxy += (1.0 - mdx) * mdy * LA(dsxy, cellY + 1, cellX);
+    mdsxy += mdx * (1.0 - mdy) * LA(dsxy, cellY, cellX + 1);
+    mdsxy += mdx * mdy * LA(dsxy, cellY + 1, cellX + 1);
+    mk[m].s.T12 += mdsxy;
+  }
+  local_free(&dsxx);
+  local_free(&dsyy);
+  local_free(&dsxy);
+  return ORC_OK;
+}
+
+/* ---- updateMarkerStrainPressure.c:9-399 -------------------------------- */
+static double bil_c(Local *f, int cellY, int cellX, double mdx, double mdy) {
+  double v = 0.0;
+  v += (1.0 - mdx) * (1.0 - mdy) * LA(*f, cellY, cellX);
+  v += (1.0 - mdx) * mdy * LA(*f, cellY + 1, cellX);
+  v += mdx * (1.0 - mdy) * LA(*f, cellY, cellX + 1);
+  v += mdx * mdy * LA(*f, cellY + 1, cellX + 1);
+  return v;
+}
+
+int orc_update_marker_strain_pressure(const GridData *g, const NodalArrays *nfa, MarkerSet *ms, double dt) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  Local dsxx, dsyy, dsxy, soxx, soyy, soxy, exx, eyy, exy, wxy, p;
+  if (local_new(&dsxx, NX, NY) || local_new(&dsyy, NX, NY) || local_new(&dsxy, NX, NY) ||
+      local_new(&soxx, NX, NY) || local_new(&soyy, NX, NY) || local_new(&soxy, NX, NY) ||
+      local_new(&exx, NX, NY) || local_new(&eyy, NX, NY) || local_new(&exy, NX, NY) ||
+      local_new(&wxy, NX, NY) || local_new(&p, NX, NY))
+    return ORC_ERR_NOMEM;
+  local_fill(&dsxx, nfa->dsxx);
+  local_fill(&dsyy, nfa->dsyy);
+  local_fill(&dsxy, nfa->dsxy);
+  local_fill(&soxx, nfa->soxx);
+  local_fill(&soyy, nfa->soyy);
+  local_fill(&soxy, nfa->soxy);
+  local_fill(&exx, nfa->edotxx);
+  local_fill(&eyy, nfa->edotyy);
+  local_fill(&exy, nfa->edotxy);
+  local_fill(&wxy, nfa->wxy);
+  local_fill(&p, nfa->p);
+  int err = ORC_OK;
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    int cellX = mk[m].cellX, cellY = mk[m].cellY;
+    int cellXMax = NX - 2;
+    if (cellX < cellXMax && mk[m].X >= g->xc[cellX + 1]) cellX++;
+    if (cellY < NY - 1 && mk[m].Y >= g->yc[cellY + 1]) cellY++;
+    if (cellY < 1) cellY = 1;
+    else if (cellY > NY - 2) cellY = NY - 2;
+    cellXMax = NX - 2;
+    if (cellX < 1) cellX = 1;
+    else if (cellX > cellXMax) cellX = cellXMax;
+    double mdx = (mk[m].X - g->xc[cellX]) / (g->xc[cellX + 1] - g->xc[cellX]);
+    double mdy = (mk[m].Y - g->yc[cellY]) / (g->yc[cellY + 1] - g->yc[cellY]);
+    mk[m].p = bil_c(&p, cellY, cellX, mdx, mdy);
+    const double mexx = bil_c(&exx, cellY, cellX, mdx, mdy);
+    (void)bil_c(&eyy, cellY, cellX, mdx, mdy); /* meyy: computed, unused (:166-170) */
+    const double msxx = bil_c(&soxx, cellY, cellX, mdx, mdy);
+    const double msyy = bil_c(&soyy, cellY, cellX, mdx, mdy);
+    const double mdsxx = bil_c(&dsxx, cellY, cellX, mdx, mdy);
+    const double mdsyy = bil_c(&dsyy, cellY, cellX, mdx, mdy);
+
+    cellX = mk[m].cellX;
+    cellY = mk[m].cellY;
+    cellXMax = NX - 2;
+    if (cellY < 0) cellY = 0;
+    else if (cellY > NY - 2) cellY = NY - 2;
+    if (cellX < 0) cellX = 0;
+    else if (cellX > cellXMax) cellX = cellXMax;
+    mdx = (mk[m].X - g->x[cellX]) / (g->x[cellX + 1] - g->x[cellX]);
+    mdy = (mk[m].Y - g->y[cellY]) / (g->y[cellY + 1] - g->y[cellY]);
+    mk[m].wxy = bil_c(&wxy, cellY, cellX, mdx, mdy);
+    const double mexy = bil_c(&exy, cellY, cellX, mdx, mdy);
+    const double msxy = bil_c(&soxy, cellY, cellX, mdx, mdy);
+    const double mdsxy = bil_c(&dsxy, cellY, cellX, mdx, mdy);
+
+    mk[m].e.T11 = msxx / 2.0 / mk[m].eta + mdsxx / 2.0 / dt / mk[m].mu;
+    mk[m].e.T22 = msyy / 2.0 / mk[m].eta + mdsyy / 2.0 / dt / mk[m].mu;
+    mk[m].e.T12 = msxy / 2.0 / mk[m].eta + mdsxy / 2.0 / dt / mk[m].mu;
+    mk[m].strainRateResidual = sqrt(pow(mk[m].e.T11 - mexx, 2.0) + pow(mk[m].e.T12 - mexy, 2.0));
+    if (isnan(mk[m].strainRateResidual) && !err) err = ORC_ERR_NAN_RESIDUAL; /* :302-306 aborts */
+  }
+  local_free(&dsxx);
+  local_free(&dsyy);
+  local_free(&dsxy);
+  local_free(&soxx);
+  local_free(&soyy);
+  local_free(&soxy);
+  local_free(&exx);
+  local_free(&eyy);
+  local_free(&exy);
+  local_free(&wxy);
+  local_free(&p);
+  return err;
+}
+
+/* ---- viscosity.c:12-67 ------------------------------------------------- */
+int orc_compute_rheology(const Marker *mk, const Options *o, const Materials *mat, double *eta, double *mu) {
+  const int mm = mk->Mat;
+  double thiseta;
+  const double thismu = mat->materialMu[mm];
+  const int law = mat->hasEtaT[mm];
+  if (law == 1) {
+    thiseta = mat->materialEta[mm] * exp(mat->QonR[mm] * (1.0 / mk->T - 1.0 / mat->Tref[mm]));
+  } else if (law == 2) {
+    const double T0 = mat->Tref[mm];
+    thiseta = mat->materialEta[mm] * exp(mat->QonR[mm] / T0 * (1.0 - (mk->T - T0) / T0));
+  } else if (law == 3) {
+    thiseta = mat->materialEta[mm] * mk->T;
+  } else if (law == 5) {
+    thiseta = mat->materialEta[mm] * exp(-mat->QonR[mm] * mk->T);
+  } else if (law == 0) {
+    thiseta = mat->materialEta[mm];
+  } else if (law == 6) {
+    const double depth = mk->Y, distance = mk->X;
+    thiseta = mat->materialEta[mm];
+    if (distance > depth / tan(o->slabAngle)) {
+      if (distance < (depth / tan(o->slabAngle) + o->decoupleThickness)) {
+        if (depth > o->minDecoupleDepth && depth < o->maxDecoupleDepth) thiseta *= o->decoupleEtaScaler;
+      }
+    }
+  } else {
+    return ORC_ERR_VISCOSITY_LAW; /* viscosity.c:60-62 aborts */
+  }
+  *eta = thiseta;
+  *mu = thismu;
+  return ORC_OK;
+}
+
+/* ---- markers.c:1381-1505 ----------------------------------------------- */
+int orc_check_plastic_yielding(const GridData *g, MarkerSet *ms, const Materials *mat, double dt,
+                               int *isYielding, const Options *o) {
+  (void)g;
+  const double etamin = o->etamin, etamax = o->etamax, fractionalEtamin = o->fractionalEtamin;
+  int err = ORC_OK;
+  Marker *mk = ms->markers;
+  for (int m = 0; m < ms->nMark; m++) {
+    if (mk[m].cellX == -1) continue;
+    double etanew = mk[m].eta;
+    const double X = mk[m].eta / (mk[m].mu * dt + mk[m].eta);
+    const double sxxnew = mk[m].s.T11 * X + 2 * mk[m].eta * mk[m].e.T11 * (1.0 - X);
+    const double syynew = mk[m].s.T22 * X + 2 * mk[m].eta * mk[m].e.T22 * (1.0 - X);
+    const double sxynew = mk[m].s.T12 * X + 2 * mk[m].eta * mk[m].e.T12 * (1.0 - X);
+    const double siinew = sqrt(0.5 * (sxxnew * sxxnew + syynew * syynew) + sxynew * sxynew);
+    {
+      double eta, mu;
+      int e = orc_compute_rheology(&mk[m], o, mat, &eta, &mu);
+      if (e) {
+        if (!err) err = e;
+        continue;
+      }
+      mk[m].eta = eta;
+      mk[m].mu = mu;
+    }
+    if (mat->hasPlasticity[mk[m].Mat]) {
+      const int thismat = mk[m].Mat;
+      double siiyield;
+      const int plast = mat->hasPlasticity[thismat];
+      if (plast == 0) {
+        siiyield = DBL_MAX;
+      } else if (plast == 1) {
+        siiyield = mat->materialCohesion[thismat] + mat->materialFriction[thismat] * mk[m].p;
+      } else if (plast == 2) {
+        const double mEii = mk[m].Eii;
+        double c, f;
+        if (mEii < mat->gamma[thismat][0]) {
+          c = mat->C[thismat][0];
+          f = sin(mat->F[thismat][0] / 180.0 * M_PI);
+        } else if (mEii >= mat->gamma[thismat][0] && mEii <= mat->gamma[thismat][1]) {
+          const double dg = (mEii - mat->gamma[thismat][0]) / (mat->gamma[thismat][1] - mat->gamma[thismat][0]);
+          c = (mat->C[thismat][1] - mat->C[thismat][0]) * dg + mat->C[thismat][0];
+          f = (sin(mat->F[thismat][1] / 180.0 * M_PI) - sin(mat->F[thismat][0] / 180.0 * M_PI)) * dg +
+              sin(mat->F[thismat][0] / 180.0 * M_PI);
+        } else {
+          c = mat->C[thismat][1];
+          f = sin(mat->F[thismat][1] / 180.0 * M_PI);
+        }
+        siiyield = c + f * mk[m].p;
+      } else {
+        siiyield = 0.0;
+      }
+      if (siiyield < 0.0) siiyield = 0.0;
+      if (siinew > siiyield) {
+        const double exx = mk[m].e.T11, eyy = mk[m].e.T22, exy = mk[m].e.T12;
+        const double eiiold = sqrt(0.5 * (exx * exx + eyy * eyy) + exy * exy);
+        etanew = siiyield / 2.0 / eiiold;
+        if (etanew / mk[m].eta < fractionalEtamin) etanew = fractionalEtamin * mk[m].eta;
+        if (etanew < etamin) etanew = etamin;
+        else if (etanew > etamax) etanew = etamax;
+        const double siiold =
+            sqrt(0.5 * (mk[m].s.T11 * mk[m].s.T11 + mk[m].s.T22 * mk[m].s.T22) + mk[m].s.T12 * mk[m].s.T12);
+        if (siiold != 0.0) {
+          mk[m].s.T11 *= siiyield / siiold;
+          mk[m].s.T22 *= siiyield / siiold;
+          mk[m].s.T12 *= siiyield / siiold;
+        }
+        isYielding[0] = 1;
+      }
+      mk[m].eta = etanew;
+    }
+  }
+  return err;
+}
+
+/* ---- viscosity.c:90-104 (every marker m < nMark, active or not) ---------- */
+void orc_limit_viscosity(MarkerSet *ms, const Options *o) {
+  for (int m = 0; m < ms->nMark; m++) {
+    Marker *mk = &ms->markers[m];
+    if (mk->eta > o->etamax) mk->eta = o->etamax;
+    else if (mk->eta < o->etamin) mk->eta = o->etamin;
+  }
+}
+
+/* ---- residual.c:12-51 --------------------------------------------------- */
+int orc_update_global_strain_rate_residual(const GridData *g, const MarkerSet *ms, double *residual) {
+  const int NX = g->NX, NY = g->NY;
+  double *srr = (double *)malloc((size_t)NX * NY * sizeof(double));
+  if (!srr) return ORC_ERR_NOMEM;
+  int e = orc_project_marker_field(g, ms, NULL, offsetof(Marker, strainRateResidual), srr, -1, -1, ARITHMETIC);
+  if (e) {
+    free(srr);
+    return e;
+  }
+  for (int ix = 0; ix < NX; ix++)
+    for (int jy = 0; jy < NY; jy++) {
+      double *v = &srr[(size_t)jy * NX + ix];
+      if (ix > 0 && jy > 0) {
+        const double dx = g->x[ix] - g->x[ix - 1];
+        const double dy = g->y[jy] - g->y[jy - 1];
+        *v *= dx * dy;
+      } else {
+        *v = 0.0;
+      }
+    }
+  double sum = 0.0;
+  for (size_t i = 0; i < (size_t)NX * NY; i++) sum += srr[i];
+  sum /= (g->LX * g->LY);
+  *residual = sum;
+  free(srr);
+  return ORC_OK;
+}
+
+/* ---- main_subduction.c glue loops --------------------------------------- */
+void orc_update_marker_density(MarkerSet *ms, const Materials *mat) {
+  for (int m = 0; m < ms->nMark; m++) {
+    Marker *mk = &ms->markers[m];
+    if (mk->cellX == -1) continue;
+    mk->rho = -mat->materialRho[mk->Mat] * (mat->materialAlpha[mk->Mat] * mk->T);
+    mk->rhodot = 0.0;
+  }
+}
+
+double orc_max_marker_T(const MarkerSet *ms) {
+  double maxT = 0.0;
+  for (int m = 0; m < ms->nMark; m++)
+    if (ms->markers[m].cellX != -1 && ms->markers[m].T > maxT) maxT = ms->markers[m].T;
+  return maxT;
+}
+
+double orc_max_abs_sxx(const MarkerSet *ms) {
+  double v = 0.0;
+  for (int m = 0; m < ms->nMark; m++)
+    if (ms->markers[m].cellX != -1 && fabs(ms->markers[m].s.T11) > v) v = fabs(ms->markers[m].s.T11);
+  return v;
+}
