@@ -32,26 +32,19 @@ PHASES = ["locate", "sort", "partials", "nodes", "subgrid_t1", "subgrid_t2", "ad
 
 
 def algorithmic_bytes(w):
-    """Compulsory state traffic per launch of each phase, for one convection_tdep step.
+    """Compulsory state traffic per launch of each phase of one convection_tdep step.
 
-    Per marker (FP64 = 8 B, int32 = 4 B) and per node; sort keys, permutations and
-    partial sums are implementation traffic and are not counted."""
+    Per active marker (FP64 = 8 B, int32 = 4 B) and per node.  Sort keys,
+    permutations and partial sums are implementation traffic and not counted."""
     n, nodes = w.n_markers, w.NX * w.NY
     return {
-        # read X, Y; write cellX, cellY, cpu
-        "locate": n * (16 + 12),
-        # read X, Y, cellX, cellY of every marker
-        "sort": n * 24,
-        # the 17 projections: read X, Y, T, eta, rho, Mat once (+ dTsub, X, Y in subgrid);
-        # spread over 5 partial launches per step -> reported per launch below
-        "partials": None,
-        "nodes": None,
-        # read X, Y, T, cell; 4 nodal gathers; write dTsub
-        "subgrid_t1": n * (8 * 3 + 8 + 8) + nodes * 32,
-        # read X, Y, T, cell, dTsub; write T, Tdot; read deltaT
-        "subgrid_t2": n * (8 * 4 + 8 + 16) + nodes * 8,
-        # read X, Y, cell, s(3), e(3), Eii; write X, Y, VX, VY, wxy, s(3), Eii; read vx, vy, wxy
-        "advect": n * (8 * 2 + 8 + 24 + 24 + 8 + 72) + nodes * 24,
+        # findMarkerCells + density loop: read X, Y, T, Mat; write cellX, cellY, cpu, rho, rhodot
+        "locate": n * (16 + 8 + 4 + 12 + 16),
+        # subgrid T pass 1: read X, Y, T, cellX, cellY; write dTsub; gather lastT, rho, Cp, k
+        "subgrid_t1": n * (24 + 8 + 8) + nodes * 32,
+        # subgrid T pass 2 + RK4: read X, Y, cellX, cellY, T, dTsub, s(3), e(3), Eii;
+        # write T, Tdot, X, Y, VX, VY, wxy, s(3), Eii; gather deltaT, vx, vy, wxy
+        "advect": n * (96 + 88) + nodes * 32,
     }
 
 

@@ -139,12 +139,16 @@ struct ProjGroup {
 };
 
 // findMarkerCells without the host round trip: counters (outside, bad) are
-// left in ctx->red[0..1] (sort.hip).
-int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank);
+// left in ctx->red[0..1].  With `dens` it also runs the density loop of
+// main_subduction.c:255-264; with build_order it also builds the bucket order
+// for the new positions (sort.hip).
+int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank, const Materials *dens,
+                     bool build_order);
 // Build the (cell, quadrant) bucket order of the active markers (sort.hip).
 int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk);
-// Run one projection group over the current bucket order (project.hip).
+// Run projection groups over the current bucket order (project.hip).
 int fvm_project_group(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg);
+int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, int ng, const ProjGroup *pgs);
 // Invalidate the cached bucket order (positions or cells changed).
 static inline void fvm_markers_moved(fvm_ctx *ctx) { ctx->epoch++; }
 
@@ -164,6 +168,14 @@ struct ProfScope {
   }
 };
 #define FVM_PHASE(ctx, ph) ProfScope fvm_prof_scope_(ctx, ph)
+
+// subgridTemperatureChanges split for fusion (ops.hip)
+int fvm_subgrid_t_pass1(fvm_ctx *ctx, const double *thisT, const fvm_grid *g, const NodalArrays *nf,
+                        fvm_markers *mk, const Materials *mat, double dt, double dsubgrid, double **dTsub_out,
+                        double **deltaT_out);
+int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays *nf, const fvm_grid *g,
+                               const BoundaryValues *bv, double dt, const double *deltaT, const double *dTsub,
+                               double dt_thermal);
 
 // deterministic double sum of n values on the device -> *out (host), ops.hip
 int fvm_reduce_sum(fvm_ctx *ctx, const double *d, int64_t n, double *out);

@@ -19,7 +19,9 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
                                                     const int32_t *__restrict__ cellXa, const int32_t *__restrict__ cellYa,
                                                     const double *__restrict__ vx, const double *__restrict__ vy,
                                                     const double *__restrict__ wxy, BoundaryValues bv, double dt,
-                                                    int jaumann) {
+                                                    int jaumann, double *__restrict__ MT, double *__restrict__ MTdot,
+                                                    const double *__restrict__ dTn, const double *__restrict__ dTsub,
+                                                    double dt_thermal) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const int c0x = cellXa[m], c0y = cellYa[m];
@@ -27,6 +29,21 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
   const int NX = g.NX, NY = g.NY;
   const int XCMAX = NX - 2, XCMIN = 0;
   const double X0 = MX[m], Y0 = MY[m];
+  if (MT) {
+    // subgridTemperatureChanges.c:89-103 (pass 2) on the same stored cell,
+    // fused here because advectMarkersRK starts from that cell too
+    const double dx = g.x[c0x + 1] - g.x[c0x];
+    const double dy = g.y[c0y + 1] - g.y[c0y];
+    const double mdx = (X0 - g.x[c0x]) / dx;
+    const double mdy = (Y0 - g.y[c0y]) / dy;
+    const double dtm = (gat(dTn, NX, NY, c0y, c0x) * (1.0 - mdy) * (1.0 - mdx) +
+                        gat(dTn, NX, NY, c0y + 1, c0x) * mdy * (1.0 - mdx) +
+                        gat(dTn, NX, NY, c0y, c0x + 1) * (1.0 - mdy) * mdx +
+                        gat(dTn, NX, NY, c0y + 1, c0x + 1) * mdy * mdx) +
+                       dTsub[m];
+    MT[m] = MT[m] + dtm;
+    MTdot[m] = dtm / dt_thermal;
+  }
   double xcur = X0, ycur = Y0;
   double mvx[4], mvy[4], mwxy[4];
 #pragma unroll
@@ -60,6 +77,8 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
     double deltaym = ycur - g.yc[vxY + 1];
     double dx = g.x[vxX + 1] - g.x[vxX];
     double dy = g.yc[vxY + 2] - g.yc[vxY + 1];
+    // vxX == cellX (cellX stays in [0, NX-2]), so this quotient is also the wxy mdx below
+    const double qx = deltaxm / dx;
     double vxul, vxur;
     if (vxY == -1 && bv.mechBCTop.type[0] == 0) {
       vxul = 2.0 * bv.mechBCTop.value[0] - gat(vx, NX, NY, vxY + 1, vxX);
@@ -73,9 +92,10 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
     }
     const double vxll = gat(vx, NX, NY, vxY + 1, vxX);
     const double vxlr = gat(vx, NX, NY, vxY + 1, vxX + 1);
-    double a = vxul * (1.0 - deltaxm / dx) * (1.0 - deltaym / dy);
-    a += vxur * (deltaxm / dx) * (1.0 - deltaym / dy);
-    a += vxll * (1.0 - deltaxm / dx) * (deltaym / dy);
+    const double qyv = deltaym / dy;
+    double a = vxul * (1.0 - qx) * (1.0 - qyv);
+    a += vxur * (qx) * (1.0 - qyv);
+    a += vxll * (1.0 - qx) * (qyv);
     a += vxlr * (deltaxm * deltaym) / dx / dy;
     mvx[irk] = a;
 
@@ -97,14 +117,19 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
     deltaym = ycur - g.y[vyY];
     dx = g.xc[vyX + 2] - g.xc[vyX + 1];
     dy = g.y[vyY + 1] - g.y[vyY];
-    double b = vyul * (1.0 - deltaxm / dx) * (1.0 - deltaym / dy);
-    b += vyur * (deltaxm / dx) * (1.0 - deltaym / dy);
-    b += vyll * (1.0 - deltaxm / dx) * (deltaym / dy);
+    // vyY == cellY (cellY stays in [0, NY-2]), so this quotient is also the wxy mdy below
+    const double qy = deltaym / dy;
+    const double qxv = deltaxm / dx;
+    double b = vyul * (1.0 - qxv) * (1.0 - qy);
+    b += vyur * (qxv) * (1.0 - qy);
+    b += vyll * (1.0 - qxv) * (qy);
     b += vylr * (deltaxm * deltaym) / dx / dy;
     mvy[irk] = b;
 
-    const double mdx = (xcur - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
-    const double mdy = (ycur - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
+    // markers.c:2089-2090: (xcur - x[cellX]) / (x[cellX+1] - x[cellX]), the same
+    // operands as qx and qy above, hence the same values
+    const double mdx = qx;
+    const double mdy = qy;
     double w = (1.0 - mdx) * (1.0 - mdy) * gat(wxy, NX, NY, cellY, cellX);
     w += (mdx) * (1.0 - mdy) * gat(wxy, NX, NY, cellY, cellX + 1);
     w += (1.0 - mdx) * (mdy) * gat(wxy, NX, NY, cellY + 1, cellX);
@@ -139,8 +164,10 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
   } else {
     const double thetaz = -wm * dt;
     const double c = cos(thetaz), s = sin(thetaz), s2 = sin(2.0 * thetaz), c2 = cos(2.0 * thetaz);
-    sxx[m] = msxx0 * pow(c, 2.0) + msyy0 * pow(s, 2.0) - msxy0 * s2;
-    syy[m] = msxx0 * pow(s, 2.0) + msyy0 * pow(c, 2.0) + msxy0 * s2;
+    // pow(v, 2.0) of markers.c:2162-2163: gcc -O3 (makefile:14) folds it to v*v,
+    // which is what the reference binary executes
+    sxx[m] = msxx0 * (c * c) + msyy0 * (s * s) - msxy0 * s2;
+    syy[m] = msxx0 * (s * s) + msyy0 * (c * c) + msxy0 * s2;
     sxy[m] = 0.5 * (msxx0 - msyy0) * s2 + msxy0 * c2;
   }
   const double e11 = exx[m], e22 = eyy[m], e12 = exy[m];
@@ -156,7 +183,8 @@ extern "C" int fvm_advect_markers_rk(fvm_ctx *ctx, fvm_markers *mk, const NodalA
     FVM_PHASE(ctx, FVM_PHASE_ADVECT);
     k_advect_rk4<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(
         dgrid(g), mk->nMark, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy,
-        mk->exy, mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann);
+        mk->exy, mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, nullptr, nullptr,
+      nullptr, nullptr, 0.0);
     FVM_LAUNCH_CHECK(ctx);
   }
   fvm_markers_moved(ctx);
@@ -225,12 +253,11 @@ __global__ void __launch_bounds__(256) k_subgrid_t2(DGrid g, int32_t n, const do
   MTdot[m] = dtm / dt;
 }
 
-extern "C" int fvm_subgrid_temperature_changes(fvm_ctx *ctx, const double *thisT, const fvm_grid *g,
-                                               const NodalArrays *nf, fvm_markers *mk, const Materials *mat,
-                                               double dt, double dsubgrid) {
-  if (!ctx || !mk || !nf || !mat || !thisT || !nf->lastT || !nf->rho || !nf->Cp || !nf->kThermal)
-    return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+// Pass 1 and the dTsub projection of subgridTemperatureChanges; leaves dTsub
+// (per marker) and deltaT (nodal) in the context scratch for pass 2.
+int fvm_subgrid_t_pass1(fvm_ctx *ctx, const double *thisT, const fvm_grid *g, const NodalArrays *nf,
+                        fvm_markers *mk, const Materials *mat, double dt, double dsubgrid, double **dTsub_out,
+                        double **deltaT_out) {
   const int32_t n = mk->nMark;
   const int64_t nn = (int64_t)g->NX * g->NY;
   FVM_TRY(fvm_ws(ctx, ctx->scratch, ((size_t)n + (size_t)nn + 64) * sizeof(double)));
@@ -238,13 +265,11 @@ extern "C" int fvm_subgrid_temperature_changes(fvm_ctx *ctx, const double *thisT
   double *deltaT = dTsub + ((n + 31) / 32) * 32;
   const DGrid dg = dgrid(g);
   if (n > 0) {
-    {
-      FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
-      k_subgrid_t1<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->T, mk->cellX, mk->cellY,
-                                                                 nf->lastT, nf->rho, nf->Cp, nf->kThermal,
-                                                                 mat->materialRho[0], dt, dsubgrid, dTsub);
-      FVM_LAUNCH_CHECK(ctx);
-    }
+    FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
+    k_subgrid_t1<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->T, mk->cellX, mk->cellY,
+                                                               nf->lastT, nf->rho, nf->Cp, nf->kThermal,
+                                                               mat->materialRho[0], dt, dsubgrid, dTsub);
+    FVM_LAUNCH_CHECK(ctx);
   }
   ProjGroup pg;
   memset(&pg, 0, sizeof(pg));
@@ -257,14 +282,44 @@ extern "C" int fvm_subgrid_temperature_changes(fvm_ctx *ctx, const double *thisT
   pg.ea = thisT;
   pg.eb = nf->lastT;
   FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  *dTsub_out = dTsub;
+  *deltaT_out = deltaT;
+  return FVM_OK;
+}
+
+extern "C" int fvm_subgrid_temperature_changes(fvm_ctx *ctx, const double *thisT, const fvm_grid *g,
+                                               const NodalArrays *nf, fvm_markers *mk, const Materials *mat,
+                                               double dt, double dsubgrid) {
+  if (!ctx || !mk || !nf || !mat || !thisT || !nf->lastT || !nf->rho || !nf->Cp || !nf->kThermal)
+    return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  double *dTsub = nullptr, *deltaT = nullptr;
+  FVM_TRY(fvm_subgrid_t_pass1(ctx, thisT, g, nf, mk, mat, dt, dsubgrid, &dTsub, &deltaT));
+  const int32_t n = mk->nMark;
   if (n > 0) {
-    {
-      FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T2);
-      k_subgrid_t2<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->T, mk->Tdot, mk->cellX,
-                                                                 mk->cellY, deltaT, dTsub, dt);
-      FVM_LAUNCH_CHECK(ctx);
-    }
+    FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T2);
+    k_subgrid_t2<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dgrid(g), n, mk->X, mk->Y, mk->T, mk->Tdot, mk->cellX,
+                                                               mk->cellY, deltaT, dTsub, dt);
+    FVM_LAUNCH_CHECK(ctx);
   }
+  return FVM_OK;
+}
+
+// advectMarkersRK with subgridTemperatureChanges pass 2 fused in front of it
+// (both start from the stored cell; T is not read by the advection).
+int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays *nf, const fvm_grid *g,
+                               const BoundaryValues *bv, double dt, const double *deltaT, const double *dTsub,
+                               double dt_thermal) {
+  if (mk->nMark == 0) return FVM_OK;
+  {
+    FVM_PHASE(ctx, FVM_PHASE_ADVECT);
+    k_advect_rk4<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(
+        dgrid(g), mk->nMark, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy,
+        mk->exy, mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, 0, mk->T, mk->Tdot, deltaT,
+        dTsub, dt_thermal);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  fvm_markers_moved(ctx);
   return FVM_OK;
 }
 
@@ -355,22 +410,25 @@ extern "C" int fvm_subgrid_stress_changes(fvm_ctx *ctx, const fvm_grid *g, const
       FVM_LAUNCH_CHECK(ctx);
     }
   }
-  ProjGroup pg;
-  memset(&pg, 0, sizeof(pg));
-  pg.stagx = pg.stagy = 0;
-  pg.method = PM_TIES;
-  pg.nf = 1;
-  pg.src[0].col = dxy;
-  pg.out[0] = nf->dsxy;
-  pg.epilogue = EPI_SUB_FROM;
-  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
-  pg.method = PM_CENTRE;
-  pg.nf = 2;
-  pg.src[0].col = dxx;
-  pg.src[1].col = dyy;
-  pg.out[0] = nf->dsxx;
-  pg.out[1] = nf->dsyy;
-  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  // subgridStressChanges.c:134-175 (sxy, basic nodes, <=/>= 0.5 ties) and
+  // :223-233 (sxx, syy, one cell-centre node); two groups, one node pass
+  ProjGroup pg[2];
+  memset(pg, 0, sizeof(pg));
+  pg[0].stagx = pg[0].stagy = 0;
+  pg[0].method = PM_TIES;
+  pg[0].nf = 1;
+  pg[0].src[0].col = dxy;
+  pg[0].out[0] = nf->dsxy;
+  pg[0].epilogue = EPI_SUB_FROM;
+  pg[1].stagx = pg[1].stagy = 0;
+  pg[1].method = PM_CENTRE;
+  pg[1].nf = 2;
+  pg[1].src[0].col = dxx;
+  pg[1].src[1].col = dyy;
+  pg[1].out[0] = nf->dsxx;
+  pg[1].out[1] = nf->dsyy;
+  pg[1].epilogue = EPI_SUB_FROM;
+  FVM_TRY(fvm_project_groups(ctx, g, mk, 2, pg));
   return FVM_OK;
 }
 
@@ -480,7 +538,9 @@ __global__ void __launch_bounds__(256) k_strain_pressure(DGrid g, int32_t n, con
   e11[m] = t11;
   e22[m] = t22;
   e12[m] = t12;
-  const double r = sqrt(pow(t11 - mexx, 2.0) + pow(t12 - mexy, 2.0));
+  // pow(v, 2.0) folded to v*v as in the reference's gcc -O3 build
+  const double d11 = t11 - mexx, d12 = t12 - mexy;
+  const double r = sqrt(d11 * d11 + d12 * d12);
   srr[m] = r;
   if (r != r) atomicOr(&flags[0], 1);
 }

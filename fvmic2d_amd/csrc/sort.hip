@@ -23,35 +23,80 @@ __device__ __forceinline__ int bisect(const double *__restrict__ x, int N, doubl
   return lo;
 }
 
+// Cell location, fused with what the step does next: the bucket key and its
+// count (when `count` is non-null) and the density loop of main_subduction.c:
+// 255-264 (when `dens` is non-null).  Counts use one atomic per distinct key
+// per wavefront; the slot they return is only a provisional position inside
+// the bucket -- k_bucket_order fixes the final order.
+struct DensityTab {
+  double rho[FVM_MAXMAT], alpha[FVM_MAXMAT];
+};
+
 template <bool LDS>
 __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double *__restrict__ X,
                                                 const double *__restrict__ Y, int32_t *__restrict__ cellX,
                                                 int32_t *__restrict__ cellY, int32_t *__restrict__ cpu, int32_t rank,
-                                                int32_t *__restrict__ counters) {
+                                                int32_t *__restrict__ counters, int32_t *__restrict__ key,
+                                                int32_t *__restrict__ slot, int32_t *__restrict__ count,
+                                                DensityTab dens, int do_dens, const int32_t *__restrict__ Mat,
+                                                const double *__restrict__ T, double *__restrict__ rho,
+                                                double *__restrict__ rhodot) {
   extern __shared__ double sgrid[];
-  const double *gx = g.x, *gy = g.y;
+  const double *gx = g.x, *gy = g.y, *gxc = g.xc, *gyc = g.yc;
   if (LDS) {
+    // x[0..NX-1], y[0..NY-1], xc[0..NX], yc[0..NY]
     for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
     for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
+    for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
+    for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
     __syncthreads();
     gx = sgrid;
     gy = sgrid + g.NX;
+    gxc = sgrid + g.NX + g.NY;
+    gyc = sgrid + 2 * g.NX + g.NY + 1;
   }
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int out = 0, bad = 0;
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int out = 0, bad = 0, k = -1;
   if (m < n) {
     const double xm = X[m], ym = Y[m];
     int cx = -1, cy = -1;
     if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
       cx = bisect(gx, g.NX, xm);
       cy = bisect(gy, g.NY, ym);
-      if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) bad = 1;
+      if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
+        bad = 1;
+      } else {
+        const int bx = xm >= gxc[cx + 1] ? 1 : 0;
+        const int by = ym >= gyc[cy + 1] ? 1 : 0;
+        k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+      }
     } else {
       out = 1;
     }
     cellX[m] = cx;
     cellY[m] = cy;
     cpu[m] = rank;
+    if (do_dens && cx != -1) {
+      const int mm = Mat[m] & (FVM_MAXMAT - 1);
+      rho[m] = -dens.rho[mm] * (dens.alpha[mm] * T[m]);
+      rhodot[m] = 0.0;
+    }
+    if (count) key[m] = k;
+  }
+  if (count) {
+    // one atomic per distinct key in the wavefront
+    const int lane = threadIdx.x & 63;
+    unsigned long long todo = __ballot(k >= 0);
+    while (todo) {
+      const int leader = __ffsll((long long)todo) - 1;
+      const int kl = __shfl(k, leader);
+      const unsigned long long peers = __ballot(k == kl) & todo;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&count[kl], __popcll(peers));
+      base = __shfl(base, leader);
+      if (k == kl) slot[m] = base + __popcll(peers & ((1ull << lane) - 1ull));
+      todo &= ~peers;
+    }
   }
   // block-wide counts of outside markers and bad cells (integer: order-free)
   __shared__ int s_out, s_bad;
@@ -72,68 +117,41 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
   }
 }
 
-static int launch_locate(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank, int32_t *counters) {
-  const DGrid dg = dgrid(g);
-  const int n = mk->nMark;
-  if (n == 0) return FVM_OK;
-  const size_t lds = (size_t)(g->NX + g->NY) * sizeof(double);
-  if (lds <= 48 * 1024) {
-    {
-      FVM_PHASE(ctx, FVM_PHASE_LOCATE);
-      k_locate<true><<<fvm_blocks(n, 256), 256, lds, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
-                                                                     mk->cpu, rank, counters);
-    }
-  } else {
-    {
-      FVM_PHASE(ctx, FVM_PHASE_LOCATE);
-      k_locate<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
-                                                                    mk->cpu, rank, counters);
-    }
-  }
-  FVM_LAUNCH_CHECK(ctx);
-  return FVM_OK;
-}
-
-int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank) {
-  FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
-  int32_t *counters = (int32_t *)ctx->red.ptr;
-  FVM_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(int32_t), ctx->stream));
-  FVM_TRY(launch_locate(ctx, g, mk, rank, counters));
-  fvm_markers_moved(ctx);
-  return FVM_OK;
-}
-
-extern "C" int fvm_find_marker_cells(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank,
-                                     int32_t *nOut) {
-  if (!ctx || !mk) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
-  FVM_TRY(fvm_locate_async(ctx, g, mk, rank));
-  int32_t *counters = (int32_t *)ctx->red.ptr;
-  int32_t *h = (int32_t *)ctx->pinned.ptr;
-  FVM_HIP(ctx, hipMemcpyAsync(h, counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (nOut) *nOut = h[0];
-  if (h[1]) return fvm_set_err(ctx, FVM_ERR_CELL_OUT_OF_BOUNDS, "%d markers located outside the cell range", h[1]);
-  return FVM_OK;
-}
-
-// ---- bucket order --------------------------------------------------------------
+// ---- bucket order ----------------------------------------------------------------
+// Bucket counts of markers whose cells are already known (a projection after a
+// caller-side findMarkerCells): same wavefront-aggregated atomics as k_locate.
 __global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *__restrict__ X,
                                               const double *__restrict__ Y, const int32_t *__restrict__ cellX,
                                               const int32_t *__restrict__ cellY, int32_t *__restrict__ key,
                                               int32_t *__restrict__ slot, int32_t *__restrict__ count) {
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  const int cx = cellX[m], cy = cellY[m];
-  if (!cell_ok(cx, cy, g.NX, g.NY)) {  // inactive (or invalid) markers take no part
-    key[m] = -1;
-    return;
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int k = -1;
+  if (m < n) {
+    const int cx = cellX[m], cy = cellY[m];
+    if (cell_ok(cx, cy, g.NX, g.NY)) {  // inactive (or invalid) markers take no part
+      const int bx = X[m] >= g.xc[cx + 1] ? 1 : 0;
+      const int by = Y[m] >= g.yc[cy + 1] ? 1 : 0;
+      k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+    }
+    key[m] = k;
   }
-  const int bx = X[m] >= g.xc[cx + 1] ? 1 : 0;
-  const int by = Y[m] >= g.yc[cy + 1] ? 1 : 0;
-  const int k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
-  key[m] = k;
-  slot[m] = atomicAdd(&count[k], 1);
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(k >= 0);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int kl = __shfl(k, leader);
+    const unsigned long long peers = __ballot(k == kl) & todo;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&count[kl], __popcll(peers));
+    base = __shfl(base, leader);
+    if (k == kl) slot[m] = base + __popcll(peers & ((1ull << lane) - 1ull));
+    todo &= ~peers;
+  }
+}
+
+__global__ void k_zero_i32(int32_t *__restrict__ a, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 // exclusive scan of count[0..nk) into start[0..nk], start[nk] = total
@@ -204,35 +222,53 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_add(int32_t *__restrict__ out,
 __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__restrict__ key,
                                                  const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
                                                  int32_t *__restrict__ perm) {
-  int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const int k = key[m];
   if (k < 0) return;
   perm[start[k] + slot[m]] = (int32_t)m;
 }
 
-// put each bucket in ascending marker order (buckets hold ~NMX*NMY/4 markers)
-__global__ void __launch_bounds__(256) k_bucket_order(int32_t nk, const int32_t *__restrict__ start,
+// Put every bucket in ascending marker order: one wavefront per basic cell
+// ranks the (usually <= 64) entries of the cell's four buckets; a cell with
+// more entries falls back to an insertion sort per bucket.
+__global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32_t *__restrict__ start,
                                                       int32_t *__restrict__ perm) {
-  int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nk) return;
-  const int s0 = start[b], s1 = start[b + 1];
-  for (int i = s0 + 1; i < s1; i++) {
-    const int32_t v = perm[i];
-    int j = i - 1;
-    while (j >= s0 && perm[j] > v) {
-      perm[j + 1] = perm[j];
-      j--;
+  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (c >= ncell) return;  // whole wavefront exits together
+  const int b0 = start[4 * c], b1 = start[4 * c + 1], b2 = start[4 * c + 2], b3 = start[4 * c + 3];
+  const int b4 = start[4 * c + 4];
+  const int n = b4 - b0;
+  if (n <= 64) {
+    const int pos = b0 + lane;
+    const bool have = lane < n;
+    const int v = have ? perm[pos] : 0x7fffffff;
+    const int bk = (pos >= b1) + (pos >= b2) + (pos >= b3);
+    int rank = 0;
+    for (int j = 0; j < n; j++) {
+      const int vj = __shfl(v, j);
+      const int bj = __shfl(bk, j);
+      rank += (bj == bk && vj < v) ? 1 : 0;
     }
-    perm[j + 1] = v;
+    const int bs = bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3));
+    if (have) perm[bs + rank] = v;
+  } else if (lane < 4) {
+    const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));
+    const int s1 = lane == 0 ? b1 : (lane == 1 ? b2 : (lane == 2 ? b3 : b4));
+    for (int i = s0 + 1; i < s1; i++) {
+      const int32_t v = perm[i];
+      int j = i - 1;
+      while (j >= s0 && perm[j] > v) {
+        perm[j + 1] = perm[j];
+        j--;
+      }
+      perm[j + 1] = v;
+    }
   }
 }
 
-int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
-  if (ctx->sort_epoch == ctx->epoch && ctx->sort_x == (const void *)g->x && ctx->sort_X == (const void *)mk->X &&
-      ctx->sort_n == mk->nMark)
-    return FVM_OK;
-  const int32_t n = mk->nMark;
+static int sort_buffers(fvm_ctx *ctx, const fvm_grid *g, int32_t n) {
   const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
   const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
   FVM_TRY(fvm_ws(ctx, ctx->key, (size_t)(n > 0 ? n : 1) * 4));
@@ -241,48 +277,112 @@ int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
   FVM_TRY(fvm_ws(ctx, ctx->count, (size_t)(nk + 1) * 4));
   FVM_TRY(fvm_ws(ctx, ctx->start, (size_t)(nk + 1) * 4));
   FVM_TRY(fvm_ws(ctx, ctx->blocksums, (size_t)(ntiles + 1) * 4));
+  return FVM_OK;
+}
+
+static int zero_counts(fvm_ctx *ctx, const fvm_grid *g) {
+  const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  k_zero_i32<<<fvm_blocks(nk + 1, 256), 256, 0, ctx->stream>>>((int32_t *)ctx->count.ptr, nk + 1);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+// scan + scatter + in-bucket order, after the counts and provisional slots exist
+static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  const int32_t n = mk->nMark;
+  const int64_t ncell = (int64_t)(g->NX - 1) * (g->NY - 1);
+  const int64_t nk = 4 * ncell;
+  const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
   int32_t *key = (int32_t *)ctx->key.ptr, *slot = (int32_t *)ctx->slot.ptr, *perm = (int32_t *)ctx->perm.ptr;
   int32_t *count = (int32_t *)ctx->count.ptr, *start = (int32_t *)ctx->start.ptr;
   int32_t *sums = (int32_t *)ctx->blocksums.ptr;
-  FVM_HIP(ctx, hipMemsetAsync(count, 0, (size_t)(nk + 1) * 4, ctx->stream));
-  const DGrid dg = dgrid(g);
+  k_scan_tiles<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(count, start, sums, nk);
+  FVM_LAUNCH_CHECK(ctx);
+  k_scan_sums<<<1, 1024, 0, ctx->stream>>>(sums, ntiles, start + nk);
+  FVM_LAUNCH_CHECK(ctx);
+  k_scan_add<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(start, sums, nk);
+  FVM_LAUNCH_CHECK(ctx);
   if (n > 0) {
-    {
-      FVM_PHASE(ctx, FVM_PHASE_SORT);
-      k_keys<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY, key, slot, count);
-      FVM_LAUNCH_CHECK(ctx);
-    }
-  }
-  {
-    FVM_PHASE(ctx, FVM_PHASE_SORT);
-    k_scan_tiles<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(count, start, sums, nk);
+    k_scatter<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, slot, start, perm);
     FVM_LAUNCH_CHECK(ctx);
   }
-  {
-    FVM_PHASE(ctx, FVM_PHASE_SORT);
-    k_scan_sums<<<1, 1024, 0, ctx->stream>>>(sums, ntiles, start + nk);
-    FVM_LAUNCH_CHECK(ctx);
-  }
-  {
-    FVM_PHASE(ctx, FVM_PHASE_SORT);
-    k_scan_add<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(start, sums, nk);
-    FVM_LAUNCH_CHECK(ctx);
-  }
-  if (n > 0) {
-    {
-      FVM_PHASE(ctx, FVM_PHASE_SORT);
-      k_scatter<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, slot, start, perm);
-      FVM_LAUNCH_CHECK(ctx);
-    }
-  }
-  {
-    FVM_PHASE(ctx, FVM_PHASE_SORT);
-    k_bucket_order<<<fvm_blocks(nk, 256), 256, 0, ctx->stream>>>((int32_t)nk, start, perm);
-    FVM_LAUNCH_CHECK(ctx);
-  }
+  k_bucket_order<<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm);
+  FVM_LAUNCH_CHECK(ctx);
   ctx->sort_epoch = ctx->epoch;
   ctx->sort_x = g->x;
   ctx->sort_X = mk->X;
   ctx->sort_n = n;
   return FVM_OK;
+}
+
+int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank, const Materials *dens,
+                     bool build_order) {
+  int32_t *counters = (int32_t *)ctx->red.ptr;
+  FVM_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(int32_t), ctx->stream));
+  const int32_t n = mk->nMark;
+  fvm_markers_moved(ctx);
+  if (build_order) {
+    FVM_TRY(sort_buffers(ctx, g, n));
+    FVM_TRY(zero_counts(ctx, g));
+  }
+  DensityTab dt;
+  for (int i = 0; i < FVM_MAXMAT; i++) {
+    dt.rho[i] = dens ? dens->materialRho[i] : 0.0;
+    dt.alpha[i] = dens ? dens->materialAlpha[i] : 0.0;
+  }
+  if (n > 0) {
+    FVM_PHASE(ctx, FVM_PHASE_LOCATE);
+    const DGrid dg = dgrid(g);
+    int32_t *key = build_order ? (int32_t *)ctx->key.ptr : nullptr;
+    int32_t *slot = build_order ? (int32_t *)ctx->slot.ptr : nullptr;
+    int32_t *count = build_order ? (int32_t *)ctx->count.ptr : nullptr;
+    const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
+    if (lds <= 48 * 1024) {
+      k_locate<true><<<fvm_blocks(n, 256), 256, lds, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
+                                                                     mk->cpu, rank, counters, key, slot, count, dt,
+                                                                     dens ? 1 : 0, mk->Mat, mk->T, mk->rho,
+                                                                     mk->rhodot);
+    } else {
+      k_locate<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
+                                                                    mk->cpu, rank, counters, key, slot, count, dt,
+                                                                    dens ? 1 : 0, mk->Mat, mk->T, mk->rho,
+                                                                    mk->rhodot);
+    }
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  if (build_order) FVM_TRY(finish_sort(ctx, g, mk));
+  return FVM_OK;
+}
+
+extern "C" int fvm_find_marker_cells(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank,
+                                     int32_t *nOut) {
+  if (!ctx || !mk) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_locate_async(ctx, g, mk, rank, nullptr, false));
+  int32_t *counters = (int32_t *)ctx->red.ptr;
+  int32_t *h = (int32_t *)ctx->pinned.ptr;
+  FVM_HIP(ctx, hipMemcpyAsync(h, counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nOut) *nOut = h[0];
+  if (h[1]) return fvm_set_err(ctx, FVM_ERR_CELL_OUT_OF_BOUNDS, "%d markers located outside the cell range", h[1]);
+  return FVM_OK;
+}
+
+int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
+  if (ctx->sort_epoch == ctx->epoch && ctx->sort_x == (const void *)g->x && ctx->sort_X == (const void *)mk->X &&
+      ctx->sort_n == mk->nMark)
+    return FVM_OK;
+  const int32_t n = mk->nMark;
+  FVM_TRY(sort_buffers(ctx, g, n));
+  FVM_TRY(zero_counts(ctx, g));
+  if (n > 0) {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    k_keys<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dgrid(g), n, mk->X, mk->Y, mk->cellX, mk->cellY,
+                                                        (int32_t *)ctx->key.ptr, (int32_t *)ctx->slot.ptr,
+                                                        (int32_t *)ctx->count.ptr);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  return finish_sort(ctx, g, mk);
 }

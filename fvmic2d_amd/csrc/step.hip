@@ -7,15 +7,18 @@ extern "C" int fvm_step_thermal(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk
                                 int32_t nproj, const fvm_projection *list) {
   if (!ctx || !mk || !nf || !mat || !opt || !bv) return FVM_ERR_ARG;
   FVM_TRY(fvm_grid_check(ctx, g));
-  FVM_TRY(fvm_locate_async(ctx, g, mk, 0));                     // :243
-  FVM_TRY(fvm_update_marker_density(ctx, mk, mat));             // :255-264
+  FVM_TRY(fvm_locate_async(ctx, g, mk, 0, mat, true));         // :243 + density :255-264
   if (list)                                                     // :268
     FVM_TRY(fvm_project_marker_fields(ctx, g, mk, nproj, list));
   else
     FVM_TRY(fvm_project_markers_nodes_all2(ctx, g, mk, nf, mat));
-  FVM_TRY(fvm_subgrid_temperature_changes(ctx, nf->thisT, g, nf, mk, mat, dt,  // :415
-                                          opt->subgridTemperatureDiffusivity));
-  FVM_TRY(fvm_advect_markers_rk(ctx, mk, nf, g, bv, dt, 0));   // :435
+  if (!nf->thisT || !nf->lastT || !nf->rho || !nf->Cp || !nf->kThermal || !nf->vx || !nf->vy || !nf->wxy)
+    return fvm_set_err(ctx, FVM_ERR_ARG, "step: missing nodal field");
+  double *dTsub = nullptr, *deltaT = nullptr;                   // :415 subgridTemperatureChanges
+  FVM_TRY(fvm_subgrid_t_pass1(ctx, nf->thisT, g, nf, mk, mat, dt, opt->subgridTemperatureDiffusivity, &dTsub,
+                              &deltaT));
+  // its pass 2 runs inside the advection kernel (:435 advectMarkersRK)
+  FVM_TRY(fvm_advect_with_subgrid_t2(ctx, mk, nf, g, bv, dt, deltaT, dTsub, dt));
   return FVM_OK;
 }
 
@@ -24,8 +27,7 @@ extern "C" int fvm_step_vep(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, co
                             const BoundaryValues *bv, double dt, int32_t *isYielding, double *residual) {
   if (!ctx || !mk || !nf || !host || !mat || !opt || !bv) return FVM_ERR_ARG;
   FVM_TRY(fvm_grid_check(ctx, g));
-  FVM_TRY(fvm_locate_async(ctx, g, mk, 0));                     // :243
-  FVM_TRY(fvm_update_marker_density(ctx, mk, mat));             // :255-264
+  FVM_TRY(fvm_locate_async(ctx, g, mk, 0, mat, true));         // :243 + density :255-264
   int32_t y = 0;
   FVM_TRY(fvm_check_plastic_yielding(ctx, g, mk, mat, dt, &y, opt, 0.0));  // :299
   FVM_TRY(fvm_limit_viscosity(ctx, mk, opt));                   // :300

@@ -85,6 +85,10 @@ _SIGS = {
     "fvm_profile_read": [P, C.POINTER(C.c_double), C.POINTER(I32)],
     "fvm_timer_start": [P],
     "fvm_timer_stop": [P, C.POINTER(C.c_float)],
+    # reference-signature wrappers (fvm_compat.h); the wrappers themselves keep
+    # the reference's names and are bound by callers with their own prototypes
+    "fvm_compat_finalize": [],
+    "fvm_compat_last_status": [],
 }
 _RESTYPES = {"fvm_last_error": C.c_char_p, "fvm_ctx_stream": P, "fvm_ctx_workspace_bytes": C.c_size_t}
 

@@ -35,6 +35,21 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(lib.exported_symbols()), set(names) - set(lib.exported_symbols())
 
 
+COMPAT_NAMES = ["findMarkerCells", "projectMarkersNodesAll2", "projectMarkerFieldToNodes",
+                "projectMarkersNodesFromScalar", "projectNodalFieldToMarkersS", "advectMarkersRK",
+                "checkPlasticYielding", "limitViscosity", "subgridTemperatureChanges",
+                "subgridStressChanges", "updateMarkerStress", "updateMarkerStrainPressure",
+                "updateGlobalStrainRateResidual", "getGlobalStrainRateResidual", "checkConvergence"]
+
+
+def test_reference_named_wrappers_are_exported():
+    txt = open(os.path.join(INC, "fvm_compat.h")).read()
+    for n in COMPAT_NAMES:
+        assert re.search(r"\b%s\s*\(" % n, txt), n
+    h = C.CDLL(lib.LIB_PATH)
+    assert all(hasattr(h, n) for n in COMPAT_NAMES)
+
+
 def test_binding_loads_without_gpu():
     L = lib.load()
     assert L.fvm_last_error(None) == b"no context"

@@ -1,0 +1,133 @@
+"""The reference-signature wrappers (include/fvm_compat.h) against the oracle:
+a driver written for the reference calls findMarkerCells, projectMarkersNodesAll2,
+subgridTemperatureChanges, advectMarkersRK, ... on its host AoS MarkerSet and
+NodalFields, and the GPU path must leave them as the oracle does."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from fvmic2d_amd import abi, lib, workloads
+from helpers import assert_bitwise, assert_close, field_scale
+
+pytestmark = pytest.mark.gpu
+
+P = C.c_void_p
+
+
+class HostVec(C.Structure):
+    _fields_ = [("n", C.c_int32), ("a", C.c_void_p)]
+
+
+NF_FIELDS = ["rho", "rhodot", "etaS", "etaN", "thisT", "lastT", "kThermal", "Cp", "boundaryT",
+             "edotxx", "edotxy", "edotyy", "edotzz", "edotxz", "edotyz", "soxx", "soxy", "soyy",
+             "dsxx", "dsyy", "dsxy", "muS", "muN", "vx", "vy", "p", "eii", "sii", "wxy", "ha",
+             "nodalHeating", "VPTensorC", "VPTensorB", "strainRateResidual"]
+
+
+class NodalFields(C.Structure):
+    _fields_ = [(n, C.c_double if n == "boundaryT" else C.c_void_p) for n in NF_FIELDS]
+
+
+class HostProblem:
+    """The host-side structures a reference driver holds."""
+
+    def __init__(self, w):
+        self.w = w
+        self.markers = np.ascontiguousarray(w.markers.copy())
+        self.pads = [a.copy() for a in (w.xp, w.yp, w.xcp, w.ycp)]
+        g = abi.GridData()
+        g.LX, g.LY, g.NX, g.NY = w.LX, w.LY, w.NX, w.NY
+        g.x, g.y, g.xc, g.yc = [a.ctypes.data + 16 for a in self.pads]
+        self.grid = g
+        self.ms = abi.MarkerSet(len(self.markers), len(self.markers), 0, self.markers.ctypes.data)
+        self.arr, self.vecs = {}, {}
+        self.nf = NodalFields()
+        for n in NF_FIELDS:
+            if n == "boundaryT":
+                continue
+            a = np.ascontiguousarray(w.nodal.get(n, np.zeros((w.NY, w.NX))).copy())
+            v = HostVec(a.size, a.ctypes.data)
+            self.arr[n], self.vecs[n] = a, v
+            setattr(self.nf, n, C.addressof(v))
+
+    def vec(self, n):
+        return C.byref(self.vecs[n])
+
+    def field_ptr(self, name):
+        return P(self.markers.ctypes.data + abi.FIELD_OFFSET[name])
+
+
+@pytest.fixture(scope="module")
+def L():
+    h = lib.load()
+    return h
+
+
+def _rc(x):
+    assert x == 0, x
+
+
+def test_compat_thermal_step(L):
+    w = workloads.convection_small()
+    hp = HostProblem(w)
+    st = oracle.State(w)
+    # reference call order of one step (main_subduction.c:243-435), via the wrappers
+    L.findMarkerCells(C.byref(hp.ms), C.byref(hp.grid))
+    assert L.fvm_compat_last_status() == 0
+    st.find_marker_cells()
+    assert hp.ms.nOut == st.ms.nOut
+    assert_bitwise(hp.markers["cellX"], st.markers["cellX"], "cellX")
+    _rc(L.projectMarkersNodesAll2(C.byref(hp.ms), C.byref(hp.grid), C.byref(hp.nf),
+                                  C.byref(w.materials), C.byref(w.options)))
+    st.project_markers_nodes_all2()
+    for n in ("rho", "etaN", "etaS", "kThermal", "lastT", "Cp", "soxx", "soyy", "soxy", "muS"):
+        assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
+    _rc(L.subgridTemperatureChanges(hp.vec("thisT"), C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms),
+                                    C.byref(w.materials), C.c_double(w.dt), C.byref(w.options)))
+    st.subgrid_temperature(w.dt, w.options.subgridTemperatureDiffusivity)
+    assert_close(hp.markers["T"], st.markers["T"], "T", scale=field_scale(st.markers["T"]))
+    _rc(L.advectMarkersRK(C.byref(hp.ms), C.byref(hp.nf), C.byref(hp.grid), C.byref(w.options),
+                          C.byref(w.bv), C.c_double(w.dt)))
+    st.advect(w.dt)
+    for f in ("X", "Y", "VX", "VY", "wxy"):
+        assert_bitwise(hp.markers[f], st.markers[f], f)
+    # single-field projection named by a pointer into the AoS
+    _rc(L.projectMarkerFieldToNodes(C.byref(hp.grid), C.byref(hp.ms), None, hp.field_ptr("T"),
+                                    hp.vec("ha"), 1, 0, abi.ARITHMETIC))
+    st.nodal["ha"] = np.zeros((w.NY, w.NX))
+    st.project_marker_field("T", "ha", 1, 0, abi.ARITHMETIC)
+    assert_close(hp.arr["ha"], st.nodal["ha"], "T->vy nodes")
+
+
+def test_compat_vep_functions(L):
+    w = workloads.vep_subduction(NX=61, NY=31, NM=3)
+    hp = HostProblem(w)
+    st = oracle.State(w)
+    L.findMarkerCells(C.byref(hp.ms), C.byref(hp.grid))
+    st.find_marker_cells()
+    y = C.c_int32(0)
+    _rc(L.checkPlasticYielding(C.byref(hp.grid), C.byref(hp.ms), C.byref(w.materials), C.c_double(w.dt),
+                               C.byref(y), C.byref(w.options), C.c_double(0.0)))
+    assert y.value == st.check_plastic_yielding(w.dt)
+    L.limitViscosity(C.byref(hp.ms), C.byref(w.options), C.byref(w.materials))
+    st.limit_viscosity()
+    _rc(L.updateMarkerStrainPressure(C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms),
+                                     C.byref(w.materials), C.byref(w.options), C.c_double(w.dt)))
+    st.update_marker_strain_pressure(w.dt)
+    _rc(L.updateGlobalStrainRateResidual(C.byref(hp.grid), C.byref(hp.ms), 0))
+    L.getGlobalStrainRateResidual.restype = C.c_double
+    r = L.getGlobalStrainRateResidual(0)
+    assert abs(r - st.strain_rate_residual()) <= 1e-12 * abs(r)
+    _rc(L.subgridStressChanges(C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms), C.byref(w.materials),
+                               C.c_double(w.dt), C.c_double(1.0)))
+    st.subgrid_stress(w.dt, 1.0)
+    _rc(L.updateMarkerStress(C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms), C.byref(w.materials)))
+    st.update_marker_stress()
+    act = st.markers["cellX"] != -1
+    for f in ("eta", "mu", "sxx", "syy", "sxy", "exx", "eyy", "exy", "p", "wxy", "strainRateResidual"):
+        assert_close(hp.markers[f][act], st.markers[f][act], f, scale=field_scale(st.markers[f][act]))
+    for n in ("dsxx", "dsyy", "dsxy"):
+        assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
+    L.fvm_compat_finalize()
