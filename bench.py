@@ -71,6 +71,98 @@ def cpu_baseline(budget_s=12.0):
             "cpu": platform.processor() or platform.machine()}
 
 
+def phase_profile(ctx, step, nsteps):
+    """Per-phase ms per step and launches per step, HIP events around every launch."""
+    import ctypes as C
+    msp = (C.c_double * len(PHASES))()
+    lau = (C.c_int32 * len(PHASES))()
+    ctx.call("fvm_profile_enable", C.c_int32(1))
+    for _ in range(nsteps):
+        step()
+    ctx.call("fvm_profile_read", msp, lau)
+    ctx.call("fvm_profile_enable", C.c_int32(0))
+    ms = {PHASES[i]: msp[i] / nsteps for i in range(len(PHASES)) if lau[i]}
+    la = {PHASES[i]: lau[i] // nsteps for i in range(len(PHASES)) if lau[i]}
+    return ms, la
+
+
+def bench_ensemble(a):
+    """The batched ensemble: 64 convection_tdep-style instances, grids 65^2..257^2,
+    16-36 markers per cell, viscosity contrast 1e2-1e6, against the same
+    instances stepped one fvm_step_thermal call after another."""
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import Ensemble, MarkerPath
+    from fvmic2d_amd.lib import Context
+
+    t0 = time.perf_counter()
+    ws = []
+    for i in range(0, a.instances, 8):
+        ws += workloads.ensemble(min(8, a.instances - i), seed=4, nmin=65, nmax=257, first=i)
+        print("ensemble: generated %d instances (%.0f s)" % (len(ws), time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
+    ctx = Context()
+    paths = []
+    for w in ws:
+        paths.append(MarkerPath(w, ctx=ctx))
+        w.markers = None  # host copy no longer needed
+    ctx.sync()
+    n0 = sum(p.markers.n for p in paths)
+    print("ensemble: %d markers on the GPU" % n0, file=sys.stderr, flush=True)
+
+    def seq_step():
+        for p in paths:
+            p.step_thermal()
+    for _ in range(a.warmup):
+        seq_step()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(a.steps):
+        seq_step()
+    seq_ms = ctx.timer_stop() / a.steps
+    print("ensemble: sequential %.3f ms/step" % seq_ms, file=sys.stderr, flush=True)
+
+    ens = Ensemble(paths)
+    (block, lanes), tuned_ms = ens.autotune(reps=3)
+    print("ensemble: autotuned block=%d lanes=%d (%.3f ms)" % (block, lanes, tuned_ms),
+          file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        ens.launch()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(a.steps):
+        ens.launch()
+    ens_ms = ctx.timer_stop() / a.steps
+    nout, nbad = ens.counters()
+    active = n0 - nout
+    phase_ms, phase_la = phase_profile(ctx, ens.step, 3)
+    step_bytes = n0 * STEP_BYTES_PER_MARKER + sum(w.NX * w.NY for w in ws) * STEP_BYTES_PER_NODE
+    nodes = sum(w.NX * w.NY for w in ws)
+    ab = {"advect": n0 * (96 + 88) + nodes * 32}
+    adv_ms = phase_ms["advect"] / phase_la["advect"]
+    achieved = ab["advect"] / (adv_ms / 1e3) / 1e9
+    value = active / (ens_ms / 1e3)
+    out = {
+        "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ens_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ensemble%d" % len(ws), "grids": "%dx%d..%dx%d" % (
+            min(w.NX for w in ws), min(w.NX for w in ws), max(w.NX for w in ws), max(w.NX for w in ws)),
+            "markers": n0, "tuning": {"block": block, "lanes": lanes}, "graph": True},
+        "sequential": {"ms_per_step": seq_ms, "markers_per_s": active / (seq_ms / 1e3)},
+        "speedup_vs_sequential": seq_ms / ens_ms,
+        "roofline": {"bound": "hbm", "kernel": "advect", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": ab["advect"], "avg_launch_ms": adv_ms},
+        "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ens_ms / 1e3) / 1e9,
+                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_la,
+                 "bad_cells": nbad},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    ens.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,16 +170,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--ensemble", action="store_true",
+                    help="time the batched 64-instance ensemble instead (own JSON line)")
+    ap.add_argument("--instances", type=int, default=64)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
               "and this build has no multi-GPU path (see DESIGN.md for how it would shard).",
               file=sys.stderr)
         return 2
+    if a.ensemble:
+        return bench_ensemble(a)
 
     from fvmic2d_amd import workloads
     from fvmic2d_amd.host import MarkerPath
-    import ctypes as C
 
     w = workloads.convection_tdep()
     mp = MarkerPath(w)
@@ -106,17 +202,7 @@ def main():
     value = active * a.steps / (ms / 1e3)
 
     # per-kernel timing pass: every launch bracketed by HIP events on the same stream
-    np_ = 10
-    msp = (C.c_double * np_)()
-    lau = (C.c_int32 * np_)()
-    prof_steps = 5
-    mp.ctx.call("fvm_profile_enable", C.c_int32(1))
-    for _ in range(prof_steps):
-        mp.step_thermal()
-    mp.ctx.call("fvm_profile_read", msp, lau)
-    mp.ctx.call("fvm_profile_enable", C.c_int32(0))
-    phase_ms = {PHASES[i]: msp[i] / prof_steps for i in range(np_) if lau[i]}
-    phase_launches = {PHASES[i]: lau[i] // prof_steps for i in range(np_) if lau[i]}
+    phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_thermal, 5)
     ab = algorithmic_bytes(w)
     cands = {k: v for k, v in phase_ms.items() if ab.get(k)}
     dom = max(cands, key=cands.get)

@@ -1,0 +1,620 @@
+// Batched ensemble of independent marker problems (uneven grids and marker
+// counts): every phase of the thermal marker step runs as ONE launch over all
+// instances, and the whole step can be captured as a hipGraph.
+//
+// There is no reference counterpart: the reference runs parameter sweeps as
+// separate processes (the Monte-Carlo loop of main_subduction.c:167-172).
+// Each instance follows the single-instance step exactly (same device bodies,
+// same bucket order), so with the same lane-group width an instance stepped
+// here is bitwise identical to the same instance stepped alone.
+//
+// Work decomposition: for each kind of work (markers, lattice cells of one
+// projection group, nodes) the instances' ranges are padded to whole blocks,
+// so every block belongs to exactly one instance; a block finds its instance
+// by a uniform binary search over the block offsets.
+#include <vector>
+
+#include "marker_common.h"
+#include "proj_common.h"
+
+int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);
+
+struct DensTab {
+  double rho[FVM_MAXMAT], alpha[FVM_MAXMAT];
+};
+
+struct InstDev {
+  DGrid g;
+  int32_t nMark;
+  int64_t mark_off, bucket_off, node_off;
+  double *X, *Y, *VX, *VY, *T, *Tdot, *sxx, *syy, *sxy, *exx, *eyy, *exy, *Eii, *wxy, *rho, *rhodot;
+  int32_t *Mat, *cellX, *cellY, *cpu;
+  const double *vx, *vy, *wxyn, *thisT, *lastT, *rhon, *Cpn, *kn;
+  DensTab dens;
+  double rho0;  // materialRho[0] (subgridTemperatureChanges.c:62)
+  BoundaryValues bv;
+  double dt, dsub;
+};
+
+struct fvm_ensemble {
+  int n = 0, ng = 0;
+  fvm_tuning tune = {256, 16};
+  std::vector<fvm_instance> inst;
+  std::vector<InstDev> hinst;
+  std::vector<ProjGroup> hgroups;   // [group][instance]; group ng is the dTsub projection
+  std::vector<int> gstag_x, gstag_y, gmethod, gnf;
+  int64_t nmark = 0, nbucket = 0, nnode = 0, ncell = 0;
+  // device memory
+  InstDev *dinst = nullptr;
+  ProjGroup *dgroups = nullptr;
+  NodeGroup *dnodes = nullptr;        // [group][instance]
+  int64_t *dblk = nullptr;            // block offsets: markers, nodes, then one row per group
+  std::vector<int64_t> hblk;
+  int64_t *dpartoff = nullptr;        // [group][instance] offsets into part
+  int32_t *key = nullptr, *slot = nullptr, *perm = nullptr, *count = nullptr, *start = nullptr,
+          *sums = nullptr, *counters = nullptr;
+  double *part = nullptr, *dTsub = nullptr, *deltaT = nullptr;
+  size_t lds_locate = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+// ---- block -> instance ----------------------------------------------------------------
+__device__ __forceinline__ int find_inst(const int64_t *__restrict__ blk, int n, int64_t b) {
+  int lo = 0, hi = n;  // blk[lo] <= b < blk[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (blk[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// ---- kernels ------------------------------------------------------------------------------
+__global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ insts, int n,
+                                                 const int64_t *__restrict__ blk, int32_t *__restrict__ key,
+                                                 int32_t *__restrict__ slot, int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ counters) {
+  extern __shared__ double sgrid[];
+  const int ii = find_inst(blk, n, blockIdx.x);
+  const InstDev &I = insts[ii];
+  const DGrid &g = I.g;
+  for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
+  for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
+  for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
+  for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
+  __syncthreads();
+  const double *gx = sgrid, *gy = sgrid + g.NX, *gxc = sgrid + g.NX + g.NY, *gyc = sgrid + 2 * g.NX + g.NY + 1;
+  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  int out = 0, bad = 0, k = -1;
+  if (m < I.nMark) {
+    const double xm = I.X[m], ym = I.Y[m];
+    int cx = -1, cy = -1;
+    if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
+      cx = bisect_grid(gx, g.NX, xm);
+      cy = bisect_grid(gy, g.NY, ym);
+      if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
+        bad = 1;
+      } else {
+        const int bx = xm >= gxc[cx + 1] ? 1 : 0;
+        const int by = ym >= gyc[cy + 1] ? 1 : 0;
+        k = (int)I.bucket_off + (((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1));
+      }
+    } else {
+      out = 1;
+    }
+    I.cellX[m] = cx;
+    I.cellY[m] = cy;
+    I.cpu[m] = 0;
+    if (cx != -1) {  // main_subduction.c:255-264
+      const int mm = I.Mat[m] & (FVM_MAXMAT - 1);
+      I.rho[m] = -I.dens.rho[mm] * (I.dens.alpha[mm] * I.T[m]);
+      I.rhodot[m] = 0.0;
+    }
+    key[I.mark_off + m] = k;
+  }
+  const int sl = wave_count_slot(k, count);
+  if (k >= 0) slot[I.mark_off + m] = sl;
+  const int lane = threadIdx.x & 63;
+  const unsigned long long bo = __ballot(out), bb = __ballot(bad);
+  if (lane == 0) {
+    if (bo) atomicAdd(&counters[0], __popcll(bo));
+    if (bb) atomicAdd(&counters[1], __popcll(bb));
+  }
+}
+
+__global__ void __launch_bounds__(512) kb_scatter(const InstDev *__restrict__ insts, int n,
+                                                  const int64_t *__restrict__ blk, const int32_t *__restrict__ key,
+                                                  const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
+                                                  int32_t *__restrict__ perm) {
+  const int ii = find_inst(blk, n, blockIdx.x);
+  const InstDev &I = insts[ii];
+  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= I.nMark) return;
+  const int k = key[I.mark_off + m];
+  if (k < 0) return;
+  perm[start[k] + slot[I.mark_off + m]] = (int32_t)m;
+}
+
+template <int METHOD, int NF, int G>
+__global__ void __launch_bounds__(512) kb_partials(const InstDev *__restrict__ insts, int n,
+                                                   const int64_t *__restrict__ blk, const ProjGroup *__restrict__ groups,
+                                                   const int64_t *__restrict__ partoff, const int32_t *__restrict__ start,
+                                                   const int32_t *__restrict__ perm, double *__restrict__ part, int sx,
+                                                   int sy) {
+  const int ii = find_inst(blk, n, blockIdx.x);
+  const InstDev &I = insts[ii];
+  const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
+  const int64_t nL = (int64_t)Lx.n * Ly.n;
+  const int64_t lt = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = lt / G;
+  const int j = (int)(lt % G);
+  const bool live = t < nL;
+  const int lx = live ? (int)(t % Lx.n) + Lx.lo : 0;
+  const int ly = live ? (int)(t / Lx.n) + Ly.lo : 0;
+  partials_cell<METHOD, NF, G>(I.g, groups[ii], start + I.bucket_off, perm, I.X, I.Y, I.Mat, part + partoff[ii], nL,
+                               t, lx, ly, j, live);
+}
+
+__global__ void __launch_bounds__(256) kb_nodes(const InstDev *__restrict__ insts, int n,
+                                                const int64_t *__restrict__ blk, const NodeGroup *__restrict__ nodes,
+                                                int g0, int g1) {
+  const int ii = find_inst(blk, n, blockIdx.x);
+  const InstDev &I = insts[ii];
+  const int64_t i = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)I.g.NX * I.g.NY) return;
+  const int ix = (int)(i % I.g.NX), jy = (int)(i / I.g.NX);
+  for (int gi = g0; gi < g1; gi++) node_group(nodes[(int64_t)gi * n + ii], ix, jy, i);
+}
+
+__global__ void __launch_bounds__(512) kb_subgrid_t1(const InstDev *__restrict__ insts, int n,
+                                                     const int64_t *__restrict__ blk, double *__restrict__ dTsub) {
+  const int ii = find_inst(blk, n, blockIdx.x);
+  const InstDev &I = insts[ii];
+  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= I.nMark) return;
+  subgrid_t1_marker(m, I.g, I.X, I.Y, I.T, I.cellX, I.cellY, I.lastT, I.rhon, I.Cpn, I.kn, I.rho0, I.dt, I.dsub,
+                    dTsub + I.mark_off);
+}
+
+__global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ insts, int n,
+                                                 const int64_t *__restrict__ blk, const double *__restrict__ deltaT,
+                                                 const double *__restrict__ dTsub) {
+  const int ii = find_inst(blk, n, blockIdx.x);
+  const InstDev &I = insts[ii];
+  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= I.nMark) return;
+  advect_marker(m, I.g, I.X, I.Y, I.VX, I.VY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.cellX,
+                I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, 0, I.T, I.Tdot, deltaT + I.node_off, dTsub + I.mark_off,
+                I.dt);
+}
+
+// ---- host side ----------------------------------------------------------------------------
+// block offsets rows: 0 markers (tune.block), 1 nodes (256), 2 + g lattice cells of group g
+static int64_t *blk_row(fvm_ensemble *E, int r) { return E->dblk + (int64_t)r * (E->n + 1); }
+
+static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
+  const int n = E->n, rows = 2 + E->ng + 1;
+  E->hblk.assign((size_t)rows * (n + 1), 0);
+  const int B = E->tune.block, G = E->tune.lanes;
+  for (int i = 0; i < n; i++) {
+    const InstDev &I = E->hinst[i];
+    const int64_t nm = I.nMark, nn = (int64_t)I.g.NX * I.g.NY;
+    E->hblk[0 * (n + 1) + i + 1] = E->hblk[0 * (n + 1) + i] + (nm + B - 1) / B;
+    E->hblk[1 * (n + 1) + i + 1] = E->hblk[1 * (n + 1) + i] + (nn + 255) / 256;
+    for (int gi = 0; gi <= E->ng; gi++) {
+      const Lat Lx = lattice(E->gstag_x[gi], I.g.NX), Ly = lattice(E->gstag_y[gi], I.g.NY);
+      const int64_t th = (int64_t)Lx.n * Ly.n * G;
+      const int r = 2 + gi;
+      E->hblk[r * (n + 1) + i + 1] = E->hblk[r * (n + 1) + i] + (th + 255) / 256;
+    }
+  }
+  FVM_HIP(ctx, hipMemcpyAsync(E->dblk, E->hblk.data(), E->hblk.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+template <int METHOD, int NF>
+static int launch_kb_partials(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
+  const int n = E->n;
+  const int64_t nblk = E->hblk[(2 + gi) * (n + 1) + n];
+  if (nblk == 0) return FVM_OK;
+  const ProjGroup *groups = E->dgroups + (int64_t)gi * n;
+  const int64_t *po = E->dpartoff + (int64_t)gi * n;
+  const int64_t *blk = blk_row(E, 2 + gi);
+  const int sx = E->gstag_x[gi], sy = E->gstag_y[gi];
+  FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
+  switch (E->tune.lanes) {
+    case 8:
+      kb_partials<METHOD, NF, 8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start,
+                                                                          E->perm, E->part, sx, sy);
+      break;
+    case 32:
+      kb_partials<METHOD, NF, 32><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start,
+                                                                           E->perm, E->part, sx, sy);
+      break;
+    default:
+      kb_partials<METHOD, NF, 16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start,
+                                                                           E->perm, E->part, sx, sy);
+  }
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+template <int METHOD>
+static int kb_partials_nf(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
+  switch (E->gnf[gi]) {
+    case 1: return launch_kb_partials<METHOD, 1>(ctx, E, gi);
+    case 2: return launch_kb_partials<METHOD, 2>(ctx, E, gi);
+    case 3: return launch_kb_partials<METHOD, 3>(ctx, E, gi);
+    case 4: return launch_kb_partials<METHOD, 4>(ctx, E, gi);
+    case 5: return launch_kb_partials<METHOD, 5>(ctx, E, gi);
+    case 6: return launch_kb_partials<METHOD, 6>(ctx, E, gi);
+    case 7: return launch_kb_partials<METHOD, 7>(ctx, E, gi);
+    case 8: return launch_kb_partials<METHOD, 8>(ctx, E, gi);
+    default: return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "ensemble group with %d fields", E->gnf[gi]);
+  }
+}
+
+static int kb_partials_dispatch(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
+  if (E->gmethod[gi] == PM_ARITH) return kb_partials_nf<PM_ARITH>(ctx, E, gi);
+  if (E->gmethod[gi] == PM_LOCAL) return kb_partials_nf<PM_LOCAL>(ctx, E, gi);
+  return fvm_set_err(ctx, FVM_ERR_BAD_METHOD, "ensemble projections support ARITHMETIC and ARITHMETIC_LOCAL");
+}
+
+static int launch_kb_nodes(fvm_ctx *ctx, fvm_ensemble *E, int g0, int g1) {
+  const int n = E->n;
+  const int64_t nblk = E->hblk[1 * (n + 1) + n];
+  if (nblk == 0) return FVM_OK;
+  FVM_PHASE(ctx, FVM_PHASE_NODES);
+  kb_nodes<<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1), E->dnodes, g0, g1);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
+  const int n = E->n;
+  const int64_t mblk = E->hblk[0 * (n + 1) + n];
+  const int B = E->tune.block;
+  // findMarkerCells + density loop + bucket keys (main_subduction.c:243, 255-264)
+  FVM_TRY(fvm_launch_zero_i32(ctx, E->count, E->nbucket + 1));
+  FVM_TRY(fvm_launch_zero_i32(ctx, E->counters, 2));
+  if (mblk) {
+    FVM_PHASE(ctx, FVM_PHASE_LOCATE);
+    kb_locate<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->key, E->slot,
+                                                                  E->count, E->counters);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  FVM_TRY(fvm_launch_scan(ctx, E->count, E->start, E->sums, E->nbucket));
+  if (mblk) {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    kb_scatter<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->key, E->slot, E->start, E->perm);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  FVM_TRY(fvm_launch_bucket_order(ctx, E->ncell, E->start, E->perm));
+  // projections (:268)
+  for (int gi = 0; gi < E->ng; gi++) FVM_TRY(kb_partials_dispatch(ctx, E, gi));
+  FVM_TRY(launch_kb_nodes(ctx, E, 0, E->ng));
+  // subgridTemperatureChanges (:415): pass 1, dTsub projection with the VecAXPBYPCZ epilogue
+  if (mblk) {
+    FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
+    kb_subgrid_t1<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->dTsub);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  FVM_TRY(kb_partials_dispatch(ctx, E, E->ng));
+  FVM_TRY(launch_kb_nodes(ctx, E, E->ng, E->ng + 1));
+  // pass 2 fused into advectMarkersRK (:435)
+  if (mblk) {
+    FVM_PHASE(ctx, FVM_PHASE_ADVECT);
+    kb_advect<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->deltaT, E->dTsub);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  return FVM_OK;
+}
+
+static void free_dev(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+extern "C" int fvm_ensemble_destroy(fvm_ensemble *E) {
+  if (!E) return FVM_OK;
+  if (E->exec) (void)hipGraphExecDestroy(E->exec);
+  if (E->graph) (void)hipGraphDestroy(E->graph);
+  void *bufs[] = {E->dinst, E->dgroups, E->dnodes, E->dblk, E->dpartoff, E->key,   E->slot,  E->perm,
+                  E->count, E->start,   E->sums,   E->counters, E->part, E->dTsub, E->deltaT};
+  for (void *b : bufs) free_dev(b);
+  delete E;
+  return FVM_OK;
+}
+
+#define ENS_ALLOC(ctx, ptr, bytes)                                                      \
+  do {                                                                                   \
+    hipError_t e_ = hipMalloc((void **)&(ptr), (bytes) > 0 ? (bytes) : 8);             \
+    if (e_ != hipSuccess) {                                                              \
+      fvm_ensemble_destroy(E);                                                           \
+      return fvm_set_err(ctx, FVM_ERR_NOMEM, "ensemble hipMalloc: %s", hipGetErrorString(e_)); \
+    }                                                                                    \
+  } while (0)
+
+extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *inst, fvm_ensemble **out) {
+  if (!ctx || !inst || !out || n < 1) return FVM_ERR_ARG;
+  // projection groups from instance 0's list; every instance must have the same pattern
+  const fvm_instance &I0 = inst[0];
+  if (I0.nproj < 1 || !I0.proj || I0.nproj > 64) return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: projection list");
+  for (int i = 0; i < n; i++) {
+    FVM_TRY(fvm_grid_check(ctx, &inst[i].grid));
+    if (inst[i].nproj != I0.nproj || !inst[i].proj)
+      return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: instance %d has a different projection list", i);
+    for (int p = 0; p < I0.nproj; p++) {
+      const fvm_projection &a = I0.proj[p], &b = inst[i].proj[p];
+      if (a.stagx != b.stagx || a.stagy != b.stagy || a.method != b.method ||
+          (a.materialProperty == nullptr) != (b.materialProperty == nullptr))
+        return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: instance %d projection %d differs in kind", i, p);
+    }
+    const NodalArrays &nf = inst[i].nodal;
+    if (!nf.vx || !nf.vy || !nf.wxy || !nf.thisT || !nf.lastT || !nf.rho || !nf.Cp || !nf.kThermal)
+      return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: instance %d lacks a nodal field", i);
+  }
+  fvm_ensemble *E = new fvm_ensemble();
+  E->n = n;
+  E->inst.assign(inst, inst + n);
+  // group assignment: same rule as fvm_project_marker_fields (first appearance order)
+  std::vector<int> gof(I0.nproj, -1), slot(I0.nproj, 0);
+  for (int p = 0; p < I0.nproj; p++) {
+    if (gof[p] >= 0) continue;
+    const int gi = E->ng++;
+    E->gstag_x.push_back(I0.proj[p].stagx);
+    E->gstag_y.push_back(I0.proj[p].stagy);
+    const int m = I0.proj[p].method;
+    E->gmethod.push_back(m == ARITHMETIC ? PM_ARITH : (m == ARITHMETIC_LOCAL ? PM_LOCAL : PM_HARM));
+    int nf = 0;
+    for (int q = p; q < I0.nproj; q++) {
+      const fvm_projection &b = I0.proj[q];
+      if (gof[q] >= 0 || b.stagx != I0.proj[p].stagx || b.stagy != I0.proj[p].stagy || b.method != m) continue;
+      if (nf == FVM_MAXF) break;
+      gof[q] = gi;
+      slot[q] = nf++;
+    }
+    E->gnf.push_back(nf);
+  }
+  // the dTsub projection group (subgridTemperatureChanges.c:74)
+  E->gstag_x.push_back(0);
+  E->gstag_y.push_back(0);
+  E->gmethod.push_back(PM_ARITH);
+  E->gnf.push_back(1);
+  const int NG = E->ng + 1;
+
+  // per-instance device views and offsets
+  E->hinst.resize(n);
+  size_t max_lds = 0;
+  std::vector<int64_t> partoff((size_t)NG * n);
+  int64_t parttot = 0;
+  for (int i = 0; i < n; i++) {
+    const fvm_instance &S = inst[i];
+    InstDev &D = E->hinst[i];
+    memset(&D, 0, sizeof(D));
+    D.g = dgrid(&S.grid);
+    D.nMark = S.markers.nMark;
+    D.mark_off = E->nmark;
+    D.bucket_off = E->nbucket;
+    D.node_off = E->nnode;
+    const fvm_markers &M = S.markers;
+    D.X = M.X, D.Y = M.Y, D.VX = M.VX, D.VY = M.VY, D.T = M.T, D.Tdot = M.Tdot;
+    D.sxx = M.sxx, D.syy = M.syy, D.sxy = M.sxy, D.exx = M.exx, D.eyy = M.eyy, D.exy = M.exy;
+    D.Eii = M.Eii, D.wxy = M.wxy, D.rho = M.rho, D.rhodot = M.rhodot;
+    D.Mat = M.Mat, D.cellX = M.cellX, D.cellY = M.cellY, D.cpu = M.cpu;
+    D.vx = S.nodal.vx, D.vy = S.nodal.vy, D.wxyn = S.nodal.wxy, D.thisT = S.nodal.thisT;
+    D.lastT = S.nodal.lastT, D.rhon = S.nodal.rho, D.Cpn = S.nodal.Cp, D.kn = S.nodal.kThermal;
+    for (int k = 0; k < FVM_MAXMAT; k++) {
+      D.dens.rho[k] = S.materials.materialRho[k];
+      D.dens.alpha[k] = S.materials.materialAlpha[k];
+    }
+    D.rho0 = S.materials.materialRho[0];
+    D.bv = S.bv;
+    D.dt = S.dt;
+    D.dsub = S.dsubgrid;
+    const int64_t ncell = (int64_t)(S.grid.NX - 1) * (S.grid.NY - 1);
+    E->nmark += S.markers.nMark;
+    E->nbucket += 4 * ncell;
+    E->ncell += ncell;
+    E->nnode += (int64_t)S.grid.NX * S.grid.NY;
+    max_lds = std::max(max_lds, (size_t)(2 * S.grid.NX + 2 * S.grid.NY + 2) * sizeof(double));
+    for (int gi = 0; gi < NG; gi++) {
+      const Lat Lx = lattice(E->gstag_x[gi], S.grid.NX), Ly = lattice(E->gstag_y[gi], S.grid.NY);
+      partoff[(size_t)gi * n + i] = parttot;
+      parttot += (int64_t)Lx.n * Ly.n * 4 * (E->gnf[gi] + 1);
+    }
+  }
+  if (max_lds > 64 * 1024) {
+    fvm_ensemble_destroy(E);
+    return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: grid too large for the batched location kernel");
+  }
+  E->lds_locate = max_lds;
+  const int64_t ntiles = fvm_scan_tiles(E->nbucket);
+  ENS_ALLOC(ctx, E->dinst, sizeof(InstDev) * n);
+  ENS_ALLOC(ctx, E->dgroups, sizeof(ProjGroup) * NG * n);
+  ENS_ALLOC(ctx, E->dnodes, sizeof(NodeGroup) * NG * n);
+  ENS_ALLOC(ctx, E->dblk, sizeof(int64_t) * (2 + NG) * (n + 1));
+  ENS_ALLOC(ctx, E->dpartoff, sizeof(int64_t) * NG * n);
+  ENS_ALLOC(ctx, E->key, sizeof(int32_t) * E->nmark);
+  ENS_ALLOC(ctx, E->slot, sizeof(int32_t) * E->nmark);
+  ENS_ALLOC(ctx, E->perm, sizeof(int32_t) * E->nmark);
+  ENS_ALLOC(ctx, E->count, sizeof(int32_t) * (E->nbucket + 1));
+  ENS_ALLOC(ctx, E->start, sizeof(int32_t) * (E->nbucket + 1));
+  ENS_ALLOC(ctx, E->sums, sizeof(int32_t) * (ntiles + 1));
+  ENS_ALLOC(ctx, E->counters, sizeof(int32_t) * 4);
+  ENS_ALLOC(ctx, E->part, sizeof(double) * parttot);
+  ENS_ALLOC(ctx, E->dTsub, sizeof(double) * E->nmark);
+  ENS_ALLOC(ctx, E->deltaT, sizeof(double) * E->nnode);
+
+  // projection and node groups per (group, instance)
+  E->hgroups.assign((size_t)NG * n, ProjGroup());
+  std::vector<NodeGroup> hnodes((size_t)NG * n);
+  for (int i = 0; i < n; i++) {
+    const fvm_instance &S = inst[i];
+    for (int gi = 0; gi < NG; gi++) {
+      ProjGroup &pg = E->hgroups[(size_t)gi * n + i];
+      memset(&pg, 0, sizeof(pg));
+      pg.stagx = E->gstag_x[gi];
+      pg.stagy = E->gstag_y[gi];
+      pg.method = E->gmethod[gi];
+      pg.nf = E->gnf[gi];
+      pg.epilogue = EPI_DIVIDE;
+    }
+    for (int p = 0; p < I0.nproj; p++) {
+      ProjGroup &pg = E->hgroups[(size_t)gof[p] * n + i];
+      ProjSrc &s = pg.src[slot[p]];
+      const fvm_projection &b = S.proj[p];
+      if (b.materialProperty) {
+        s.use_mat = 1;
+        for (int k = 0; k < FVM_MAXMAT; k++) s.mat[k] = b.materialProperty[k];
+      } else {
+        s.col = b.markerField;
+      }
+      pg.out[slot[p]] = b.nodalField;
+    }
+    ProjGroup &dg = E->hgroups[(size_t)E->ng * n + i];
+    dg.src[0].col = E->dTsub + E->hinst[i].mark_off;
+    dg.out[0] = E->deltaT + E->hinst[i].node_off;
+    dg.epilogue = EPI_SUBGRID_T;
+    dg.ea = S.nodal.thisT;
+    dg.eb = S.nodal.lastT;
+    for (int gi = 0; gi < NG; gi++) {
+      const ProjGroup &pg = E->hgroups[(size_t)gi * n + i];
+      NodeGroup &q = hnodes[(size_t)gi * n + i];
+      memset(&q, 0, sizeof(q));
+      const Lat Lx = lattice(pg.stagx, S.grid.NX), Ly = lattice(pg.stagy, S.grid.NY);
+      q.part = E->part + partoff[(size_t)gi * n + i];
+      q.nf = pg.nf;
+      q.harm = pg.method == PM_HARM;
+      q.nLx = Lx.n;
+      q.nLy = Ly.n;
+      q.lox = Lx.lo;
+      q.loy = Ly.lo;
+      q.epilogue = pg.epilogue;
+      for (int k = 0; k < pg.nf; k++) q.out[k] = pg.out[k];
+      q.ea = pg.ea;
+      q.eb = pg.eb;
+    }
+  }
+  FVM_HIP(ctx, hipMemcpyAsync(E->dinst, E->hinst.data(), sizeof(InstDev) * n, hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dgroups, E->hgroups.data(), sizeof(ProjGroup) * NG * n, hipMemcpyHostToDevice,
+                              ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dnodes, hnodes.data(), sizeof(NodeGroup) * NG * n, hipMemcpyHostToDevice,
+                              ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dpartoff, partoff.data(), sizeof(int64_t) * NG * n, hipMemcpyHostToDevice,
+                              ctx->stream));
+  int rc = build_blocks(ctx, E);
+  if (rc) {
+    fvm_ensemble_destroy(E);
+    return rc;
+  }
+  *out = E;
+  return FVM_OK;
+}
+
+extern "C" int fvm_ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
+  if (!ctx || !E) return FVM_ERR_ARG;
+  return ensemble_step(ctx, E);
+}
+
+extern "C" int fvm_ensemble_set_tuning(fvm_ctx *ctx, fvm_ensemble *E, const fvm_tuning *t) {
+  if (!ctx || !E || !t) return FVM_ERR_ARG;
+  if ((t->block != 128 && t->block != 256 && t->block != 512) || (t->lanes != 8 && t->lanes != 16 && t->lanes != 32))
+    return fvm_set_err(ctx, FVM_ERR_ARG, "tuning: block in {128,256,512}, lanes in {8,16,32}");
+  if (E->tune.block == t->block && E->tune.lanes == t->lanes) return FVM_OK;
+  E->tune = *t;
+  if (E->exec) {  // a captured graph holds the old launch shapes
+    (void)hipGraphExecDestroy(E->exec);
+    E->exec = nullptr;
+  }
+  if (E->graph) {
+    (void)hipGraphDestroy(E->graph);
+    E->graph = nullptr;
+  }
+  return build_blocks(ctx, E);
+}
+
+extern "C" int fvm_ensemble_get_tuning(const fvm_ensemble *E, fvm_tuning *t) {
+  if (!E || !t) return FVM_ERR_ARG;
+  *t = E->tune;
+  return FVM_OK;
+}
+
+extern "C" int64_t fvm_ensemble_markers(const fvm_ensemble *E) { return E ? E->nmark : 0; }
+
+extern "C" int fvm_ensemble_capture(fvm_ctx *ctx, fvm_ensemble *E) {
+  if (!ctx || !E) return FVM_ERR_ARG;
+  if (E->exec) {
+    (void)hipGraphExecDestroy(E->exec);
+    E->exec = nullptr;
+  }
+  if (E->graph) {
+    (void)hipGraphDestroy(E->graph);
+    E->graph = nullptr;
+  }
+  const int prof = ctx->profiling;
+  ctx->profiling = 0;  // no event records inside the graph
+  FVM_HIP(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = ensemble_step(ctx, E);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+  ctx->profiling = prof;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fvm_set_err(ctx, FVM_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  E->graph = g;
+  FVM_HIP(ctx, hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
+  return FVM_OK;
+}
+
+extern "C" int fvm_ensemble_launch(fvm_ctx *ctx, fvm_ensemble *E) {
+  if (!ctx || !E) return FVM_ERR_ARG;
+  if (!E->exec) FVM_TRY(fvm_ensemble_capture(ctx, E));
+  FVM_HIP(ctx, hipGraphLaunch(E->exec, ctx->stream));
+  return FVM_OK;
+}
+
+// Autotuner: time every (block, lanes) pair on `reps` graph launches of the
+// real step and keep the fastest.  Each trial advances the ensemble's state.
+extern "C" int fvm_ensemble_autotune(fvm_ctx *ctx, fvm_ensemble *E, int32_t reps, fvm_tuning *best,
+                                     float *ms_per_step) {
+  if (!ctx || !E || reps < 1) return FVM_ERR_ARG;
+  const int blocks[3] = {128, 256, 512}, lanes[3] = {8, 16, 32};
+  fvm_tuning bt = E->tune;
+  float bms = 1e30f;
+  for (int bi = 0; bi < 3; bi++)
+    for (int li = 0; li < 3; li++) {
+      fvm_tuning t = {blocks[bi], lanes[li]};
+      FVM_TRY(fvm_ensemble_set_tuning(ctx, E, &t));
+      FVM_TRY(fvm_ensemble_capture(ctx, E));
+      FVM_TRY(fvm_ensemble_launch(ctx, E));  // warm
+      FVM_TRY(fvm_timer_start(ctx));
+      for (int r = 0; r < reps; r++) FVM_TRY(fvm_ensemble_launch(ctx, E));
+      float ms = 0.f;
+      FVM_TRY(fvm_timer_stop(ctx, &ms));
+      ms /= reps;
+      if (ms < bms) {
+        bms = ms;
+        bt = t;
+      }
+    }
+  FVM_TRY(fvm_ensemble_set_tuning(ctx, E, &bt));
+  FVM_TRY(fvm_ensemble_capture(ctx, E));
+  if (best) *best = bt;
+  if (ms_per_step) *ms_per_step = bms;
+  return FVM_OK;
+}
+
+extern "C" int fvm_ensemble_counters(fvm_ctx *ctx, fvm_ensemble *E, int32_t *nOut, int32_t *nBad) {
+  if (!ctx || !E) return FVM_ERR_ARG;
+  int32_t h[2] = {0, 0};
+  FVM_HIP(ctx, hipMemcpyAsync(h, E->counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (nOut) *nOut = h[0];
+  if (nBad) *nBad = h[1];
+  return FVM_OK;
+}

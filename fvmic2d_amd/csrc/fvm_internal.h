@@ -67,6 +67,48 @@ static inline int fvm_grid_check(fvm_ctx *ctx, const fvm_grid *g) {
 
 static inline unsigned fvm_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// ---- XCD-aware block order ----------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each with
+// its own L2.  Kernels whose neighbouring blocks share cache lines (projection
+// passes: adjacent lattice cells read the same marker lines) renumber their
+// blocks so that every XCD works on one contiguous range: block b runs on XCD
+// b % 8 and becomes tile (b % 8) * q + min(b % 8, r) + b / 8, q = nb / 8,
+// r = nb % 8 -- a bijection on [0, nb).
+#define FVM_NUM_XCD 8
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
+  const int64_t q = nb / FVM_NUM_XCD, r = nb % FVM_NUM_XCD;
+  const int64_t x = b % FVM_NUM_XCD, k = b / FVM_NUM_XCD;
+  return x * q + (x < r ? x : r) + k;
+}
+
+// Wavefront-uniform lane broadcast (v_readlane): cheaper than ds_bpermute.
+__device__ __forceinline__ int read_lane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// Wavefront-aggregated bucket counting: lanes with the same key k >= 0 share
+// one atomicAdd on count[k]; every lane gets a provisional slot (the old count
+// plus its rank among equal keys in lane order).  All leaders issue their
+// atomics together, so the wavefront waits for one round trip.
+__device__ __forceinline__ int wave_count_slot(int k, int32_t *__restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(k >= 0);
+  int leader = 0, rank = 0, npeer = 0;
+  while (todo) {
+    const int l = __ffsll((long long)todo) - 1;
+    const int kl = read_lane(k, l);
+    const unsigned long long peers = __ballot(k == kl) & todo;
+    if (k == kl) {
+      leader = l;
+      rank = __popcll(peers & ((1ull << lane) - 1ull));
+      npeer = __popcll(peers);
+    }
+    todo &= ~peers;
+  }
+  int base = 0;
+  if (k >= 0 && lane == leader) base = atomicAdd(&count[k], npeer);
+  base = __shfl(base, leader);
+  return base + rank;
+}
+
 // ---- device-side grid view -------------------------------------------------
 struct DGrid {
   int NX, NY;
@@ -176,6 +218,24 @@ int fvm_subgrid_t_pass1(fvm_ctx *ctx, const double *thisT, const fvm_grid *g, co
 int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays *nf, const fvm_grid *g,
                                const BoundaryValues *bv, double dt, const double *deltaT, const double *dTsub,
                                double dt_thermal);
+
+// sort building blocks reused by the batched ensemble (sort.hip)
+int fvm_launch_zero_i32(fvm_ctx *ctx, int32_t *a, int64_t n);
+int fvm_launch_scan(fvm_ctx *ctx, const int32_t *count, int32_t *start, int32_t *sums, int64_t nk);
+int64_t fvm_scan_tiles(int64_t nk);
+int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm);
+__device__ __forceinline__ int bisect_grid(const double *__restrict__ x, int N, double X) {
+  // findMarkerCell's bisection (markers.c:48-59): largest i in [0, N-2] with x[i] <= X
+  int lo = 0, hi = N - 1;
+  while (hi - lo > 1) {
+    int mid = (hi + lo) / 2;
+    if (mid == lo) mid++;
+    else if (mid == hi) mid--;
+    if (X >= x[mid]) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
 
 // deterministic double sum of n values on the device -> *out (host), ops.hip
 int fvm_reduce_sum(fvm_ctx *ctx, const double *d, int64_t n, double *out);

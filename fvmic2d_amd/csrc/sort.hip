@@ -84,19 +84,8 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
     if (count) key[m] = k;
   }
   if (count) {
-    // one atomic per distinct key in the wavefront
-    const int lane = threadIdx.x & 63;
-    unsigned long long todo = __ballot(k >= 0);
-    while (todo) {
-      const int leader = __ffsll((long long)todo) - 1;
-      const int kl = __shfl(k, leader);
-      const unsigned long long peers = __ballot(k == kl) & todo;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(&count[kl], __popcll(peers));
-      base = __shfl(base, leader);
-      if (k == kl) slot[m] = base + __popcll(peers & ((1ull << lane) - 1ull));
-      todo &= ~peers;
-    }
+    const int sl = wave_count_slot(k, count);
+    if (k >= 0) slot[m] = sl;
   }
   // block-wide counts of outside markers and bad cells (integer: order-free)
   __shared__ int s_out, s_bad;
@@ -135,18 +124,8 @@ __global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *
     }
     key[m] = k;
   }
-  const int lane = threadIdx.x & 63;
-  unsigned long long todo = __ballot(k >= 0);
-  while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int kl = __shfl(k, leader);
-    const unsigned long long peers = __ballot(k == kl) & todo;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&count[kl], __popcll(peers));
-    base = __shfl(base, leader);
-    if (k == kl) slot[m] = base + __popcll(peers & ((1ull << lane) - 1ull));
-    todo &= ~peers;
-  }
+  const int sl = wave_count_slot(k, count);
+  if (k >= 0) slot[m] = sl;
 }
 
 __global__ void k_zero_i32(int32_t *__restrict__ a, int64_t n) {
@@ -247,8 +226,8 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32
     const int bk = (pos >= b1) + (pos >= b2) + (pos >= b3);
     int rank = 0;
     for (int j = 0; j < n; j++) {
-      const int vj = __shfl(v, j);
-      const int bj = __shfl(bk, j);
+      const int vj = read_lane(v, j);
+      const int bj = read_lane(bk, j);
       rank += (bj == bk && vj < v) ? 1 : 0;
     }
     const int bs = bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3));
@@ -385,4 +364,33 @@ int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
     FVM_LAUNCH_CHECK(ctx);
   }
   return finish_sort(ctx, g, mk);
+}
+
+// ---- helpers for the batched ensemble (ensemble.hip) ---------------------------------
+int fvm_launch_zero_i32(fvm_ctx *ctx, int32_t *a, int64_t n) {
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  k_zero_i32<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(a, n);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+int fvm_launch_scan(fvm_ctx *ctx, const int32_t *count, int32_t *start, int32_t *sums, int64_t nk) {
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
+  k_scan_tiles<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(count, start, sums, nk);
+  FVM_LAUNCH_CHECK(ctx);
+  k_scan_sums<<<1, 1024, 0, ctx->stream>>>(sums, ntiles, start + nk);
+  FVM_LAUNCH_CHECK(ctx);
+  k_scan_add<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(start, sums, nk);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+int64_t fvm_scan_tiles(int64_t nk) { return (nk + SCAN_TILE - 1) / SCAN_TILE; }
+
+int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm) {
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  k_bucket_order<<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
 }

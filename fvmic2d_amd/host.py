@@ -10,7 +10,8 @@ import ctypes as C
 import numpy as np
 
 from . import abi
-from .lib import (Context, DeviceGrid, DeviceMarkers, DeviceNodal, fvm_projection, I32, P)
+from .lib import (Context, DeviceGrid, DeviceMarkers, DeviceNodal, fvm_instance, fvm_projection,
+                  fvm_tuning, I32, P)
 
 VEP_PROJ = ["proj_rho", "proj_etaN", "proj_etaS", "proj_kThermal", "proj_lastT", "proj_Cp",
             "proj_soxx", "proj_soyy", "proj_soxy", "proj_rhodot", "proj_muS", "proj_muN"]
@@ -181,3 +182,79 @@ class MarkerPath:
 
     def step(self):
         return self.step_vep() if self.w.kind == "vep" else self.step_thermal()
+
+
+class Ensemble:
+    """Many independent thermal-marker problems stepped together.
+
+    ``paths`` are MarkerPath objects on one Context whose workloads carry a
+    projection list of the same pattern; their device arrays are shared with
+    the ensemble (stepping the ensemble advances each path's state).
+    """
+
+    def __init__(self, paths):
+        assert paths and all(p.ctx is paths[0].ctx for p in paths)
+        self.paths, self.ctx = list(paths), paths[0].ctx
+        arr = (fvm_instance * len(paths))()
+        self._views = []
+        for i, p in enumerate(paths):
+            assert p.proj_list is not None, "ensemble members need a projection list"
+            w, e = p.w, arr[i]
+            e.grid = p.grid.desc
+            e.markers = p.markers.desc
+            v = p.thermal_view()
+            self._views.append(v)
+            e.nodal = v
+            e.nproj = len(p.proj_list)
+            e.proj = C.cast(p.proj_list, P)
+            e.materials = w.materials
+            e.bv = w.bv
+            e.dt = w.dt
+            e.dsubgrid = w.options.subgridTemperatureDiffusivity
+        self._arr = arr
+        h = P()
+        self.ctx.call("fvm_ensemble_create", I32(len(paths)), C.cast(arr, P), C.byref(h))
+        self.h = h
+
+    @property
+    def n_markers(self):
+        return int(self.ctx.lib.fvm_ensemble_markers(self.h))
+
+    def step(self):
+        self.ctx.call("fvm_ensemble_step", self.h)
+
+    def capture(self):
+        self.ctx.call("fvm_ensemble_capture", self.h)
+
+    def launch(self):
+        self.ctx.call("fvm_ensemble_launch", self.h)
+
+    def set_tuning(self, block, lanes):
+        t = fvm_tuning(block, lanes)
+        self.ctx.call("fvm_ensemble_set_tuning", self.h, C.byref(t))
+
+    def get_tuning(self):
+        t = fvm_tuning()
+        self.ctx._check(self.ctx.lib.fvm_ensemble_get_tuning(self.h, C.byref(t)))
+        return t.block, t.lanes
+
+    def autotune(self, reps=5):
+        t, ms = fvm_tuning(), C.c_float()
+        self.ctx.call("fvm_ensemble_autotune", self.h, I32(reps), C.byref(t), C.byref(ms))
+        return (t.block, t.lanes), ms.value
+
+    def counters(self):
+        a, b = I32(), I32()
+        self.ctx.call("fvm_ensemble_counters", self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.fvm_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

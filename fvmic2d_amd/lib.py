@@ -42,6 +42,16 @@ class fvm_projection(C.Structure):
                 ("method", C.c_int32)]
 
 
+class fvm_instance(C.Structure):
+    _fields_ = [("grid", fvm_grid), ("markers", fvm_markers), ("nodal", abi.NodalArrays),
+                ("nproj", C.c_int32), ("proj", C.c_void_p), ("materials", abi.Materials),
+                ("bv", abi.BoundaryValues), ("dt", C.c_double), ("dsubgrid", C.c_double)]
+
+
+class fvm_tuning(C.Structure):
+    _fields_ = [("block", C.c_int32), ("lanes", C.c_int32)]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 D = C.c_double
@@ -85,12 +95,23 @@ _SIGS = {
     "fvm_profile_read": [P, C.POINTER(C.c_double), C.POINTER(I32)],
     "fvm_timer_start": [P],
     "fvm_timer_stop": [P, C.POINTER(C.c_float)],
+    "fvm_ensemble_create": [P, I32, P, C.POINTER(P)],
+    "fvm_ensemble_destroy": [P],
+    "fvm_ensemble_step": [P, P],
+    "fvm_ensemble_set_tuning": [P, P, P],
+    "fvm_ensemble_get_tuning": [P, P],
+    "fvm_ensemble_markers": [P],
+    "fvm_ensemble_capture": [P, P],
+    "fvm_ensemble_launch": [P, P],
+    "fvm_ensemble_autotune": [P, P, I32, P, C.POINTER(C.c_float)],
+    "fvm_ensemble_counters": [P, P, C.POINTER(I32), C.POINTER(I32)],
     # reference-signature wrappers (fvm_compat.h); the wrappers themselves keep
     # the reference's names and are bound by callers with their own prototypes
     "fvm_compat_finalize": [],
     "fvm_compat_last_status": [],
 }
-_RESTYPES = {"fvm_last_error": C.c_char_p, "fvm_ctx_stream": P, "fvm_ctx_workspace_bytes": C.c_size_t}
+_RESTYPES = {"fvm_last_error": C.c_char_p, "fvm_ctx_stream": P, "fvm_ctx_workspace_bytes": C.c_size_t,
+             "fvm_ensemble_markers": C.c_int64}
 
 
 def load(path=None):

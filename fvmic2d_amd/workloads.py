@@ -362,11 +362,14 @@ def convection_tdep(seed=2, N=201, NM=6, contrast=1e5, name="convection_tdep"):
     return w
 
 
-def ensemble(n_instances=64, seed=4, nmin=65, nmax=257, small=False):
-    """Independent convection_tdep instances with random sizes and marker densities."""
+def ensemble(n_instances=64, seed=4, nmin=65, nmax=257, first=0):
+    """Independent convection_tdep instances with random sizes (nmin..nmax nodes a
+    side), 16/25/36 markers per cell and viscosity contrast 1e2..1e6; instance i
+    depends only on (seed, i), so ``first`` generates a slice of a larger set."""
     r = Rng(seed, 7)
+    r.pos = 3 * first
     out = []
-    for i in range(n_instances):
+    for i in range(first, first + n_instances):
         u = r.uniform(3)
         N = int(nmin + min(int(u[0] * (nmax - nmin + 1)), nmax - nmin))
         NM = 4 + min(int(u[1] * 3), 2)

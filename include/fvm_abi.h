@@ -235,6 +235,59 @@ int fvm_timer_start(fvm_ctx *ctx);
 /* Milliseconds between fvm_timer_start and now (synchronises). */
 int fvm_timer_stop(fvm_ctx *ctx, float *ms);
 
+/* ---- batched ensemble of independent problems ------------------------------
+ * No counterpart in the reference, which runs a parameter sweep as separate
+ * processes (main_subduction.c:167-172 loops over Monte-Carlo realisations).
+ * An ensemble steps many independent thermal-marker problems (own grid, marker
+ * set, nodal fields, materials, dt; sizes may differ) with one launch per
+ * phase over all of them, optionally replayed as one captured HIP graph.
+ * Each instance's step equals fvm_step_thermal with the same projection list
+ * (findMarkerCells + density loop, projections, subgridTemperatureChanges,
+ * advectMarkersRK), and with lanes = 16 it is bitwise identical to it.
+ * Every instance must use the same projection pattern (stag/method/material
+ * kind per entry; the fields and property tables may differ); methods
+ * ARITHMETIC and ARITHMETIC_LOCAL.  Device arrays stay owned by the caller and
+ * must outlive the ensemble. */
+typedef struct {
+  fvm_grid grid;
+  fvm_markers markers;   /* nMark is fixed at creation */
+  NodalArrays nodal;     /* vx, vy, wxy, thisT read; lastT, rho, Cp, kThermal + list outputs written */
+  int32_t nproj;
+  const fvm_projection *proj;  /* copied at creation */
+  Materials materials;
+  BoundaryValues bv;
+  double dt;
+  double dsubgrid;       /* Options.subgridTemperatureDiffusivity */
+} fvm_instance;
+
+/* Launch shape: threads per block {128, 256, 512}; lanes per lattice cell in
+ * the projection pass {8, 16, 32}. */
+typedef struct {
+  int32_t block;
+  int32_t lanes;
+} fvm_tuning;
+
+typedef struct fvm_ensemble fvm_ensemble;
+
+int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *inst, fvm_ensemble **out);
+int fvm_ensemble_destroy(fvm_ensemble *e);
+/* One step of every instance, launched on the context's stream. */
+int fvm_ensemble_step(fvm_ctx *ctx, fvm_ensemble *e);
+int fvm_ensemble_set_tuning(fvm_ctx *ctx, fvm_ensemble *e, const fvm_tuning *t);
+int fvm_ensemble_get_tuning(const fvm_ensemble *e, fvm_tuning *t);
+int64_t fvm_ensemble_markers(const fvm_ensemble *e);
+/* Capture one step as a HIP graph (replaces a previous capture). */
+int fvm_ensemble_capture(fvm_ctx *ctx, fvm_ensemble *e);
+/* Replay the captured step (captures first if needed). */
+int fvm_ensemble_launch(fvm_ctx *ctx, fvm_ensemble *e);
+/* Time every launch shape on `reps` graph replays and keep the fastest; each
+ * trial advances the ensemble's state. */
+int fvm_ensemble_autotune(fvm_ctx *ctx, fvm_ensemble *e, int32_t reps, fvm_tuning *best,
+                          float *ms_per_step);
+/* Markers outside the domain / with no valid cell, summed over instances, in
+ * the last step (synchronises). */
+int fvm_ensemble_counters(fvm_ctx *ctx, fvm_ensemble *e, int32_t *nOut, int32_t *nBad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,146 @@
+"""Batched ensemble (fvm_ensemble_*): many independent thermal-marker problems of
+different sizes stepped by one launch per phase.
+
+Bars: with 16 lanes per lattice cell (the single-instance projection width) an
+ensemble member is bit-identical to the same problem stepped alone with
+fvm_step_thermal; with every launch shape it matches the oracle to the
+single-instance bars (positions bitwise, FP64 fields to 1e-12 relative); graph
+replays equal eager steps bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from fvmic2d_amd import workloads
+from fvmic2d_amd.host import Ensemble, MarkerPath
+from fvmic2d_amd.lib import Context, FvmError
+from helpers import assert_bitwise, assert_close, field_scale
+
+pytestmark = pytest.mark.gpu
+
+MF = ["X", "Y", "VX", "VY", "T", "Tdot", "rho", "rhodot", "wxy", "cellX", "cellY", "cpu"]
+
+
+def _instances(n=5, seed=9, empty=True):
+    ws = workloads.ensemble(n, seed=seed, nmin=17, nmax=61)
+    if empty:  # an instance without markers must be skipped cleanly
+        ws[1].markers = ws[1].markers[:0]
+    return ws
+
+
+def _paths(ws, ctx):
+    return [MarkerPath(w, ctx=ctx) for w in ws]
+
+
+def _outs(w):
+    return [p[4] for p in w.projections]
+
+
+def _assert_same(a, b, what):
+    ma, mb = a.get_markers(), b.get_markers()
+    for f in MF:
+        assert_bitwise(ma[f], mb[f], "%s %s" % (what, f))
+    for o in _outs(a.w):
+        assert_bitwise(a.get_nodal(o), b.get_nodal(o), "%s %s" % (what, o))
+
+
+def _assert_oracle(st, mp, what):
+    g, r = mp.get_markers(), st.markers
+    for f in ("cellX", "cellY", "X", "Y", "VX", "VY", "wxy"):
+        assert_bitwise(g[f], r[f], what + f)
+    act = r["cellX"] != -1
+    for f in ("T", "Tdot", "rho"):
+        assert_close(g[f][act], r[f][act], what + f, scale=field_scale(r[f][act]))
+    for o in _outs(st.w):
+        assert_close(mp.get_nodal(o), st.nodal[o], what + o, scale=field_scale(st.nodal[o]))
+
+
+def test_ensemble_equals_single_instance():
+    print("CASE ensemble vs single, 5 instances", flush=True)
+    ctx = Context()
+    ws = _instances()
+    alone, batch = _paths(ws, ctx), _paths(ws, ctx)
+    ens = Ensemble(batch)
+    assert ens.get_tuning() == (256, 16)
+    assert ens.n_markers == sum(w.n_markers for w in ws)
+    for s in range(2):
+        for p in alone:
+            p.step_thermal()
+        ens.step()
+        for i, (a, b) in enumerate(zip(alone, batch)):
+            _assert_same(a, b, "step %d instance %d" % (s, i))
+    ens.close()
+
+
+@pytest.mark.parametrize("block,lanes", [(128, 8), (256, 16), (512, 32), (128, 32), (512, 8)])
+def test_ensemble_vs_oracle(block, lanes):
+    print("CASE ensemble vs oracle block=%d lanes=%d" % (block, lanes), flush=True)
+    ctx = Context()
+    ws = _instances(4, seed=13)
+    paths = _paths(ws, ctx)
+    ens = Ensemble(paths)
+    ens.set_tuning(block, lanes)
+    ens.step()
+    nout = 0
+    for i, (w, mp) in enumerate(zip(ws, paths)):
+        st = oracle.State(w, extra=_outs(w))
+        st.step_thermal()
+        nout += st.ms.nOut
+        _assert_oracle(st, mp, "instance %d " % i)
+    assert ens.counters() == (nout, 0)
+    ens.close()
+
+
+def test_ensemble_graph_replay_equals_eager():
+    print("CASE ensemble graph replay", flush=True)
+    ctx = Context()
+    ws = _instances(4, seed=21)
+    eager_p, graph_p = _paths(ws, ctx), _paths(ws, ctx)
+    eager, graph = Ensemble(eager_p), Ensemble(graph_p)
+    graph.capture()
+    for s in range(4):
+        eager.step()
+        graph.launch()
+        ctx.sync()
+        for i, (a, b) in enumerate(zip(eager_p, graph_p)):
+            _assert_same(a, b, "replay %d instance %d" % (s, i))
+    eager.close()
+    graph.close()
+
+
+def test_ensemble_autotune_keeps_results_valid():
+    print("CASE ensemble autotune", flush=True)
+    ctx = Context()
+    ws = _instances(3, seed=5, empty=False)
+    paths = _paths(ws, ctx)
+    ens = Ensemble(paths)
+    (block, lanes), ms = ens.autotune(reps=1)
+    assert block in (128, 256, 512) and lanes in (8, 16, 32) and ms > 0.0
+    assert ens.get_tuning() == (block, lanes)
+    # 9 shapes x 2 launches + 1 more: positions stay bit-identical to the oracle
+    ens.launch()
+    ctx.sync()
+    for i, (w, mp) in enumerate(zip(ws, paths)):
+        st = oracle.State(w, extra=_outs(w))
+        for _ in range(19):
+            st.step_thermal()
+        g = mp.get_markers()
+        for f in ("X", "Y", "cellX", "cellY"):
+            assert_bitwise(g[f], st.markers[f], "instance %d %s" % (i, f))
+    ens.close()
+
+
+def test_ensemble_rejects_bad_input():
+    ctx = Context()
+    ws = _instances(2, seed=3, empty=False)
+    paths = _paths(ws, ctx)
+    ens = Ensemble(paths)
+    with pytest.raises(FvmError):
+        ens.set_tuning(96, 16)
+    with pytest.raises(FvmError):
+        ens.set_tuning(256, 12)
+    ens.close()
+    # projection patterns must agree
+    paths[1].proj_list[0].stagx = 1
+    with pytest.raises(FvmError):
+        Ensemble(paths)
