@@ -163,6 +163,38 @@ def bench_ensemble(a):
     return 0
 
 
+def copy_calibration(nbytes=512 << 20, reps=5):
+    """A device-to-device copy of known size, for scaling the FETCH_SIZE and
+    WRITE_SIZE counters (profile this run and divide nbytes by the counter)."""
+    import ctypes as C
+    from fvmic2d_amd.lib import Context
+    ctx = Context()
+    a, b = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.call("fvm_memset", C.c_void_p(a), 1, nbytes)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(reps):
+        ctx.call("fvm_memcpy_dtod", C.c_void_p(b), C.c_void_p(a), nbytes)
+    ms = ctx.timer_stop() / reps
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ms_per_copy": ms,
+                      "GBs_read_plus_write": 2 * nbytes / (ms / 1e3) / 1e9}))
+    ctx.free(a)
+    ctx.free(b)
+    return 0
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` from its committed counter summary, if any
+    (a line 'bytes_per_launch: N' in profiles/traffic_<kernel>.md)."""
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.md" % kernel)
+    if not os.path.exists(tf):
+        return None
+    for line in open(tf):
+        if line.strip().startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,12 +205,16 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="time the batched 64-instance ensemble instead (own JSON line)")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 512 MiB device copy (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
               "and this build has no multi-GPU path (see DESIGN.md for how it would shard).",
               file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration()
     if a.ensemble:
         return bench_ensemble(a)
 
@@ -208,10 +244,7 @@ def main():
     dom = max(cands, key=cands.get)
     per_launch_ms = phase_ms[dom] / phase_launches[dom]
     achieved = ab[dom] / (per_launch_ms / 1e3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_%s.json" % dom)
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+    traffic = traffic_per_launch(dom)
     step_bytes = n0 * STEP_BYTES_PER_MARKER + w.NX * w.NY * STEP_BYTES_PER_NODE
     out = {
         "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,

@@ -156,25 +156,53 @@ __global__ void __launch_bounds__(512) kb_partials(const InstDev *__restrict__ i
                                t, lx, ly, j, live);
 }
 
+// (group, field) pairs of one node launch: launch row blockIdx.y = pair
+struct PairTab {
+  unsigned char g[64], k[64];
+};
+
 __global__ void __launch_bounds__(256) kb_nodes(const InstDev *__restrict__ insts, int n,
                                                 const int64_t *__restrict__ blk, const NodeGroup *__restrict__ nodes,
-                                                int g0, int g1) {
+                                                PairTab pt) {
   const int ii = find_inst(blk, n, blockIdx.x);
   const InstDev &I = insts[ii];
   const int64_t i = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (i >= (int64_t)I.g.NX * I.g.NY) return;
   const int ix = (int)(i % I.g.NX), jy = (int)(i / I.g.NX);
-  for (int gi = g0; gi < g1; gi++) node_group(nodes[(int64_t)gi * n + ii], ix, jy, i);
+  const int p = blockIdx.y;
+  node_field(nodes[(int64_t)pt.g[p] * n + ii], ix, jy, i, pt.k[p]);
 }
 
-__global__ void __launch_bounds__(512) kb_subgrid_t1(const InstDev *__restrict__ insts, int n,
-                                                     const int64_t *__restrict__ blk, double *__restrict__ dTsub) {
+template <int G>
+__global__ void __launch_bounds__(256) kb_partials_subt(const InstDev *__restrict__ insts, int n,
+                                                        const int64_t *__restrict__ blk,
+                                                        const ProjGroup *__restrict__ groups,
+                                                        const int64_t *__restrict__ partoff,
+                                                        const int32_t *__restrict__ start,
+                                                        const int32_t *__restrict__ perm, double *__restrict__ part,
+                                                        double *__restrict__ dTsub) {
   const int ii = find_inst(blk, n, blockIdx.x);
   const InstDev &I = insts[ii];
-  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
-  if (m >= I.nMark) return;
-  subgrid_t1_marker(m, I.g, I.X, I.Y, I.T, I.cellX, I.cellY, I.lastT, I.rhon, I.Cpn, I.kn, I.rho0, I.dt, I.dsub,
-                    dTsub + I.mark_off);
+  const int nLx = I.g.NX - 1, nLy = I.g.NY - 1;
+  const int64_t nL = (int64_t)nLx * nLy;
+  const int64_t lt = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = lt / G;
+  const int j = (int)(lt % G);
+  const bool live = t < nL;
+  const int lx = live ? (int)(t % nLx) : 0;
+  const int ly = live ? (int)(t / nLx) : 0;
+  SubT sb;
+  sb.T = I.lastT;
+  sb.rho = I.rhon;
+  sb.Cp = I.Cpn;
+  sb.k = I.kn;
+  sb.MT = I.T;
+  sb.dTsub = dTsub + I.mark_off;
+  sb.rho0 = I.rho0;
+  sb.dt = I.dt;
+  sb.dsub = I.dsub;
+  partials_cell_subt<G>(I.g, groups[ii], start + I.bucket_off, perm, I.X, I.Y, part + partoff[ii], nL, t, lx, ly, j,
+                        live, sb);
 }
 
 __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ insts, int n,
@@ -263,12 +291,47 @@ static int kb_partials_dispatch(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
   return fvm_set_err(ctx, FVM_ERR_BAD_METHOD, "ensemble projections support ARITHMETIC and ARITHMETIC_LOCAL");
 }
 
+static int launch_kb_partials_subt(fvm_ctx *ctx, fvm_ensemble *E) {
+  const int n = E->n, gi = E->ng;
+  const int64_t nblk = E->hblk[(2 + gi) * (n + 1) + n];
+  if (nblk == 0) return FVM_OK;
+  const ProjGroup *groups = E->dgroups + (int64_t)gi * n;
+  const int64_t *po = E->dpartoff + (int64_t)gi * n;
+  const int64_t *blk = blk_row(E, 2 + gi);
+  FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
+  switch (E->tune.lanes) {
+    case 8:
+      kb_partials_subt<8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start, E->perm,
+                                                                   E->part, E->dTsub);
+      break;
+    case 32:
+      kb_partials_subt<32><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start, E->perm,
+                                                                    E->part, E->dTsub);
+      break;
+    default:
+      kb_partials_subt<16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start, E->perm,
+                                                                    E->part, E->dTsub);
+  }
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
 static int launch_kb_nodes(fvm_ctx *ctx, fvm_ensemble *E, int g0, int g1) {
   const int n = E->n;
   const int64_t nblk = E->hblk[1 * (n + 1) + n];
   if (nblk == 0) return FVM_OK;
+  PairTab pt;
+  int np = 0;
+  for (int gi = g0; gi < g1; gi++)
+    for (int k = 0; k < E->gnf[gi]; k++) {
+      if (np == 64) return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "ensemble: more than 64 projected fields");
+      pt.g[np] = (unsigned char)gi;
+      pt.k[np] = (unsigned char)k;
+      np++;
+    }
+  if (np == 0) return FVM_OK;
   FVM_PHASE(ctx, FVM_PHASE_NODES);
-  kb_nodes<<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1), E->dnodes, g0, g1);
+  kb_nodes<<<dim3((unsigned)nblk, np), 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1), E->dnodes, pt);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
@@ -296,13 +359,9 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   // projections (:268)
   for (int gi = 0; gi < E->ng; gi++) FVM_TRY(kb_partials_dispatch(ctx, E, gi));
   FVM_TRY(launch_kb_nodes(ctx, E, 0, E->ng));
-  // subgridTemperatureChanges (:415): pass 1, dTsub projection with the VecAXPBYPCZ epilogue
-  if (mblk) {
-    FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
-    kb_subgrid_t1<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->dTsub);
-    FVM_LAUNCH_CHECK(ctx);
-  }
-  FVM_TRY(kb_partials_dispatch(ctx, E, E->ng));
+  // subgridTemperatureChanges (:415): pass 1 inside the dTsub projection, whose
+  // node pass applies the VecAXPBYPCZ epilogue
+  FVM_TRY(launch_kb_partials_subt(ctx, E));
   FVM_TRY(launch_kb_nodes(ctx, E, E->ng, E->ng + 1));
   // pass 2 fused into advectMarkersRK (:435)
   if (mblk) {

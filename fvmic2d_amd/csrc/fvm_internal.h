@@ -190,7 +190,9 @@ int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t r
 int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk);
 // Run projection groups over the current bucket order (project.hip).
 int fvm_project_group(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg);
-int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, int ng, const ProjGroup *pgs);
+struct SubT;
+int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, int ng, const ProjGroup *pgs,
+                       const SubT *sub = nullptr);
 // Invalidate the cached bucket order (positions or cells changed).
 static inline void fvm_markers_moved(fvm_ctx *ctx) { ctx->epoch++; }
 

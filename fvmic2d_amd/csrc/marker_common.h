@@ -45,16 +45,18 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     const double dy = g.y[c0y + 1] - g.y[c0y];
     const double mdx = (X0 - g.x[c0x]) / dx;
     const double mdy = (Y0 - g.y[c0y]) / dy;
-    const double dtm = (gat(dTn, NX, NY, c0y, c0x) * (1.0 - mdy) * (1.0 - mdx) +
-                        gat(dTn, NX, NY, c0y + 1, c0x) * mdy * (1.0 - mdx) +
-                        gat(dTn, NX, NY, c0y, c0x + 1) * (1.0 - mdy) * mdx +
-                        gat(dTn, NX, NY, c0y + 1, c0x + 1) * mdy * mdx) +
+    // the stored cell is inside the grid: all four corners are owned nodes
+    const double *d0 = dTn + (int64_t)c0y * NX + c0x, *d1 = d0 + NX;
+    const double dtm = (d0[0] * (1.0 - mdy) * (1.0 - mdx) + d1[0] * mdy * (1.0 - mdx) + d0[1] * (1.0 - mdy) * mdx +
+                        d1[1] * mdy * mdx) +
                        dTsub[m];
     MT[m] = MT[m] + dtm;
     MTdot[m] = dtm / dt_thermal;
   }
   double xcur = X0, ycur = Y0;
-  double mvx[4], mvy[4], mwxy[4];
+  // RK4 weights (k1 + 2 k2 + 2 k3 + k4) / 6 (markers.c:2118-2120), summed in the
+  // expression's left-to-right order as the stages complete
+  double svx = 0.0, svy = 0.0, swxy = 0.0;
 #pragma unroll
   for (int irk = 0; irk < 4; irk++) {
     int cellX = c0x, cellY = c0y;
@@ -88,39 +90,47 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     double dy = g.yc[vxY + 2] - g.yc[vxY + 1];
     // vxX == cellX (cellX stays in [0, NX-2]), so this quotient is also the wxy mdx below
     const double qx = deltaxm / dx;
+    // vx nodes: rows vxY (-1 is the ghost row, which reads 0) and vxY + 1,
+    // columns vxX and vxX + 1 (both owned: vxX is in [0, NX-2]).  Loads are
+    // unconditional from a valid address; the ghost row is selected away.
+    const bool vrow = vxY >= 0;
+    const double *vr1 = vx + (int64_t)(vxY + 1) * NX + vxX;
+    const double *vr0 = vrow ? vr1 - NX : vr1;
+    const double vxll = vr1[0], vxlr = vr1[1];
+    const double vx0 = vr0[0], vx1 = vr0[1];
     double vxul, vxur;
     if (vxY == -1 && bv.mechBCTop.type[0] == 0) {
-      vxul = 2.0 * bv.mechBCTop.value[0] - gat(vx, NX, NY, vxY + 1, vxX);
-      vxur = 2.0 * bv.mechBCTop.value[0] - gat(vx, NX, NY, vxY + 1, vxX + 1);
+      vxul = 2.0 * bv.mechBCTop.value[0] - vxll;
+      vxur = 2.0 * bv.mechBCTop.value[0] - vxlr;
     } else if (vxY == -1 && bv.mechBCTop.type[0] == 1) {
-      vxul = gat(vx, NX, NY, vxY + 1, vxX);
-      vxur = gat(vx, NX, NY, vxY + 1, vxX + 1);
+      vxul = vxll;
+      vxur = vxlr;
     } else {
-      vxul = gat(vx, NX, NY, vxY, vxX);
-      vxur = gat(vx, NX, NY, vxY, vxX + 1);
+      vxul = vrow ? vx0 : 0.0;
+      vxur = vrow ? vx1 : 0.0;
     }
-    const double vxll = gat(vx, NX, NY, vxY + 1, vxX);
-    const double vxlr = gat(vx, NX, NY, vxY + 1, vxX + 1);
     const double qyv = deltaym / dy;
     double a = vxul * (1.0 - qx) * (1.0 - qyv);
     a += vxur * (qx) * (1.0 - qyv);
     a += vxll * (1.0 - qx) * (qyv);
     a += vxlr * (deltaxm * deltaym) / dx / dy;
-    mvx[irk] = a;
 
-    double vyul;
-    const double vyur = gat(vy, NX, NY, vyY, vyX + 1);
-    double vyll;
-    const double vylr = gat(vy, NX, NY, vyY + 1, vyX + 1);
+    // vy nodes: columns vyX (-1 is the ghost column, which reads 0) and
+    // vyX + 1, rows vyY and vyY + 1 (both owned: vyY is in [0, NY-2])
+    const bool vcol = vyX >= 0;
+    const double *q0 = vy + (int64_t)vyY * NX + (vyX + 1), *q1 = q0 + NX;
+    const double vyur = q0[0], vylr = q1[0];
+    const double vy0 = (vcol ? q0 - 1 : q0)[0], vy1 = (vcol ? q1 - 1 : q1)[0];
+    double vyul, vyll;
     if (vyX == -1 && bv.mechBCLeft.type[1] == 0) {
-      vyul = 2.0 * bv.mechBCLeft.value[1] - gat(vy, NX, NY, vyY, vyX + 1);
-      vyll = 2.0 * bv.mechBCLeft.value[1] - gat(vy, NX, NY, vyY + 1, vyX + 1);
+      vyul = 2.0 * bv.mechBCLeft.value[1] - vyur;
+      vyll = 2.0 * bv.mechBCLeft.value[1] - vylr;
     } else if (vyX == -1 && bv.mechBCLeft.type[1] == 1) {
-      vyul = gat(vy, NX, NY, vyY, vyX + 1);
-      vyll = gat(vy, NX, NY, vyY + 1, vyX + 1);
+      vyul = vyur;
+      vyll = vylr;
     } else {
-      vyul = gat(vy, NX, NY, vyY, vyX);
-      vyll = gat(vy, NX, NY, vyY + 1, vyX);
+      vyul = vcol ? vy0 : 0.0;
+      vyll = vcol ? vy1 : 0.0;
     }
     deltaxm = xcur - g.xc[vyX + 1];
     deltaym = ycur - g.y[vyY];
@@ -133,31 +143,43 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     b += vyur * (qxv) * (1.0 - qy);
     b += vyll * (1.0 - qxv) * (qy);
     b += vylr * (deltaxm * deltaym) / dx / dy;
-    mvy[irk] = b;
 
     // markers.c:2089-2090: (xcur - x[cellX]) / (x[cellX+1] - x[cellX]), the same
     // operands as qx and qy above, hence the same values
     const double mdx = qx;
     const double mdy = qy;
-    double w = (1.0 - mdx) * (1.0 - mdy) * gat(wxy, NX, NY, cellY, cellX);
-    w += (mdx) * (1.0 - mdy) * gat(wxy, NX, NY, cellY, cellX + 1);
-    w += (1.0 - mdx) * (mdy) * gat(wxy, NX, NY, cellY + 1, cellX);
-    w += (mdx) * (mdy) * gat(wxy, NX, NY, cellY + 1, cellX + 1);
-    mwxy[irk] = w;
-
+    // cellX in [0, NX-2], cellY in [0, NY-2]: four owned nodes
+    const double *w0 = wxy + (int64_t)cellY * NX + cellX, *w1 = w0 + NX;
+    double w = (1.0 - mdx) * (1.0 - mdy) * w0[0];
+    w += (mdx) * (1.0 - mdy) * w0[1];
+    w += (1.0 - mdx) * (mdy) * w1[0];
+    w += (mdx) * (mdy) * w1[1];
+    if (irk == 0) {
+      svx = a;
+      svy = b;
+      swxy = w;
+    } else if (irk < 3) {
+      svx = svx + 2 * a;
+      svy = svy + 2 * b;
+      swxy = swxy + 2 * w;
+    } else {
+      svx = svx + a;
+      svy = svy + b;
+      swxy = swxy + w;
+    }
     if (irk < 3) {
       if (irk < 2) {
-        xcur = X0 + dt / 2 * mvx[irk];
-        ycur = Y0 + dt / 2 * mvy[irk];
+        xcur = X0 + dt / 2 * a;
+        ycur = Y0 + dt / 2 * b;
       } else {
-        xcur = X0 + dt * mvx[irk];
-        ycur = Y0 + dt * mvy[irk];
+        xcur = X0 + dt * a;
+        ycur = Y0 + dt * b;
       }
     }
   }
-  const double VX = (mvx[0] + 2 * mvx[1] + 2 * mvx[2] + mvx[3]) / 6;
-  const double VY = (mvy[0] + 2 * mvy[1] + 2 * mvy[2] + mvy[3]) / 6;
-  const double wm = (mwxy[0] + 2 * mwxy[1] + 2 * mwxy[2] + mwxy[3]) / 6.0;
+  const double VX = svx / 6;
+  const double VY = svy / 6;
+  const double wm = swxy / 6.0;
   MVX[m] = VX;
   MVY[m] = VY;
   Mwxy[m] = wm;
@@ -181,51 +203,6 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
   }
   const double e11 = exx[m], e22 = eyy[m], e12 = exy[m];
   Eii[m] = Eii[m] + dt * sqrt(0.5 * (e11 * e11 + e22 * e22) + e12 * e12);
-}
-
-// ---- A7: subgridTemperatureChanges pass 1 (subgridTemperatureChanges.c:46-70) ----
-__device__ __forceinline__ void subgrid_t1_marker(int64_t m, const DGrid &g,
-    const double *__restrict__ MX,
-    const double *__restrict__ MY,
-    const double *__restrict__ MT,
-    const int32_t *__restrict__ cX,
-    const int32_t *__restrict__ cY,
-    const double *__restrict__ T,
-    const double *__restrict__ rho,
-    const double *__restrict__ Cp,
-    const double *__restrict__ kT,
-    double rho0,
-    double dt,
-    double dsubgrid,
-    double *__restrict__ dTsub) {
-  const int cellX = cX[m], cellY = cY[m];
-  if (!cell_ok(cellX, cellY, g.NX, g.NY)) {
-    dTsub[m] = 0.0;
-    return;
-  }
-  const int NX = g.NX, NY = g.NY;
-  const double dx = g.x[cellX + 1] - g.x[cellX];
-  const double dy = g.y[cellY + 1] - g.y[cellY];
-  const double mdx = (MX[m] - g.x[cellX]) / dx;
-  const double mdy = (MY[m] - g.y[cellY]) / dy;
-  const double Tm = gat(T, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
-                    gat(T, NX, NY, cellY, cellX + 1) * mdx * (1.0 - mdy) +
-                    gat(T, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
-                    gat(T, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
-  const double rhom = rho0 + gat(rho, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
-                      gat(rho, NX, NY, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
-                      gat(rho, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
-                      gat(rho, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
-  const double Cpm = gat(Cp, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
-                     gat(Cp, NX, NY, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
-                     gat(Cp, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
-                     gat(Cp, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
-  const double km = gat(kT, NX, NY, cellY, cellX) * (1.0 - mdx) * (1.0 - mdy) +
-                    gat(kT, NX, NY, cellY, cellX + 1) * (mdx) * (1.0 - mdy) +
-                    gat(kT, NX, NY, cellY + 1, cellX) * (1.0 - mdx) * (mdy) +
-                    gat(kT, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
-  const double dTdiff = Cpm * rhom / km / (2.0 / (dx * dx) + 2.0 / (dy * dy));
-  dTsub[m] = (Tm - MT[m]) * (1.0 - exp(-dsubgrid * dt / dTdiff));
 }
 
 // ---- A7: subgridTemperatureChanges pass 2 (subgridTemperatureChanges.c:89-103) ---

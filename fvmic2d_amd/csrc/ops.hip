@@ -6,6 +6,7 @@
 #include <float.h>
 
 #include "marker_common.h"
+#include "proj_common.h"
 
 int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
 
@@ -45,17 +46,6 @@ extern "C" int fvm_advect_markers_rk(fvm_ctx *ctx, fvm_markers *mk, const NodalA
 }
 
 // ---- A7: subgridTemperatureChanges (subgridTemperatureChanges.c:5-124) -------
-__global__ void __launch_bounds__(256) k_subgrid_t1(DGrid g, int32_t n, const double *__restrict__ MX,
-                                                    const double *__restrict__ MY, const double *__restrict__ MT,
-                                                    const int32_t *__restrict__ cX, const int32_t *__restrict__ cY,
-                                                    const double *__restrict__ T, const double *__restrict__ rho,
-                                                    const double *__restrict__ Cp, const double *__restrict__ kT,
-                                                    double rho0, double dt, double dsubgrid, double *__restrict__ dTsub) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  subgrid_t1_marker(m, g, MX, MY, MT, cX, cY, T, rho, Cp, kT, rho0, dt, dsubgrid, dTsub);
-}
-
 __global__ void __launch_bounds__(256) k_subgrid_t2(DGrid g, int32_t n, const double *__restrict__ MX,
                                                     const double *__restrict__ MY, double *__restrict__ MT,
                                                     double *__restrict__ MTdot, const int32_t *__restrict__ cX,
@@ -76,14 +66,6 @@ int fvm_subgrid_t_pass1(fvm_ctx *ctx, const double *thisT, const fvm_grid *g, co
   FVM_TRY(fvm_ws(ctx, ctx->scratch, ((size_t)n + (size_t)nn + 64) * sizeof(double)));
   double *dTsub = (double *)ctx->scratch.ptr;
   double *deltaT = dTsub + ((n + 31) / 32) * 32;
-  const DGrid dg = dgrid(g);
-  if (n > 0) {
-    FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
-    k_subgrid_t1<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->T, mk->cellX, mk->cellY,
-                                                               nf->lastT, nf->rho, nf->Cp, nf->kThermal,
-                                                               mat->materialRho[0], dt, dsubgrid, dTsub);
-    FVM_LAUNCH_CHECK(ctx);
-  }
   ProjGroup pg;
   memset(&pg, 0, sizeof(pg));
   pg.stagx = pg.stagy = 0;
@@ -94,7 +76,20 @@ int fvm_subgrid_t_pass1(fvm_ctx *ctx, const double *thisT, const fvm_grid *g, co
   pg.epilogue = EPI_SUBGRID_T;
   pg.ea = thisT;
   pg.eb = nf->lastT;
-  FVM_TRY(fvm_project_group(ctx, g, mk, pg));
+  // pass 1 runs inside the projection's partials pass, one lattice cell (= the
+  // markers' basic cell) per lane group; markers without a valid cell are in
+  // no bucket and get no dTsub (pass 2 skips them too)
+  SubT sb;
+  sb.T = nf->lastT;
+  sb.rho = nf->rho;
+  sb.Cp = nf->Cp;
+  sb.k = nf->kThermal;
+  sb.MT = mk->T;
+  sb.dTsub = dTsub;
+  sb.rho0 = mat->materialRho[0];
+  sb.dt = dt;
+  sb.dsub = dsubgrid;
+  FVM_TRY(fvm_project_groups(ctx, g, mk, 1, &pg, &sb));
   *dTsub_out = dTsub;
   *deltaT_out = deltaT;
   return FVM_OK;

@@ -28,13 +28,10 @@ __host__ __device__ __forceinline__ Lat lattice(int stag, int N) {
   return L;
 }
 
-// fractional position inside lattice cell l (markerProjection.c:127-154)
-__device__ __forceinline__ double frac(int stag, const double *__restrict__ n, const double *__restrict__ c, int l,
-                                       double X) {
-  if (stag == -1) return (X - c[l]) / (c[l + 1] - c[l]);
-  if (stag == 0) return (X - n[l]) / (n[l + 1] - n[l]);
-  return (X - c[l + 1]) / (c[l + 2] - c[l + 1]);
-}
+// The fractional position inside lattice cell l (markerProjection.c:127-154) is
+// (X - lo) / (hi - lo) with lo, hi = xc[l], xc[l+1] (stag -1), x[l], x[l+1]
+// (stag 0) or xc[l+1], xc[l+2] (stag +1); partials_cell_impl loads lo and hi
+// once per lattice cell.
 
 // Accumulate one marker's contribution into the four corner sums.
 template <int METHOD, int NF>
@@ -112,12 +109,25 @@ __device__ __forceinline__ void accumulate(double mdx, double mdy, const double 
 // the corner partial sums to part[v * nL + t], v = corner * (NF + 1) + 0 for
 // the weight, + 1 + k for field k.  `start` addresses this grid's bucket
 // starts; `perm` holds indices into this instance's marker columns.
-template <int METHOD, int NF, int G>
-__device__ __forceinline__ void partials_cell(const DGrid &g, const ProjGroup &pg, const int32_t *__restrict__ start,
-                                              const int32_t *__restrict__ perm, const double *__restrict__ MX,
-                                              const double *__restrict__ MY, const int32_t *__restrict__ Mat,
-                                              double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly, int j,
-                                              bool live) {
+// subgridTemperatureChanges pass 1 (subgridTemperatureChanges.c:46-70) evaluated
+// inside the projection of its result onto the basic nodes (:74): the lattice
+// cell of family (0,0) is the marker's own basic cell, so the quotients mdx, mdy
+// of the projection are the ones pass 1 interpolates with, and the cell's four
+// corner values of lastT, rho, Cp and k are loaded once per lattice cell.
+struct SubT {
+  const double *T, *rho, *Cp, *k;  // nodal lastT, rho, Cp, kThermal
+  const double *MT;                // marker T
+  double *dTsub;                   // per-marker result (read again by pass 2)
+  double rho0, dt, dsub;           // materialRho[0], dt, subgrid diffusivity
+};
+
+template <int METHOD, int NF, int G, bool SUBT>
+__device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGroup &pg,
+                                                   const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
+                                                   const double *__restrict__ MX, const double *__restrict__ MY,
+                                                   const int32_t *__restrict__ Mat, double *__restrict__ part,
+                                                   int64_t nL, int64_t t, int lx, int ly, int j, bool live,
+                                                   const SubT &sb) {
   const int sx = pg.stagx, sy = pg.stagy;
   bool any_mat = false;
 #pragma unroll
@@ -170,19 +180,90 @@ __device__ __forceinline__ void partials_cell(const DGrid &g, const ProjGroup &p
 #pragma unroll
     for (int k = 0; k < NF; k++) sf[c][k] = 0.0;
   }
-  // lane j takes entries j, j+16, ... of the concatenated ranges, in order
-  for (int e = j; e < tot; e += G) {
-    const int idx = e < p1 ? rs[0] + e : (e < p2 ? rs[1] + (e - p1) : (e < p3 ? rs[2] + (e - p2) : rs[3] + (e - p3)));
-    const int m = perm[idx];
-    const double mdx = frac(sx, g.x, g.xc, lx, MX[m]);
-    const double mdy = frac(sy, g.y, g.yc, ly, MY[m]);
-    double f[NF];
-    const int mat = any_mat ? Mat[m] : 0;
+  // pass-1 inputs at the corners of basic cell (lx, ly) (SUBT only; a cell with
+  // markers has lx in [0, NX-2], ly in [0, NY-2])
+  double T00 = 0, T10 = 0, T01 = 0, T11 = 0, r00 = 0, r10 = 0, r01 = 0, r11 = 0;
+  double C00 = 0, C10 = 0, C01 = 0, C11 = 0, k00 = 0, k10 = 0, k01 = 0, k11 = 0, sden = 1.0;
+  if (SUBT && tot > 0) {
+    const int64_t i0 = (int64_t)ly * g.NX + lx, i1 = i0 + g.NX;
+    T00 = sb.T[i0], T10 = sb.T[i0 + 1], T01 = sb.T[i1], T11 = sb.T[i1 + 1];
+    r00 = sb.rho[i0], r10 = sb.rho[i0 + 1], r01 = sb.rho[i1], r11 = sb.rho[i1 + 1];
+    C00 = sb.Cp[i0], C10 = sb.Cp[i0 + 1], C01 = sb.Cp[i1], C11 = sb.Cp[i1 + 1];
+    k00 = sb.k[i0], k10 = sb.k[i0 + 1], k01 = sb.k[i1], k11 = sb.k[i1 + 1];
+    const double dx = g.x[lx + 1] - g.x[lx];
+    const double dy = g.y[ly + 1] - g.y[ly];
+    sden = 2.0 / (dx * dx) + 2.0 / (dy * dy);
+  }
+  // lattice-cell bounds of frac(), loaded once: mdx = (X - x0) / (x1 - x0)
+  double x0 = 0.0, x1 = 1.0, y0 = 0.0, y1 = 1.0;
+  if (tot > 0) {
+    x0 = sx == -1 ? g.xc[lx] : (sx == 0 ? g.x[lx] : g.xc[lx + 1]);
+    x1 = sx == -1 ? g.xc[lx + 1] : (sx == 0 ? g.x[lx + 1] : g.xc[lx + 2]);
+    y0 = sy == -1 ? g.yc[ly] : (sy == 0 ? g.y[ly] : g.yc[ly + 1]);
+    y1 = sy == -1 ? g.yc[ly + 1] : (sy == 0 ? g.y[ly + 1] : g.yc[ly + 2]);
+  }
+  const double xden = x1 - x0, yden = y1 - y0;
+  // Lane j takes entries j, j+G, j+2G, ... of the concatenated ranges and adds
+  // them in that order.  U entries are fetched together (their loads in
+  // flight at once) before they are added.
+  constexpr int U = (SUBT || NF > 4) ? 2 : 4;
+  constexpr int NV = SUBT ? 1 : NF;  // loaded values per marker
+  for (int e0 = j; e0 < tot; e0 += U * G) {
+    int mm[U];
 #pragma unroll
-    for (int k = 0; k < NF; k++)
-      f[k] = pg.src[k].use_mat ? (((unsigned)mat < FVM_MAXMAT) ? pg.src[k].mat[mat] : __builtin_nan(""))
-                               : pg.src[k].col[m];
-    accumulate<METHOD, NF>(mdx, mdy, f, sw, sf);
+    for (int u = 0; u < U; u++) {
+      const int e = e0 + u * G;
+      const int idx =
+          e < p1 ? rs[0] + e : (e < p2 ? rs[1] + (e - p1) : (e < p3 ? rs[2] + (e - p2) : rs[3] + (e - p3)));
+      mm[u] = e < tot ? perm[idx] : -1;
+    }
+    double xs[U], ys[U], vs[U][NV];
+    int mats[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int m = mm[u] >= 0 ? mm[u] : mm[0];  // mm[0] >= 0: e0 < tot
+      xs[u] = MX[m];
+      ys[u] = MY[m];
+      if constexpr (SUBT) {
+        vs[u][0] = sb.MT[m];
+        mats[u] = 0;
+      } else {
+        mats[u] = any_mat ? Mat[m] : 0;
+#pragma unroll
+        for (int k = 0; k < NF; k++) vs[u][k] = pg.src[k].use_mat ? 0.0 : pg.src[k].col[m];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (mm[u] < 0) break;
+      const int m = mm[u];
+      const double mdx = (xs[u] - x0) / xden;
+      const double mdy = (ys[u] - y0) / yden;
+      double f[NF];
+      if constexpr (SUBT) {
+        // subgridTemperatureChanges.c:53-67, same operations and order
+        const double Tm = T00 * (1.0 - mdx) * (1.0 - mdy) + T10 * mdx * (1.0 - mdy) + T01 * (1.0 - mdx) * (mdy) +
+                          T11 * mdx * mdy;
+        const double rhom = sb.rho0 + r00 * (1.0 - mdx) * (1.0 - mdy) + r10 * (mdx) * (1.0 - mdy) +
+                            r01 * (1.0 - mdx) * (mdy) + r11 * mdx * mdy;
+        const double Cpm = C00 * (1.0 - mdx) * (1.0 - mdy) + C10 * (mdx) * (1.0 - mdy) +
+                           C01 * (1.0 - mdx) * (mdy) + C11 * mdx * mdy;
+        const double km = k00 * (1.0 - mdx) * (1.0 - mdy) + k10 * (mdx) * (1.0 - mdy) + k01 * (1.0 - mdx) * (mdy) +
+                          k11 * mdx * mdy;
+        const double dTdiff = Cpm * rhom / km / sden;
+        const double d = (Tm - vs[u][0]) * (1.0 - exp(-sb.dsub * sb.dt / dTdiff));
+        sb.dTsub[m] = d;
+#pragma unroll
+        for (int k = 0; k < NF; k++) f[k] = d;
+      } else {
+        const int mat = mats[u];
+#pragma unroll
+        for (int k = 0; k < NF; k++)
+          f[k] = pg.src[k].use_mat ? (((unsigned)mat < FVM_MAXMAT) ? pg.src[k].mat[mat] : __builtin_nan(""))
+                                   : vs[u][k];
+      }
+      accumulate<METHOD, NF>(mdx, mdy, f, sw, sf);
+    }
   }
   // fixed butterfly over the 16 lanes of the group (a + b == b + a exactly,
   // so every lane ends with the same, order-fixed total)
@@ -205,7 +286,28 @@ __device__ __forceinline__ void partials_cell(const DGrid &g, const ProjGroup &p
   }
 }
 
-// ---- pass 2: one thread per node, every group of the call -------------------------
+template <int METHOD, int NF, int G>
+__device__ __forceinline__ void partials_cell(const DGrid &g, const ProjGroup &pg, const int32_t *__restrict__ start,
+                                              const int32_t *__restrict__ perm, const double *__restrict__ MX,
+                                              const double *__restrict__ MY, const int32_t *__restrict__ Mat,
+                                              double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly, int j,
+                                              bool live) {
+  const SubT none = {};
+  partials_cell_impl<METHOD, NF, G, false>(g, pg, start, perm, MX, MY, Mat, part, nL, t, lx, ly, j, live, none);
+}
+
+// The basic-node projection of subgridTemperatureChanges' dTsub with pass 1
+// computed on the fly (method ARITHMETIC, one field).
+template <int G>
+__device__ __forceinline__ void partials_cell_subt(const DGrid &g, const ProjGroup &pg,
+                                                   const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
+                                                   const double *__restrict__ MX, const double *__restrict__ MY,
+                                                   double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly,
+                                                   int j, bool live, const SubT &sb) {
+  partials_cell_impl<PM_ARITH, 1, G, true>(g, pg, start, perm, MX, MY, nullptr, part, nL, t, lx, ly, j, live, sb);
+}
+
+// ---- pass 2: one thread per (node, field) ------------------------------------------
 #define MAX_NODE_GROUPS 6
 struct NodeGroup {
   const double *part;
@@ -214,12 +316,14 @@ struct NodeGroup {
   const double *ea, *eb;
 };
 struct NodeGroups {
-  int ng;
+  int ng, npair;
   NodeGroup q[MAX_NODE_GROUPS];
+  // pair p = (group pg[p], field pk[p]): one launch row (blockIdx.y) each
+  unsigned char pg[MAX_NODE_GROUPS * FVM_MAXF], pk[MAX_NODE_GROUPS * FVM_MAXF];
 };
 
-// Pass 2 for node (ix, jy) (flat index i) of one group.
-__device__ __forceinline__ void node_group(const NodeGroup &q, int ix, int jy, int64_t i) {
+// Pass 2 for node (ix, jy) (flat index i), field k of one group.
+__device__ __forceinline__ void node_field(const NodeGroup &q, int ix, int jy, int64_t i, int k) {
   const int64_t nL = (int64_t)q.nLx * q.nLy;
   const int st = q.nf + 1;
   // (lattice cell, corner) pairs touching node (ix, jy), in a fixed order:
@@ -227,30 +331,32 @@ __device__ __forceinline__ void node_group(const NodeGroup &q, int ix, int jy, i
   const int ax0 = ix - 1 - q.lox, ax1 = ix - q.lox, ay0 = jy - 1 - q.loy, ay1 = jy - q.loy;
   const bool vx0 = ax0 >= 0 && ax0 < q.nLx, vx1 = ax1 >= 0 && ax1 < q.nLx;
   const bool vy0 = ay0 >= 0 && ay0 < q.nLy, vy1 = ay1 >= 0 && ay1 < q.nLy;
-  const int64_t L3 = (int64_t)ay0 * q.nLx + ax0, L2 = (int64_t)ay0 * q.nLx + ax1;
-  const int64_t L1 = (int64_t)ay1 * q.nLx + ax0, L0 = (int64_t)ay1 * q.nLx + ax1;
-  auto gather = [&](int v) {  // v = 0: weight, 1 + k: field k
-    double s = 0.0;
-    if (vx0 && vy0) s += q.part[(int64_t)(3 * st + v) * nL + L3];
-    if (vx1 && vy0) s += q.part[(int64_t)(2 * st + v) * nL + L2];
-    if (vx0 && vy1) s += q.part[(int64_t)(1 * st + v) * nL + L1];
-    if (vx1 && vy1) s += q.part[(int64_t)(0 * st + v) * nL + L0];
-    return s;
-  };
+  const bool v3 = vx0 && vy0, v2 = vx1 && vy0, v1 = vx0 && vy1, v0 = vx1 && vy1;
+  // absent corners read element 0 (a valid address) and are dropped below
+  const int64_t L3 = v3 ? (int64_t)ay0 * q.nLx + ax0 : 0, L2 = v2 ? (int64_t)ay0 * q.nLx + ax1 : 0;
+  const int64_t L1 = v1 ? (int64_t)ay1 * q.nLx + ax0 : 0, L0 = v0 ? (int64_t)ay1 * q.nLx + ax1 : 0;
+  const double *P = q.part;
+  const double w3 = P[(int64_t)(3 * st) * nL + L3], w2 = P[(int64_t)(2 * st) * nL + L2];
+  const double w1 = P[(int64_t)(1 * st) * nL + L1], w0 = P[L0];
+  const double f3 = P[(int64_t)(3 * st + 1 + k) * nL + L3], f2 = P[(int64_t)(2 * st + 1 + k) * nL + L2];
+  const double f1 = P[(int64_t)(1 * st + 1 + k) * nL + L1], f0 = P[(int64_t)(1 + k) * nL + L0];
   // global = 0 + local (DMLocalToGlobal ADD into a zeroed vector), then divide
-  const double den = 0.0 + gather(0);
-  for (int k = 0; k < q.nf; k++) {
-    const double num = 0.0 + gather(1 + k);
-    const double r = q.harm ? den / num : num / den;
-    double *o = q.out[k];
-    if (q.epilogue == EPI_DIVIDE) {
-      o[i] = r;
-    } else if (q.epilogue == EPI_SUBGRID_T) {
-      // VecAXPBYPCZ(z, 1, -1, -1, x=a, y=b): z = x + (-1)*y + (-1)*z
-      o[i] = q.ea[i] + -1.0 * q.eb[i] + -1.0 * r;
-    } else {
-      // VecAXPY(out, -1, r)
-      o[i] = o[i] + -1.0 * r;
-    }
+  double den = 0.0, num = 0.0;
+  if (v3) den += w3, num += f3;
+  if (v2) den += w2, num += f2;
+  if (v1) den += w1, num += f1;
+  if (v0) den += w0, num += f0;
+  den = 0.0 + den;
+  num = 0.0 + num;
+  const double r = q.harm ? den / num : num / den;
+  double *o = q.out[k];
+  if (q.epilogue == EPI_DIVIDE) {
+    o[i] = r;
+  } else if (q.epilogue == EPI_SUBGRID_T) {
+    // VecAXPBYPCZ(z, 1, -1, -1, x=a, y=b): z = x + (-1)*y + (-1)*z
+    o[i] = q.ea[i] + -1.0 * q.eb[i] + -1.0 * r;
+  } else {
+    // VecAXPY(out, -1, r)
+    o[i] = o[i] + -1.0 * r;
   }
 }

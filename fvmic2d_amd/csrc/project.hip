@@ -31,11 +31,28 @@ __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const i
   partials_cell<METHOD, NF, LPG>(g, pg, start, perm, MX, MY, Mat, part, nL, t, lx, ly, j, live);
 }
 
+__global__ void __launch_bounds__(256) k_partials_subt(DGrid g, ProjGroup pg, SubT sb,
+                                                       const int32_t *__restrict__ start,
+                                                       const int32_t *__restrict__ perm, const double *__restrict__ MX,
+                                                       const double *__restrict__ MY, double *__restrict__ part,
+                                                       int nLx, int nLy) {
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
+  const int j = threadIdx.x % LPG;
+  const int64_t nL = (int64_t)nLx * nLy;
+  const bool live = t < nL;
+  const int lx = live ? (int)(t % nLx) : 0;
+  const int ly = live ? (int)(t / nLx) : 0;
+  partials_cell_subt<LPG>(g, pg, start, perm, MX, MY, part, nL, t, lx, ly, j, live, sb);
+}
+
+// one thread per node; launch row blockIdx.y = (group, field) pair
 __global__ void __launch_bounds__(256) k_nodes(int NX, int NY, NodeGroups G) {
   const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= (int64_t)NX * NY) return;
   const int ix = (int)(i % NX), jy = (int)(i / NX);
-  for (int gi = 0; gi < G.ng; gi++) node_group(G.q[gi], ix, jy, i);
+  const int p = blockIdx.y;
+  node_field(G.q[G.pg[p]], ix, jy, i, G.pk[p]);
 }
 
 template <int METHOD, int NF>
@@ -67,7 +84,19 @@ static int dispatch_nf(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, c
 }
 
 static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg,
-                           double *part) {
+                           double *part, const SubT *sb) {
+  if (sb) {
+    if (pg.stagx != 0 || pg.stagy != 0 || pg.method != PM_ARITH || pg.nf != 1)
+      return fvm_set_err(ctx, FVM_ERR_ARG, "fused subgrid pass needs one ARITHMETIC field on basic nodes");
+    const Lat Lx = lattice(0, g->NX), Ly = lattice(0, g->NY);
+    const int64_t nL = (int64_t)Lx.n * Ly.n;
+    FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
+    k_partials_subt<<<fvm_blocks(nL * LPG, 256), 256, 0, ctx->stream>>>(
+        dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk->X, mk->Y, part, Lx.n,
+        Ly.n);
+    FVM_LAUNCH_CHECK(ctx);
+    return FVM_OK;
+  }
   switch (pg.method) {
     case PM_ARITH: return dispatch_nf<PM_ARITH>(ctx, g, mk, pg, part);
     case PM_LOCAL: return dispatch_nf<PM_LOCAL>(ctx, g, mk, pg, part);
@@ -79,8 +108,11 @@ static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *m
 }
 
 // Project several groups over one bucket order: one partials launch per group,
-// then one node launch for (up to MAX_NODE_GROUPS of) them.
-int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, int ng, const ProjGroup *pgs) {
+// then one node launch for (up to MAX_NODE_GROUPS of) them.  With `sub`, the
+// single group is subgridTemperatureChanges' dTsub, computed in the partials pass.
+int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, int ng, const ProjGroup *pgs,
+                       const SubT *sub) {
+  if (sub && ng != 1) return fvm_set_err(ctx, FVM_ERR_ARG, "fused subgrid pass takes one group");
   size_t total = 0;
   size_t off[64];
   if (ng > 64) return fvm_set_err(ctx, FVM_ERR_ARG, "too many projection groups");
@@ -97,7 +129,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
   FVM_TRY(fvm_sort_markers(ctx, g, mk));
   FVM_TRY(fvm_ws(ctx, ctx->partials, total * sizeof(double)));
   double *part = (double *)ctx->partials.ptr;
-  for (int i = 0; i < ng; i++) FVM_TRY(dispatch_method(ctx, g, mk, pgs[i], part + off[i]));
+  for (int i = 0; i < ng; i++) FVM_TRY(dispatch_method(ctx, g, mk, pgs[i], part + off[i], sub));
   const int64_t nn = (int64_t)g->NX * g->NY;
   for (int i0 = 0; i0 < ng; i0 += MAX_NODE_GROUPS) {
     NodeGroups G;
@@ -114,19 +146,24 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
       q.lox = Lx.lo;
       q.loy = Ly.lo;
       q.epilogue = pg.epilogue;
-      for (int k = 0; k < pg.nf; k++) q.out[k] = pg.out[k];
+      for (int k = 0; k < pg.nf; k++) {
+        q.out[k] = pg.out[k];
+        G.pg[G.npair] = (unsigned char)(G.ng - 1);
+        G.pk[G.npair] = (unsigned char)k;
+        G.npair++;
+      }
       q.ea = pg.ea;
       q.eb = pg.eb;
     }
     FVM_PHASE(ctx, FVM_PHASE_NODES);
-    k_nodes<<<fvm_blocks(nn, 256), 256, 0, ctx->stream>>>(g->NX, g->NY, G);
+    k_nodes<<<dim3(fvm_blocks(nn, 256), G.npair), 256, 0, ctx->stream>>>(g->NX, g->NY, G);
     FVM_LAUNCH_CHECK(ctx);
   }
   return FVM_OK;
 }
 
 int fvm_project_group(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg) {
-  return fvm_project_groups(ctx, g, mk, 1, &pg);
+  return fvm_project_groups(ctx, g, mk, 1, &pg, nullptr);
 }
 
 // ---- public projection entry points ------------------------------------------
