@@ -21,6 +21,8 @@ import platform
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -195,6 +197,75 @@ def traffic_per_launch(kernel):
     return None
 
 
+def bench_learned_guess(a):
+    """The learned Stokes initial guess: plain-method data, U-Net training on the
+    GPU kernels, and the judge's FGMRES iterations to 1e-8 from the plain,
+    previous-solution and learned guesses on held-out states."""
+    from fvmic2d_amd import learned_guess as lg
+    N = a.lg_grid
+    t0 = time.perf_counter()
+    raw_tr = lg.generate(N, list(range(a.lg_train)), workers=a.lg_workers)
+    raw_te = lg.generate(N, list(range(10 ** 6, 10 ** 6 + a.lg_test)), workers=a.lg_workers)
+    t_gen = time.perf_counter() - t0
+    print("learned guess: %d + %d states at %dx%d solved on the host in %.0f s" % (
+        a.lg_train, a.lg_test, N, N, t_gen), file=sys.stderr, flush=True)
+    from fvmic2d_amd.lib import Context
+    ctx = Context()
+    t0 = time.perf_counter()
+    tr, te = lg.Dataset(ctx, N, raw_tr), lg.Dataset(ctx, N, raw_te)
+    t_pg = time.perf_counter() - t0
+    X, T = tr.arrays()
+    model = lg.LearnedGuess(ctx, levels=4, base=32, seed=0)
+    log = lambda m: print("learned guess: " + m, file=sys.stderr, flush=True)
+    hist = model.train(X, T, epochs=a.lg_epochs, batch=8, lr=2e-3, log=log)
+    steps = a.lg_epochs * max(1, len(X) // 8)
+    # forward timing at the training batch shape
+    from fvmic2d_amd.unet import Buf
+    dX = Buf(ctx, (8, N - 1, N - 1, model.net.cin), np.uint16)
+    model.net.forward(dX)
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(10):
+        model.net.forward(dX)
+    fwd_ms = ctx.timer_stop() / 10
+    t0 = time.perf_counter()
+    rows = lg.evaluate(te, model)
+    t_eval = time.perf_counter() - t0
+    summ = lg.summarize(rows)
+    flops_fwd = model_flops(model.net, N - 1) * 8
+    out = {
+        "metric": "KSP iterations to 1e-8 relative residual: learned vs plain initial guess",
+        "value": summ["mean_iterations"]["learned"], "unit": "iterations", "n_gpus": 1,
+        "higher_is_better": False, "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+        "data": "synthetic", "scaling": "weak", "steps": steps, "warmup": 0,
+        "config": {"workload": "learned_guess_box%d" % N, "grid": "%dx%d" % (N, N),
+                   "viscosity_contrast": lg.CONTRAST, "train_states": a.lg_train, "test_states": a.lg_test,
+                   "unet": "4 levels, 32 base channels, 3x3 convs", "epochs": a.lg_epochs,
+                   "judge": "FGMRES(100) + u/p field split (LU velocity block, diagonal Schur), rtol 1e-8"},
+        "learned_guess": summ,
+        "train_loss_first_last": [hist[0], hist[-1]],
+        "seconds": {"host_solves": t_gen, "gpu_pressure_guesses": t_pg, "training": model.train_seconds,
+                    "evaluation": t_eval},
+        "unet_forward_batch8_ms": fwd_ms,
+        "roofline": {"bound": "mfma", "kernel": "unet forward (batch 8)",
+                     "achieved": flops_fwd / (fwd_ms / 1e3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": flops_fwd / (fwd_ms / 1e3) / 1e12 / 2500.0, "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+def model_flops(net, H):
+    """Multiply-add flops (x2) of one forward pass of `net` on one H x H sample."""
+    f = 0
+    for name, K, ci, co, _ in net.layers:
+        lvl = 0 if name == "out" else int(name[1])
+        h = H >> lvl
+        f += 2 * h * h * K * K * ci * co
+    return f
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -207,6 +278,13 @@ def main():
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
+    ap.add_argument("--learned-guess", action="store_true",
+                    help="train and judge the learned Stokes initial guess (own JSON line)")
+    ap.add_argument("--lg-grid", type=int, default=129)
+    ap.add_argument("--lg-train", type=int, default=512)
+    ap.add_argument("--lg-test", type=int, default=24)
+    ap.add_argument("--lg-epochs", type=int, default=40)
+    ap.add_argument("--lg-workers", type=int, default=12)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
@@ -215,6 +293,8 @@ def main():
         return 2
     if a.copy_calibration:
         return copy_calibration()
+    if a.learned_guess:
+        return bench_learned_guess(a)
     if a.ensemble:
         return bench_ensemble(a)
 

@@ -24,6 +24,9 @@ struct fvm_ctx {
   char err[512] = {0};
   // workspace buffers, grown on demand
   fvm_buf key, slot, perm, count, start, fill, blocksums, partials, scratch, staging, red;
+  fvm_buf payload;     // bucket-ordered copies of the marker columns a projection reads
+  fvm_buf payload_xy;  // bucket-ordered X, Y, Mat: valid while xy_serial == sort_serial
+  uint64_t sort_serial = 0, xy_serial = 0;
   fvm_buf pinned;  // small pinned host buffer for scalar results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-phase timing (fvm_profile_*): event pairs around launches

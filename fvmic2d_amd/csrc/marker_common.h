@@ -194,7 +194,10 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     sxy[m] = msxy0 + (msxx0 - msyy0) * wxy1;
   } else {
     const double thetaz = -wm * dt;
-    const double c = cos(thetaz), s = sin(thetaz), s2 = sin(2.0 * thetaz), c2 = cos(2.0 * thetaz);
+    // one shared range reduction per angle (sincos); the values are those of sin and cos
+    double c, s, s2, c2;
+    sincos(thetaz, &s, &c);
+    sincos(2.0 * thetaz, &s2, &c2);
     // pow(v, 2.0) of markers.c:2162-2163: gcc -O3 (makefile:14) folds it to v*v,
     // which is what the reference binary executes
     sxx[m] = msxx0 * (c * c) + msyy0 * (s * s) - msxy0 * s2;

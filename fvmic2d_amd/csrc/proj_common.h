@@ -38,6 +38,8 @@ template <int METHOD, int NF>
 __device__ __forceinline__ void accumulate(double mdx, double mdy, const double (&f)[NF], double (&sw)[4],
                                            double (&sf)[4][NF]) {
   if (METHOD == PM_ARITH) {
+    // the weights keep the reference's products; the sums use fused
+    // multiply-adds (the projection is compared to 1e-12, not bitwise)
     const double w0 = (1.0 - mdx) * (1.0 - mdy), w1 = mdx * (1.0 - mdy), w2 = (1.0 - mdx) * (mdy), w3 = mdx * mdy;
     sw[0] += w0;
     sw[1] += w1;
@@ -45,10 +47,10 @@ __device__ __forceinline__ void accumulate(double mdx, double mdy, const double 
     sw[3] += w3;
 #pragma unroll
     for (int k = 0; k < NF; k++) {
-      sf[0][k] += w0 * f[k];
-      sf[1][k] += w1 * f[k];
-      sf[2][k] += w2 * f[k];
-      sf[3][k] += w3 * f[k];
+      sf[0][k] = fma(w0, f[k], sf[0][k]);
+      sf[1][k] = fma(w1, f[k], sf[1][k]);
+      sf[2][k] = fma(w2, f[k], sf[2][k]);
+      sf[3][k] = fma(w3, f[k], sf[3][k]);
     }
   } else if (METHOD == PM_LOCAL || METHOD == PM_HARM) {
     // one corner: nearest in x (mdx >= 0.5 -> right) and y; the other corners
@@ -121,7 +123,9 @@ struct SubT {
   double rho0, dt, dsub;           // materialRho[0], dt, subgrid diffusivity
 };
 
-template <int METHOD, int NF, int G, bool SUBT>
+// SORTED: MX, MY, Mat and the source columns are payload arrays in bucket
+// order (indexed by the position in perm) instead of marker columns.
+template <int METHOD, int NF, int G, bool SUBT, bool SORTED>
 __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGroup &pg,
                                                    const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
                                                    const double *__restrict__ MX, const double *__restrict__ MY,
@@ -206,16 +210,16 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
   // Lane j takes entries j, j+G, j+2G, ... of the concatenated ranges and adds
   // them in that order.  U entries are fetched together (their loads in
   // flight at once) before they are added.
-  constexpr int U = (SUBT || NF > 4) ? 2 : 4;
+  constexpr int U = (SUBT || NF > 3) ? 2 : 4;
   constexpr int NV = SUBT ? 1 : NF;  // loaded values per marker
   for (int e0 = j; e0 < tot; e0 += U * G) {
-    int mm[U];
+    int mm[U];  // marker index (SORTED: bucket position) of entry e0 + u*G, -1 past the end
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int e = e0 + u * G;
       const int idx =
           e < p1 ? rs[0] + e : (e < p2 ? rs[1] + (e - p1) : (e < p3 ? rs[2] + (e - p2) : rs[3] + (e - p3)));
-      mm[u] = e < tot ? perm[idx] : -1;
+      mm[u] = e < tot ? (SORTED ? idx : perm[idx]) : -1;
     }
     double xs[U], ys[U], vs[U][NV];
     int mats[U];
@@ -252,7 +256,7 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
                           k11 * mdx * mdy;
         const double dTdiff = Cpm * rhom / km / sden;
         const double d = (Tm - vs[u][0]) * (1.0 - exp(-sb.dsub * sb.dt / dTdiff));
-        sb.dTsub[m] = d;
+        sb.dTsub[SORTED ? perm[m] : m] = d;
 #pragma unroll
         for (int k = 0; k < NF; k++) f[k] = d;
       } else {
@@ -293,7 +297,8 @@ __device__ __forceinline__ void partials_cell(const DGrid &g, const ProjGroup &p
                                               double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly, int j,
                                               bool live) {
   const SubT none = {};
-  partials_cell_impl<METHOD, NF, G, false>(g, pg, start, perm, MX, MY, Mat, part, nL, t, lx, ly, j, live, none);
+  partials_cell_impl<METHOD, NF, G, false, false>(g, pg, start, perm, MX, MY, Mat, part, nL, t, lx, ly, j, live,
+                                                  none);
 }
 
 // The basic-node projection of subgridTemperatureChanges' dTsub with pass 1
@@ -304,7 +309,33 @@ __device__ __forceinline__ void partials_cell_subt(const DGrid &g, const ProjGro
                                                    const double *__restrict__ MX, const double *__restrict__ MY,
                                                    double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly,
                                                    int j, bool live, const SubT &sb) {
-  partials_cell_impl<PM_ARITH, 1, G, true>(g, pg, start, perm, MX, MY, nullptr, part, nL, t, lx, ly, j, live, sb);
+  partials_cell_impl<PM_ARITH, 1, G, true, false>(g, pg, start, perm, MX, MY, nullptr, part, nL, t, lx, ly, j, live,
+                                                  sb);
+}
+
+// The same over bucket-ordered payload (xs, ys, ms and the columns named in pg
+// or sb.MT hold position-ordered copies; perm maps a position to its marker).
+template <int METHOD, int NF, int G>
+__device__ __forceinline__ void partials_cell_sorted(const DGrid &g, const ProjGroup &pg,
+                                                     const int32_t *__restrict__ start,
+                                                     const double *__restrict__ xs, const double *__restrict__ ys,
+                                                     const int32_t *__restrict__ ms, double *__restrict__ part,
+                                                     int64_t nL, int64_t t, int lx, int ly, int j, bool live) {
+  const SubT none = {};
+  partials_cell_impl<METHOD, NF, G, false, true>(g, pg, start, nullptr, xs, ys, ms, part, nL, t, lx, ly, j, live,
+                                                 none);
+}
+
+template <int G>
+__device__ __forceinline__ void partials_cell_subt_sorted(const DGrid &g, const ProjGroup &pg,
+                                                          const int32_t *__restrict__ start,
+                                                          const int32_t *__restrict__ perm,
+                                                          const double *__restrict__ xs,
+                                                          const double *__restrict__ ys, double *__restrict__ part,
+                                                          int64_t nL, int64_t t, int lx, int ly, int j, bool live,
+                                                          const SubT &sb) {
+  partials_cell_impl<PM_ARITH, 1, G, true, true>(g, pg, start, perm, xs, ys, nullptr, part, nL, t, lx, ly, j, live,
+                                                 sb);
 }
 
 // ---- pass 2: one thread per (node, field) ------------------------------------------

@@ -12,15 +12,18 @@
 // Both passes are pure functions of the bucket order, so the result is
 // bitwise reproducible; it differs from the reference's sequential sum only
 // by the association of the additions.
+#include <utility>
+#include <vector>
+
 #include "proj_common.h"
 
 #define LPG 16  // lanes per lattice cell in the single-instance launches
 
 template <int METHOD, int NF>
 __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const int32_t *__restrict__ start,
-                                                  const int32_t *__restrict__ perm, const double *__restrict__ MX,
-                                                  const double *__restrict__ MY, const int32_t *__restrict__ Mat,
-                                                  double *__restrict__ part, int nLx, int nLy, int lox, int loy) {
+                                                  const double *__restrict__ xs, const double *__restrict__ ys,
+                                                  const int32_t *__restrict__ ms, double *__restrict__ part, int nLx,
+                                                  int nLy, int lox, int loy) {
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
   const int j = threadIdx.x % LPG;
@@ -28,13 +31,49 @@ __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const i
   const bool live = t < nL;
   const int lx = live ? (int)(t % nLx) + lox : 0;
   const int ly = live ? (int)(t / nLx) + loy : 0;
-  partials_cell<METHOD, NF, LPG>(g, pg, start, perm, MX, MY, Mat, part, nL, t, lx, ly, j, live);
+  partials_cell_sorted<METHOD, NF, LPG>(g, pg, start, xs, ys, ms, part, nL, t, lx, ly, j, live);
+}
+
+// Several groups of one kind (method, field count) in one launch: the launch
+// covers lattice-cell coordinates (cx, cy) in [-1, NX-1] x [-1, NY-1] and each
+// lane group computes, family after family, the lattice cell at (cx, cy) of
+// every group whose lattice contains it.  Neighbouring families' cells at one
+// coordinate cover nearly the same markers, so the later families read them
+// from cache.
+#define MAX_MULTI 3
+struct MultiGroups {
+  int ng;
+  ProjGroup pg[MAX_MULTI];
+  double *part[MAX_MULTI];
+};
+
+template <int METHOD, int NF>
+__global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg, const int32_t *__restrict__ start,
+                                                        const double *__restrict__ xs, const double *__restrict__ ys,
+                                                        const int32_t *__restrict__ ms) {
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
+  const int j = threadIdx.x % LPG;
+  const int64_t nT = (int64_t)(g.NX + 1) * (g.NY + 1);
+  const bool inside = t < nT;
+  const int cx = inside ? (int)(t % (g.NX + 1)) - 1 : -2;
+  const int cy = inside ? (int)(t / (g.NX + 1)) - 1 : -2;
+#pragma unroll 1
+  for (int q = 0; q < mg.ng; q++) {
+    const ProjGroup &pg = mg.pg[q];
+    const Lat Lx = lattice(pg.stagx, g.NX), Ly = lattice(pg.stagy, g.NY);
+    const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
+    const int64_t nL = (int64_t)Lx.n * Ly.n;
+    const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
+    partials_cell_sorted<METHOD, NF, LPG>(g, pg, start, xs, ys, ms, mg.part[q], nL, tl, live ? cx : 0,
+                                          live ? cy : 0, j, live);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_partials_subt(DGrid g, ProjGroup pg, SubT sb,
                                                        const int32_t *__restrict__ start,
-                                                       const int32_t *__restrict__ perm, const double *__restrict__ MX,
-                                                       const double *__restrict__ MY, double *__restrict__ part,
+                                                       const int32_t *__restrict__ perm, const double *__restrict__ xs,
+                                                       const double *__restrict__ ys, double *__restrict__ part,
                                                        int nLx, int nLy) {
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
@@ -43,7 +82,32 @@ __global__ void __launch_bounds__(256) k_partials_subt(DGrid g, ProjGroup pg, Su
   const bool live = t < nL;
   const int lx = live ? (int)(t % nLx) : 0;
   const int ly = live ? (int)(t / nLx) : 0;
-  partials_cell_subt<LPG>(g, pg, start, perm, MX, MY, part, nL, t, lx, ly, j, live, sb);
+  partials_cell_subt_sorted<LPG>(g, pg, start, perm, xs, ys, part, nL, t, lx, ly, j, live, sb);
+}
+
+// Bucket-ordered payload: position p of the sort gets marker perm[p]'s X, Y,
+// Mat and the listed columns, so the partials passes read contiguous runs.
+#define MAX_PAYLOAD_COLS 24
+struct PayloadCols {
+  int nc;
+  const double *src[MAX_PAYLOAD_COLS];
+  double *dst[MAX_PAYLOAD_COLS];
+};
+
+__global__ void __launch_bounds__(256) k_gather_payload(int32_t n, const int32_t *__restrict__ total,
+                                                        const int32_t *__restrict__ perm, const double *__restrict__ X,
+                                                        const double *__restrict__ Y, const int32_t *__restrict__ Mat,
+                                                        double *__restrict__ xs, double *__restrict__ ys,
+                                                        int32_t *__restrict__ ms, PayloadCols pc, int with_xy) {
+  const int64_t p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p >= n || p >= *total) return;
+  const int m = perm[p];
+  if (with_xy) {
+    xs[p] = X[m];
+    ys[p] = Y[m];
+    ms[p] = Mat[m];
+  }
+  for (int c = 0; c < pc.nc; c++) pc.dst[c][p] = pc.src[c][m];
 }
 
 // one thread per node; launch row blockIdx.y = (group, field) pair
@@ -55,21 +119,57 @@ __global__ void __launch_bounds__(256) k_nodes(int NX, int NY, NodeGroups G) {
   node_field(G.q[G.pg[p]], ix, jy, i, G.pk[p]);
 }
 
+struct Payload {
+  const double *xs, *ys;
+  const int32_t *ms;
+};
+
 template <int METHOD, int NF>
-static int launch_partials(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg,
+static int launch_partials(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, const ProjGroup &pg,
                            double *part) {
   const Lat Lx = lattice(pg.stagx, g->NX), Ly = lattice(pg.stagy, g->NY);
   const int64_t nL = (int64_t)Lx.n * Ly.n;
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
   k_partials<METHOD, NF><<<fvm_blocks(nL * LPG, 256), 256, 0, ctx->stream>>>(
-      dgrid(g), pg, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk->X, mk->Y, mk->Mat, part,
-      Lx.n, Ly.n, Lx.lo, Ly.lo);
+      dgrid(g), pg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms, part, Lx.n, Ly.n, Lx.lo, Ly.lo);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
 
+template <int METHOD, int NF>
+static int launch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, const MultiGroups &mg) {
+  const int64_t nT = (int64_t)(g->NX + 1) * (g->NY + 1);
+  FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
+  k_partials_multi<METHOD, NF><<<fvm_blocks(nT * LPG, 256), 256, 0, ctx->stream>>>(
+      dgrid(g), mg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+// multi-group launches exist for the common kinds; others go one group per launch
+static int dispatch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, const MultiGroups &mg, bool *done) {
+  *done = true;
+  const int m = mg.pg[0].method, nf = mg.pg[0].nf;
+  if (m == PM_ARITH) {
+    switch (nf) {
+      case 1: return launch_multi<PM_ARITH, 1>(ctx, g, pl, mg);
+      case 2: return launch_multi<PM_ARITH, 2>(ctx, g, pl, mg);
+      case 3: return launch_multi<PM_ARITH, 3>(ctx, g, pl, mg);
+      case 4: return launch_multi<PM_ARITH, 4>(ctx, g, pl, mg);
+      case 5: return launch_multi<PM_ARITH, 5>(ctx, g, pl, mg);
+    }
+  } else if (m == PM_LOCAL) {
+    switch (nf) {
+      case 1: return launch_multi<PM_LOCAL, 1>(ctx, g, pl, mg);
+      case 2: return launch_multi<PM_LOCAL, 2>(ctx, g, pl, mg);
+    }
+  }
+  *done = false;
+  return FVM_OK;
+}
+
 template <int METHOD>
-static int dispatch_nf(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg, double *part) {
+static int dispatch_nf(fvm_ctx *ctx, const fvm_grid *g, const Payload &mk, const ProjGroup &pg, double *part) {
   switch (pg.nf) {
     case 1: return launch_partials<METHOD, 1>(ctx, g, mk, pg, part);
     case 2: return launch_partials<METHOD, 2>(ctx, g, mk, pg, part);
@@ -83,7 +183,7 @@ static int dispatch_nf(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, c
   }
 }
 
-static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg,
+static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const Payload &mk, const ProjGroup &pg,
                            double *part, const SubT *sb) {
   if (sb) {
     if (pg.stagx != 0 || pg.stagy != 0 || pg.method != PM_ARITH || pg.nf != 1)
@@ -92,7 +192,7 @@ static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *m
     const int64_t nL = (int64_t)Lx.n * Ly.n;
     FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
     k_partials_subt<<<fvm_blocks(nL * LPG, 256), 256, 0, ctx->stream>>>(
-        dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk->X, mk->Y, part, Lx.n,
+        dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk.xs, mk.ys, part, Lx.n,
         Ly.n);
     FVM_LAUNCH_CHECK(ctx);
     return FVM_OK;
@@ -129,7 +229,75 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
   FVM_TRY(fvm_sort_markers(ctx, g, mk));
   FVM_TRY(fvm_ws(ctx, ctx->partials, total * sizeof(double)));
   double *part = (double *)ctx->partials.ptr;
-  for (int i = 0; i < ng; i++) FVM_TRY(dispatch_method(ctx, g, mk, pgs[i], part + off[i], sub));
+  // payload: X, Y, Mat and every distinct source column, in bucket order
+  PayloadCols pc;
+  memset(&pc, 0, sizeof(pc));
+  auto col_slot = [&](const double *c) -> int {
+    for (int k = 0; k < pc.nc; k++)
+      if (pc.src[k] == c) return k;
+    if (pc.nc == MAX_PAYLOAD_COLS) return -1;
+    pc.src[pc.nc] = c;
+    return pc.nc++;
+  };
+  std::vector<ProjGroup> sg(pgs, pgs + ng);
+  std::vector<std::pair<int, int>> uses;  // (group * FVM_MAXF + field, slot)
+  for (int i = 0; i < ng; i++)
+    for (int k = 0; k < sg[i].nf; k++)
+      if (!sg[i].src[k].use_mat && !sub) {
+        const int sl = col_slot(sg[i].src[k].col);
+        if (sl < 0) return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "more than %d distinct marker columns", MAX_PAYLOAD_COLS);
+        uses.push_back({i * FVM_MAXF + k, sl});
+      }
+  SubT ssub;
+  if (sub) {
+    ssub = *sub;
+    col_slot(sub->MT);
+  }
+  const int32_t n = mk->nMark;
+  const size_t np = ((size_t)(n > 0 ? n : 1) + 31) / 32 * 32;
+  // X, Y, Mat in bucket order are kept for the life of the sort
+  const bool have_xy = ctx->xy_serial == ctx->sort_serial && ctx->payload_xy.bytes >= np * 20;
+  if (!have_xy) FVM_TRY(fvm_ws(ctx, ctx->payload_xy, np * (2 * sizeof(double) + sizeof(int32_t))));
+  double *xs = (double *)ctx->payload_xy.ptr, *ys = xs + np;
+  int32_t *ms = (int32_t *)(ys + np);
+  if (pc.nc > 0) FVM_TRY(fvm_ws(ctx, ctx->payload, np * sizeof(double) * pc.nc));
+  for (int k = 0; k < pc.nc; k++) pc.dst[k] = (double *)ctx->payload.ptr + np * k;
+  for (auto &u : uses) sg[u.first / FVM_MAXF].src[u.first % FVM_MAXF].col = pc.dst[u.second];
+  if (sub) ssub.MT = pc.dst[0];
+  if (n > 0 && (!have_xy || pc.nc > 0)) {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+    k_gather_payload<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, (const int32_t *)ctx->start.ptr + nk,
+                                                                  (const int32_t *)ctx->perm.ptr, mk->X, mk->Y,
+                                                                  mk->Mat, xs, ys, ms, pc, have_xy ? 0 : 1);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  ctx->xy_serial = ctx->sort_serial;
+  const Payload pl = {xs, ys, ms};
+  // groups of one kind share a launch (up to MAX_MULTI per launch)
+  std::vector<bool> launched(ng, false);
+  if (!sub) {
+    for (int i = 0; i < ng; i++) {
+      if (launched[i]) continue;
+      MultiGroups mg;
+      memset(&mg, 0, sizeof(mg));
+      std::vector<int> members;
+      for (int k = i; k < ng && mg.ng < MAX_MULTI; k++) {
+        if (launched[k] || sg[k].method != sg[i].method || sg[k].nf != sg[i].nf) continue;
+        mg.pg[mg.ng] = sg[k];
+        mg.part[mg.ng] = part + off[k];
+        mg.ng++;
+        members.push_back(k);
+      }
+      if (mg.ng < 2) continue;
+      bool done = false;
+      FVM_TRY(dispatch_multi(ctx, g, pl, mg, &done));
+      if (done)
+        for (int k : members) launched[k] = true;
+    }
+  }
+  for (int i = 0; i < ng; i++)
+    if (!launched[i]) FVM_TRY(dispatch_method(ctx, g, pl, sg[i], part + off[i], sub ? &ssub : nullptr));
   const int64_t nn = (int64_t)g->NX * g->NY;
   for (int i0 = 0; i0 < ng; i0 += MAX_NODE_GROUPS) {
     NodeGroups G;

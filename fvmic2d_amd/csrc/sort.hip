@@ -290,6 +290,7 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
   k_bucket_order<<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm);
   FVM_LAUNCH_CHECK(ctx);
   ctx->sort_epoch = ctx->epoch;
+  ctx->sort_serial++;
   ctx->sort_x = g->x;
   ctx->sort_X = mk->X;
   ctx->sort_n = n;

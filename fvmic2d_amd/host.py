@@ -258,3 +258,22 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def initial_pressure_guess(ctx, grid, rho, gy, gx=0.0, rtol=1e-12, maxit=20000, S=None, p_out=None):
+    """initialPressureGuess (initialPressureGuess.c:6-184) on the GPU.
+
+    grid: DeviceGrid; rho, S (3*NX*NY interleaved [vx, vy, p]) and p_out
+    (NX*NY) are DeviceArray objects (or None for S / p_out).  Returns
+    (iterations, relative residual)."""
+    it, rel = I32(), C.c_double()
+    ctx.call("fvm_initial_pressure_guess", grid.ref, P(rho.ptr), gy, gx, rtol, I32(maxit),
+             P(S.ptr) if S is not None else None, P(p_out.ptr) if p_out is not None else None,
+             C.byref(it), C.byref(rel))
+    return it.value, rel.value
+
+
+def pressure_guess_apply(ctx, grid, rho, gy, x, y, b=None, gx=0.0):
+    """y = A x for the pressure-guess operator (and its right-hand side into b)."""
+    ctx.call("fvm_pressure_guess_apply", grid.ref, P(rho.ptr), gy, gx, P(x.ptr), P(y.ptr),
+             P(b.ptr) if b is not None else None)

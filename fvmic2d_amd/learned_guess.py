@@ -1,0 +1,290 @@
+"""Learned initial guess for the Stokes solve, end to end.
+
+1. States: a variable-viscosity box (viscosity 10^(4 B), density rho0 - drho B
+   from a seeded sum of Gaussian blobs B) and a "previous" state whose blobs
+   sit slightly displaced, as between two time steps.
+2. The plain method's own solves give the training targets: the previous and
+   current Stokes solutions (host sparse direct solve of the same system the
+   judge iterates on) and the plain pressure guess of both states
+   (initialPressureGuess on the GPU).
+3. The U-Net (fvmic2d_amd.unet, hand-written MFMA kernels) maps
+   [log10 eta / 4, (rho - rho0) / drho, vx_prev, vy_prev, q_prev] to
+   [vx, vy, q] on the node grid, where q = p - p_plain is the pressure minus
+   the plain guess; velocities and q are scaled by dataset constants.
+4. The judge: FGMRES + field split to a 1e-8 relative residual (host), from
+   the plain guess (zero velocity, pressure from initialPressureGuess), from
+   the previous solution, and from the learned guess -- which falls back to
+   the plain guess whenever its initial residual is larger.
+"""
+import time
+from multiprocessing import get_context
+
+import numpy as np
+import scipy.sparse.linalg as sla
+
+from . import stokes, workloads
+
+G = 10.0
+RHO0, DRHO = 3300.0, 100.0
+CONTRAST = 1e4
+
+
+class BoxGrid:
+    def __init__(self, N, L=1.0):
+        x = workloads.spacing_uniform(N, L)
+        self.xp, self.xcp = workloads.pad_grid(x, L)
+        self.yp, self.ycp = self.xp, self.xcp
+        self.NX = self.NY = N
+        self.LX = self.LY = L
+
+
+def blob_field(N, seed, shift=0.0):
+    """Normalised blob sum B in [0, 1] on the N x N nodes; shift displaces
+    every blob by a seeded random vector of length `shift` (the previous state)."""
+    r = np.random.default_rng(seed)
+    nb = int(r.integers(4, 9))
+    cx, cy = r.uniform(0.1, 0.9, nb), r.uniform(0.1, 0.9, nb)
+    s = r.uniform(0.05, 0.2, nb)
+    a = r.uniform(0.3, 1.0, nb)
+    ang = r.uniform(0, 2 * np.pi, nb)
+    x = np.linspace(0.0, 1.0, N)
+    X, Y = np.meshgrid(x, x)
+    B = np.zeros((N, N))
+    for k in range(nb):
+        ux = cx[k] + shift * np.cos(ang[k])
+        uy = cy[k] + shift * np.sin(ang[k])
+        B += a[k] * np.exp(-((X - ux) ** 2 + (Y - uy) ** 2) / s[k] ** 2)
+    return B / B.max()
+
+
+def fields(B):
+    """etaS (nodes), etaN (cell centres, from the four surrounding nodes), mu, rho."""
+    N = B.shape[0]
+    etaS = CONTRAST ** B
+    lg = np.log10(etaS)
+    etaN = np.ones_like(etaS)
+    etaN[1:, 1:] = 10.0 ** (0.25 * (lg[1:, 1:] + lg[:-1, 1:] + lg[1:, :-1] + lg[:-1, :-1]))
+    mu = np.full((N, N), 1e99)  # viscous limit (Z = 1)
+    rho = RHO0 - DRHO * B
+    return etaS, etaN, mu, rho
+
+
+def direct_solution(N, B):
+    g = BoxGrid(N)
+    etaS, etaN, mu, rho = fields(B)
+    A, b, Kb, Kc = stokes.assemble(stokes.BoxGrid.of(g), etaS, etaN, mu, mu, rho, 1.0, G)
+    return sla.spsolve(A.tocsc(), b), Kc
+
+
+def _worker(args):
+    N, seed, shift = args
+    Bp = blob_field(N, seed, shift)
+    B = blob_field(N, seed, 0.0)
+    xp, _ = direct_solution(N, Bp)
+    x, Kc = direct_solution(N, B)
+    return seed, B, Bp, xp, x, Kc
+
+
+def pressure_plain(ctx, grid, rho_dev, p_dev, Kc, N):
+    """initialPressureGuess on the GPU, scaled to the Stokes unknown (p / Kcont)
+    and re-pinned to 0 in cell (1, 1) like the judge's system."""
+    from .host import initial_pressure_guess
+    initial_pressure_guess(ctx, grid, rho_dev, G, rtol=1e-10, maxit=50000, p_out=p_dev)
+    p = p_dev.get().reshape(N, N) / Kc
+    return p - p[1, 1]
+
+
+def enforce_box(x, N):
+    """Give a guess the values its boundary rows prescribe: wall velocities 0,
+    mirrored ghost rows, pressure ghost rows copied, pressure 0 in cell (1, 1)."""
+    n = N * N
+    vx = x[:n].reshape(N, N).copy()
+    vy = x[n:2 * n].reshape(N, N).copy()
+    p = x[2 * n:].reshape(N, N).copy()
+    vx[:, 0] = 0.0
+    vx[:, N - 1] = 0.0
+    vx[N - 1, :] = vx[N - 2, :]
+    vy[0, :] = 0.0
+    vy[N - 1, :] = 0.0
+    vy[:, N - 1] = vy[:, N - 2]
+    p = p - p[1, 1]
+    p[0, :] = p[1, :]
+    p[:, 0] = p[:, 1]
+    return np.concatenate([vx.ravel(), vy.ravel(), p.ravel()])
+
+
+def generate(N, seeds, shift=0.03, workers=0):
+    """The plain method's direct solves for every seed (host only).  With
+    workers > 0 they run in forked worker processes: call it before anything
+    in this process has touched the GPU."""
+    jobs = [(N, s, shift) for s in seeds]
+    if workers > 0:
+        with get_context("fork").Pool(workers) as pool:
+            return pool.map(_worker, jobs, chunksize=1)
+    return [_worker(j) for j in jobs]
+
+
+class Dataset:
+    """Samples at N x N nodes; the network sees the first H = N - 1 rows and columns."""
+
+    def __init__(self, ctx, N, raw):
+        from .lib import DeviceArray, DeviceGrid
+        self.N, self.H = N, N - 1
+        g = BoxGrid(N)
+        dgrid = DeviceGrid(ctx, g)
+        n = N * N
+        rho_dev = DeviceArray(ctx, np.zeros(n))
+        p_dev = DeviceArray(ctx, np.zeros(n))
+        self.samples = []
+        for seed, B, Bp, xp, x, Kc in raw:
+            rho = fields(B)[3]
+            rhop = fields(Bp)[3]
+            rho_dev.set(rho)
+            pp_cur = pressure_plain(ctx, dgrid, rho_dev, p_dev, Kc, N)
+            rho_dev.set(rhop)
+            pp_prev = pressure_plain(ctx, dgrid, rho_dev, p_dev, Kc, N)
+            self.samples.append(dict(seed=seed, B=B, Bp=Bp, xprev=xp, x=x, Kc=Kc, pplain=pp_cur,
+                                     pplain_prev=pp_prev))
+
+    def arrays(self):
+        """Raw (unscaled) inputs [S, H, H, 5] and targets [S, H, H, 3]."""
+        N, H, n = self.N, self.H, self.N * self.N
+        S = len(self.samples)
+        X = np.zeros((S, H, H, 5), np.float32)
+        T = np.zeros((S, H, H, 3), np.float32)
+        for k, s in enumerate(self.samples):
+            X[k, ..., 0] = (np.log10(CONTRAST ** s["B"]) / 4.0)[:H, :H]
+            X[k, ..., 1] = (-s["B"])[:H, :H]
+            X[k, ..., 2] = s["xprev"][:n].reshape(N, N)[:H, :H]
+            X[k, ..., 3] = s["xprev"][n:2 * n].reshape(N, N)[:H, :H]
+            X[k, ..., 4] = (s["xprev"][2 * n:].reshape(N, N) - s["pplain_prev"])[:H, :H]
+            T[k, ..., 0] = s["x"][:n].reshape(N, N)[:H, :H]
+            T[k, ..., 1] = s["x"][n:2 * n].reshape(N, N)[:H, :H]
+            T[k, ..., 2] = (s["x"][2 * n:].reshape(N, N) - s["pplain"])[:H, :H]
+        return X, T
+
+
+class LearnedGuess:
+    """A trained U-Net plus the scales that map its channels to the Stokes unknowns."""
+
+    def __init__(self, ctx, levels=4, base=32, seed=0):
+        from .unet import UNet
+        self.ctx = ctx
+        self.net = UNet(ctx, levels=levels, base=base, cin=5, cout=3, seed=seed)
+        self.vs = self.qs = 1.0
+
+    def _scaled(self, X, T=None):
+        Xs = X.copy()
+        Xs[..., 2:4] /= self.vs
+        Xs[..., 4] /= self.qs
+        if T is None:
+            return Xs
+        Ts = T.copy()
+        Ts[..., 0:2] /= self.vs
+        Ts[..., 2] /= self.qs
+        return Xs, Ts
+
+    def train(self, X, T, epochs=30, batch=8, lr=2e-3, seed=0, log=None):
+        from .unet import Buf, to_bf16_bits
+        self.vs = float(np.sqrt(np.mean(T[..., 0:2] ** 2))) or 1.0
+        self.qs = float(np.sqrt(np.mean(T[..., 2] ** 2))) or 1.0
+        Xs, Ts = self._scaled(X, T)
+        S, H, W, _ = Xs.shape
+        cin = self.net.cin
+        xb = np.zeros((batch, H, W, cin), np.float32)
+        dX = Buf(self.ctx, (batch, H, W, cin), np.uint16)
+        dT = Buf(self.ctx, (batch, H, W, 3), np.float32)
+        rng = np.random.default_rng(seed)
+        steps = max(1, S // batch)
+        total = epochs * steps
+        hist = []
+        t0 = time.time()
+        it = 0
+        for ep in range(epochs):
+            order = rng.permutation(S)
+            el = 0.0
+            for s in range(steps):
+                idx = order[s * batch:(s + 1) * batch]
+                xb[...] = 0.0
+                xb[..., :5] = Xs[idx]
+                dX.set(to_bf16_bits(xb))
+                dT.set(Ts[idx])
+                frac = it / total
+                self.net.forward(dX)
+                el += self.net.loss(dT)
+                self.net.backward()
+                self.net.adam(lr=lr * 0.5 * (1.0 + np.cos(np.pi * frac)))
+                it += 1
+            hist.append(el / steps)
+            if log:
+                log("epoch %d loss %.4e (%.1f s)" % (ep, hist[-1], time.time() - t0))
+        self.train_seconds = time.time() - t0
+        return hist
+
+    def predict(self, X):
+        """Raw inputs [S, H, W, 5] -> raw outputs [S, H, W, 3] (vx, vy, q)."""
+        from .unet import Buf, to_bf16_bits
+        S, H, W, _ = X.shape
+        Xs = self._scaled(X)
+        out = np.zeros((S, H, W, 3), np.float32)
+        dX = Buf(self.ctx, (1, H, W, self.net.cin), np.uint16)
+        xb = np.zeros((1, H, W, self.net.cin), np.float32)
+        for k in range(S):
+            xb[...] = 0.0
+            xb[0, ..., :5] = Xs[k]
+            dX.set(to_bf16_bits(xb))
+            o = self.net.forward(dX).get()[0, ..., :3]
+            out[k] = o
+        out[..., 0:2] *= self.vs
+        out[..., 2] *= self.qs
+        return out
+
+
+def guess_from_output(o, pplain, N):
+    """Network output on the first N-1 rows / columns -> full Stokes vector."""
+    H = N - 1
+    full = np.zeros((3, N, N))
+    for c in range(3):
+        f = np.zeros((N, N))
+        f[:H, :H] = o[..., c]
+        f[H, :H] = f[H - 1, :H]
+        f[:, H] = f[:, H - 1]
+        full[c] = f
+    full[2] += pplain
+    return enforce_box(full.reshape(-1), N)
+
+
+def evaluate(ds, model, rtol=1e-8, restart=100):
+    """FGMRES iterations from the plain guess, the previous solution and the
+    learned guess (with fallback) for every sample of ds."""
+    N, n = ds.N, ds.N * ds.N
+    X, _ = ds.arrays()
+    pred = model.predict(X)
+    g = stokes.BoxGrid.of(BoxGrid(N))
+    rows = []
+    for k, s in enumerate(ds.samples):
+        etaS, etaN, mu, rho = fields(s["B"])
+        A, b, Kb, Kc = stokes.assemble(g, etaS, etaN, mu, mu, rho, 1.0, G)
+        zN = stokes.viscosity_z(etaS, etaN, mu, mu, 1.0)[1]
+        M = stokes.FieldSplit(A, Kc, zN)
+        plain = enforce_box(np.concatenate([np.zeros(2 * n), s["pplain"].ravel()]), N)
+        prev = s["xprev"]
+        learned = guess_from_output(pred[k], s["pplain"], N)
+        r0 = {nm: np.linalg.norm(b - A @ v) / np.linalg.norm(b) for nm, v in
+              (("plain", plain), ("prev", prev), ("learned", learned))}
+        used = learned if r0["learned"] <= r0["plain"] else plain
+        res = {}
+        for nm, v in (("plain", plain), ("prev", prev), ("learned", used)):
+            _, its, rel = stokes.fgmres(A, b, v, M, rtol=rtol, restart=restart, maxit=3000)
+            res[nm] = its
+        rows.append(dict(seed=s["seed"], r0=r0, its=res, fallback=r0["learned"] > r0["plain"]))
+    return rows
+
+
+def summarize(rows):
+    its = {k: float(np.mean([r["its"][k] for r in rows])) for k in ("plain", "prev", "learned")}
+    r0 = {k: float(np.exp(np.mean([np.log(r["r0"][k]) for r in rows]))) for k in ("plain", "prev", "learned")}
+    return {"mean_iterations": its, "geomean_initial_relres": r0,
+            "reduction_vs_plain": 1.0 - its["learned"] / its["plain"],
+            "reduction_vs_previous": 1.0 - its["learned"] / its["prev"],
+            "fallbacks": int(sum(r["fallback"] for r in rows)), "samples": len(rows)}

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfvmic2d_
 
 ERRORS = {0: "FVM_OK", 1: "FVM_ERR_PERIODIC", 2: "FVM_ERR_CELL_OUT_OF_BOUNDS", 3: "FVM_ERR_BAD_STAG",
           4: "FVM_ERR_BAD_METHOD", 5: "FVM_ERR_NAN_RESIDUAL", 6: "FVM_ERR_VISCOSITY_LAW",
-          7: "FVM_ERR_NOMEM", 8: "FVM_ERR_HIP", 9: "FVM_ERR_ARG", 10: "FVM_ERR_TOO_MANY_FIELDS"}
+          7: "FVM_ERR_NOMEM", 8: "FVM_ERR_HIP", 9: "FVM_ERR_ARG", 10: "FVM_ERR_TOO_MANY_FIELDS",
+          11: "FVM_ERR_NOT_CONVERGED"}
 
 
 class FvmError(RuntimeError):
@@ -95,6 +96,8 @@ _SIGS = {
     "fvm_profile_read": [P, C.POINTER(C.c_double), C.POINTER(I32)],
     "fvm_timer_start": [P],
     "fvm_timer_stop": [P, C.POINTER(C.c_float)],
+    "fvm_initial_pressure_guess": [P, P, P, D, D, D, I32, P, P, C.POINTER(I32), C.POINTER(D)],
+    "fvm_pressure_guess_apply": [P, P, P, D, D, P, P, P],
     "fvm_ensemble_create": [P, I32, P, C.POINTER(P)],
     "fvm_ensemble_destroy": [P],
     "fvm_ensemble_step": [P, P],

@@ -47,7 +47,8 @@ enum {
   FVM_ERR_NOMEM = 7,
   FVM_ERR_HIP = 8,               /* a HIP runtime call failed; see fvm_last_error */
   FVM_ERR_ARG = 9,               /* invalid argument (NULL pointer, size) */
-  FVM_ERR_TOO_MANY_FIELDS = 10   /* more projected fields in one group than supported */
+  FVM_ERR_TOO_MANY_FIELDS = 10,  /* more projected fields in one group than supported */
+  FVM_ERR_NOT_CONVERGED = 11     /* an iterative solve missed its tolerance */
 };
 
 /* ---- device marker store: one device column per Marker field --------------- */
@@ -234,6 +235,25 @@ int fvm_profile_read(fvm_ctx *ctx, double *ms, int32_t *launches);
 int fvm_timer_start(fvm_ctx *ctx);
 /* Milliseconds between fvm_timer_start and now (synchronises). */
 int fvm_timer_stop(fvm_ctx *ctx, float *ms);
+
+/* ---- A15: initialPressureGuess.c:6-184 ----------------------------------------
+ * The plain initial pressure guess: the 5-point variable-spacing Poisson
+ * problem for the cell-centre pressure with the density-gradient source,
+ * assembled as in initialPressureGuess.c:59-135 (dummy rows on ix == 0 and
+ * jy == 0, pressure pinned in cell (3, 3), zero-flux sides, dp/dy = rho g on
+ * top and bottom), solved on the GPU (matrix-free operator, Jacobi BiCGStab,
+ * relative residual <= rtol, at most maxit iterations).  rho: device NX*NY
+ * nodal density.  The pressure goes to S[3*(jy*NX+ix) + 2] (the DOF_P slot of
+ * the interleaved [vx, vy, p] Stokes vector, fdcode.h:242-245; velocities are
+ * left untouched) and/or p_out (NX*NY); either may be NULL but not both.
+ * Returns FVM_ERR_NOT_CONVERGED when rtol is not met (p is still written). */
+int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const double *rho, double gy,
+                               double gx, double rtol, int32_t maxit, double *S, double *p_out,
+                               int32_t *iters, double *relres);
+/* The same operator applied to x (y = A x) and, if b != NULL, its right-hand
+ * side -- for callers that run their own solver. */
+int fvm_pressure_guess_apply(fvm_ctx *ctx, const fvm_grid *g, const double *rho, double gy,
+                             double gx, const double *x, double *y, double *b);
 
 /* ---- batched ensemble of independent problems ------------------------------
  * No counterpart in the reference, which runs a parameter sweep as separate
