@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(256) k_mse(const float *__restrict__ out, cons
     if (c < Creal) {
       const float m = mask ? mask[p * Creal + c] : 1.f;
       const float d = (out[i] - tgt[p * Creal + c]) * m;
-      l = d * d;
+      l = d * d * inv_count;
       g = 2.f * d * inv_count;
     }
     dOut[i] = f2bf(g);
