@@ -89,9 +89,9 @@ def phase_profile(ctx, step, nsteps):
 
 
 def bench_ensemble(a):
-    """The batched ensemble: 64 convection_tdep-style instances, grids 65^2..257^2,
-    16-36 markers per cell, viscosity contrast 1e2-1e6, against the same
-    instances stepped one fvm_step_thermal call after another."""
+    """The batched ensemble: 64 convection_tdep-style instances, grids 65^2..257^2
+    (--ens-nmin/--ens-nmax), 16-36 markers per cell, viscosity contrast 1e2-1e6,
+    against the same instances stepped one fvm_step_thermal call after another."""
     from fvmic2d_amd import workloads
     from fvmic2d_amd.host import Ensemble, MarkerPath
     from fvmic2d_amd.lib import Context
@@ -99,7 +99,7 @@ def bench_ensemble(a):
     t0 = time.perf_counter()
     ws = []
     for i in range(0, a.instances, 8):
-        ws += workloads.ensemble(min(8, a.instances - i), seed=4, nmin=65, nmax=257, first=i)
+        ws += workloads.ensemble(min(8, a.instances - i), seed=4, nmin=a.ens_nmin, nmax=a.ens_nmax, first=i)
         print("ensemble: generated %d instances (%.0f s)" % (len(ws), time.perf_counter() - t0),
               file=sys.stderr, flush=True)
     ctx = Context()
@@ -275,7 +275,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--ensemble", action="store_true",
                     help="time the batched 64-instance ensemble instead (own JSON line)")
+    ap.add_argument("--lanes", type=int, default=8, help="lanes per lattice cell in the projections")
     ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--ens-nmin", type=int, default=65)
+    ap.add_argument("--ens-nmax", type=int, default=257)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--learned-guess", action="store_true",
@@ -303,6 +306,9 @@ def main():
 
     w = workloads.convection_tdep()
     mp = MarkerPath(w)
+    if a.lanes != 8:
+        import ctypes as C
+        mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(a.lanes))
     n0 = w.n_markers
     for _ in range(a.warmup):
         mp.step_thermal()

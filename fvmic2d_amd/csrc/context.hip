@@ -78,8 +78,9 @@ extern "C" int fvm_ctx_create(fvm_ctx **out, void *stream) {
 extern "C" int fvm_ctx_destroy(fvm_ctx *ctx) {
   if (!ctx) return FVM_OK;
   (void)hipStreamSynchronize(ctx->stream);
-  fvm_buf *bufs[] = {&ctx->key,      &ctx->slot,    &ctx->perm,    &ctx->count, &ctx->start, &ctx->fill,
-                     &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging, &ctx->red};
+  fvm_buf *bufs[] = {&ctx->key,      &ctx->slot,     &ctx->perm,    &ctx->count,   &ctx->start,
+                     &ctx->fill,     &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
+                     &ctx->red,      &ctx->payload,   &ctx->payload_xy};
   for (fvm_buf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
@@ -100,11 +101,19 @@ extern "C" int fvm_ctx_sync(fvm_ctx *ctx) {
 }
 extern "C" size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx) {
   if (!ctx) return 0;
-  const fvm_buf *bufs[] = {&ctx->key,      &ctx->slot,    &ctx->perm,    &ctx->count, &ctx->start, &ctx->fill,
-                           &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging, &ctx->red};
+  const fvm_buf *bufs[] = {&ctx->key,  &ctx->slot,    &ctx->perm,     &ctx->count,   &ctx->start,
+                           &ctx->fill, &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
+                           &ctx->red,  &ctx->payload,   &ctx->payload_xy};
   size_t s = 0;
   for (const fvm_buf *b : bufs) s += b->bytes;
   return s;
+}
+
+extern "C" int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes) {
+  if (!ctx) return FVM_ERR_ARG;
+  if (lanes != 8 && lanes != 16 && lanes != 32) return fvm_set_err(ctx, FVM_ERR_ARG, "lanes must be 8, 16 or 32");
+  ctx->lanes = lanes;
+  return FVM_OK;
 }
 
 extern "C" int fvm_device_count(int *n) {

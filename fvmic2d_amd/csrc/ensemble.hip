@@ -38,7 +38,7 @@ struct InstDev {
 
 struct fvm_ensemble {
   int n = 0, ng = 0;
-  fvm_tuning tune = {256, 16};
+  fvm_tuning tune = {256, 8};
   std::vector<fvm_instance> inst;
   std::vector<InstDev> hinst;
   std::vector<ProjGroup> hgroups;   // [group][instance]; group ng is the dTsub projection

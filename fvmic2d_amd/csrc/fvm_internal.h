@@ -27,6 +27,7 @@ struct fvm_ctx {
   fvm_buf payload;     // bucket-ordered copies of the marker columns a projection reads
   fvm_buf payload_xy;  // bucket-ordered X, Y, Mat: valid while xy_serial == sort_serial
   uint64_t sort_serial = 0, xy_serial = 0;
+  int lanes = 8;  // lanes per lattice cell in the projection partials (8, 16 or 32)
   fvm_buf pinned;  // small pinned host buffer for scalar results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-phase timing (fvm_profile_*): event pairs around launches

@@ -269,7 +269,7 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
       accumulate<METHOD, NF>(mdx, mdy, f, sw, sf);
     }
   }
-  // fixed butterfly over the 16 lanes of the group (a + b == b + a exactly,
+  // fixed butterfly over the G lanes of the group (a + b == b + a exactly,
   // so every lane ends with the same, order-fixed total)
 #pragma unroll
   for (int off = G / 2; off > 0; off >>= 1) {

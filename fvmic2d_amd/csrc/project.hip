@@ -2,7 +2,7 @@
 // scatter parts of markers.c:1507-1594, subgridStressChanges.c:128-233):
 // single-instance launches and the public projection entry points.
 //
-// Pass 1 (k_partials): a group of 16 lanes per lattice cell walks the (at
+// Pass 1 (k_partials): a group of G lanes per lattice cell walks the (at
 //   most four) buckets that make up the cell, in bucket order and ascending
 //   marker index within a bucket (lane j takes entries j, j+16, ...), and a
 //   fixed shuffle tree sums the lanes' corner partials.  No atomics: each
@@ -17,21 +17,22 @@
 
 #include "proj_common.h"
 
-#define LPG 16  // lanes per lattice cell in the single-instance launches
+// Lanes per lattice cell (G) of the single-instance launches: 8 by default,
+// 16 or 32 through fvm_ctx_set_lanes.
 
-template <int METHOD, int NF>
+template <int METHOD, int NF, int G>
 __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const int32_t *__restrict__ start,
                                                   const double *__restrict__ xs, const double *__restrict__ ys,
                                                   const int32_t *__restrict__ ms, double *__restrict__ part, int nLx,
                                                   int nLy, int lox, int loy) {
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
-  const int j = threadIdx.x % LPG;
+  const int64_t t = (tile * blockDim.x + threadIdx.x) / G;
+  const int j = threadIdx.x % G;
   const int64_t nL = (int64_t)nLx * nLy;
   const bool live = t < nL;
   const int lx = live ? (int)(t % nLx) + lox : 0;
   const int ly = live ? (int)(t / nLx) + loy : 0;
-  partials_cell_sorted<METHOD, NF, LPG>(g, pg, start, xs, ys, ms, part, nL, t, lx, ly, j, live);
+  partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, part, nL, t, lx, ly, j, live);
 }
 
 // Several groups of one kind (method, field count) in one launch: the launch
@@ -47,13 +48,13 @@ struct MultiGroups {
   double *part[MAX_MULTI];
 };
 
-template <int METHOD, int NF>
+template <int METHOD, int NF, int G>
 __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg, const int32_t *__restrict__ start,
                                                         const double *__restrict__ xs, const double *__restrict__ ys,
                                                         const int32_t *__restrict__ ms) {
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
-  const int j = threadIdx.x % LPG;
+  const int64_t t = (tile * blockDim.x + threadIdx.x) / G;
+  const int j = threadIdx.x % G;
   const int64_t nT = (int64_t)(g.NX + 1) * (g.NY + 1);
   const bool inside = t < nT;
   const int cx = inside ? (int)(t % (g.NX + 1)) - 1 : -2;
@@ -65,24 +66,25 @@ __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg,
     const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
     const int64_t nL = (int64_t)Lx.n * Ly.n;
     const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
-    partials_cell_sorted<METHOD, NF, LPG>(g, pg, start, xs, ys, ms, mg.part[q], nL, tl, live ? cx : 0,
+    partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, mg.part[q], nL, tl, live ? cx : 0,
                                           live ? cy : 0, j, live);
   }
 }
 
+template <int G>
 __global__ void __launch_bounds__(256) k_partials_subt(DGrid g, ProjGroup pg, SubT sb,
                                                        const int32_t *__restrict__ start,
                                                        const int32_t *__restrict__ perm, const double *__restrict__ xs,
                                                        const double *__restrict__ ys, double *__restrict__ part,
                                                        int nLx, int nLy) {
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int64_t t = (tile * blockDim.x + threadIdx.x) / LPG;
-  const int j = threadIdx.x % LPG;
+  const int64_t t = (tile * blockDim.x + threadIdx.x) / G;
+  const int j = threadIdx.x % G;
   const int64_t nL = (int64_t)nLx * nLy;
   const bool live = t < nL;
   const int lx = live ? (int)(t % nLx) : 0;
   const int ly = live ? (int)(t / nLx) : 0;
-  partials_cell_subt_sorted<LPG>(g, pg, start, perm, xs, ys, part, nL, t, lx, ly, j, live, sb);
+  partials_cell_subt_sorted<G>(g, pg, start, perm, xs, ys, part, nL, t, lx, ly, j, live, sb);
 }
 
 // Bucket-ordered payload: position p of the sort gets marker perm[p]'s X, Y,
@@ -130,8 +132,14 @@ static int launch_partials(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, c
   const Lat Lx = lattice(pg.stagx, g->NX), Ly = lattice(pg.stagy, g->NY);
   const int64_t nL = (int64_t)Lx.n * Ly.n;
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
-  k_partials<METHOD, NF><<<fvm_blocks(nL * LPG, 256), 256, 0, ctx->stream>>>(
-      dgrid(g), pg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms, part, Lx.n, Ly.n, Lx.lo, Ly.lo);
+  const int G = ctx->lanes;
+#define LAUNCH_P(GG)                                                                                     \
+  k_partials<METHOD, NF, GG><<<fvm_blocks(nL * GG, 256), 256, 0, ctx->stream>>>(                          \
+      dgrid(g), pg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms, part, Lx.n, Ly.n, Lx.lo, Ly.lo)
+  if (G == 8) LAUNCH_P(8);
+  else if (G == 32) LAUNCH_P(32);
+  else LAUNCH_P(16);
+#undef LAUNCH_P
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
@@ -140,8 +148,14 @@ template <int METHOD, int NF>
 static int launch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, const MultiGroups &mg) {
   const int64_t nT = (int64_t)(g->NX + 1) * (g->NY + 1);
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
-  k_partials_multi<METHOD, NF><<<fvm_blocks(nT * LPG, 256), 256, 0, ctx->stream>>>(
-      dgrid(g), mg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms);
+  const int G = ctx->lanes;
+#define LAUNCH_M(GG)                                                                                     \
+  k_partials_multi<METHOD, NF, GG><<<fvm_blocks(nT * GG, 256), 256, 0, ctx->stream>>>(                    \
+      dgrid(g), mg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms)
+  if (G == 8) LAUNCH_M(8);
+  else if (G == 32) LAUNCH_M(32);
+  else LAUNCH_M(16);
+#undef LAUNCH_M
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
@@ -191,9 +205,15 @@ static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const Payload &mk, c
     const Lat Lx = lattice(0, g->NX), Ly = lattice(0, g->NY);
     const int64_t nL = (int64_t)Lx.n * Ly.n;
     FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
-    k_partials_subt<<<fvm_blocks(nL * LPG, 256), 256, 0, ctx->stream>>>(
-        dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk.xs, mk.ys, part, Lx.n,
-        Ly.n);
+    const int G = ctx->lanes;
+#define LAUNCH_S(GG)                                                                                     \
+  k_partials_subt<GG><<<fvm_blocks(nL * GG, 256), 256, 0, ctx->stream>>>(                                 \
+      dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk.xs, mk.ys, part, \
+      Lx.n, Ly.n)
+    if (G == 8) LAUNCH_S(8);
+    else if (G == 32) LAUNCH_S(32);
+    else LAUNCH_S(16);
+#undef LAUNCH_S
     FVM_LAUNCH_CHECK(ctx);
     return FVM_OK;
   }

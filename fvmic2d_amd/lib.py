@@ -63,6 +63,7 @@ _SIGS = {
     "fvm_ctx_stream": [P],
     "fvm_ctx_sync": [P],
     "fvm_ctx_workspace_bytes": [P],
+    "fvm_ctx_set_lanes": [P, I32],
     "fvm_device_count": [C.POINTER(C.c_int)],
     "fvm_malloc": [C.POINTER(P), C.c_size_t],
     "fvm_free": [P],

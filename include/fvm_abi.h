@@ -84,6 +84,10 @@ void *fvm_ctx_stream(fvm_ctx *ctx);
 int fvm_ctx_sync(fvm_ctx *ctx);
 /* Bytes of device workspace the context currently holds (sort buffers, partials). */
 size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
+/* Lanes per lattice cell in the projection passes: 8 (default), 16 or 32.  The
+ * lane count fixes the summation tree, so results are bitwise reproducible for
+ * a given setting and agree across settings to the projection tolerance. */
+int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes);
 
 /* ---- device memory helpers (so a host program needs no other GPU API) ---- */
 int fvm_device_count(int *n);
@@ -263,7 +267,8 @@ int fvm_pressure_guess_apply(fvm_ctx *ctx, const fvm_grid *g, const double *rho,
  * phase over all of them, optionally replayed as one captured HIP graph.
  * Each instance's step equals fvm_step_thermal with the same projection list
  * (findMarkerCells + density loop, projections, subgridTemperatureChanges,
- * advectMarkersRK), and with lanes = 16 it is bitwise identical to it.
+ * advectMarkersRK), and with the context's lane count (fvm_ctx_set_lanes,
+ * default 8) it is bitwise identical to it.
  * Every instance must use the same projection pattern (stag/method/material
  * kind per entry; the fields and property tables may differ); methods
  * ARITHMETIC and ARITHMETIC_LOCAL.  Device arrays stay owned by the caller and

@@ -1,7 +1,7 @@
 """Batched ensemble (fvm_ensemble_*): many independent thermal-marker problems of
 different sizes stepped by one launch per phase.
 
-Bars: with 16 lanes per lattice cell (the single-instance projection width) an
+Bars: with the single-instance path's lanes per lattice cell (8) an
 ensemble member is bit-identical to the same problem stepped alone with
 fvm_step_thermal; with every launch shape it matches the oracle to the
 single-instance bars (positions bitwise, FP64 fields to 1e-12 relative); graph
@@ -61,7 +61,7 @@ def test_ensemble_equals_single_instance():
     ws = _instances()
     alone, batch = _paths(ws, ctx), _paths(ws, ctx)
     ens = Ensemble(batch)
-    assert ens.get_tuning() == (256, 16)
+    assert ens.get_tuning() == (256, 8)
     assert ens.n_markers == sum(w.n_markers for w in ws)
     for s in range(2):
         for p in alone:

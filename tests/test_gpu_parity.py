@@ -294,9 +294,11 @@ def test_step_convection_small():
         assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
 
 
-def test_step_convection_tdep_full():
+@pytest.mark.parametrize("lanes", [8, 16, 32])
+def test_step_convection_tdep_full(lanes):
     w = workloads.convection_tdep()
     st, mp = _pair(w)
+    mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(lanes))
     st.step_thermal()
     mp.step_thermal()
     _cmp_markers(st, mp, exact=("X", "Y", "VX", "VY", "wxy"), fields=["X", "Y", "T", "Tdot", "rho"])
