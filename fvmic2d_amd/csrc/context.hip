@@ -67,7 +67,7 @@ extern "C" int fvm_ctx_create(fvm_ctx **out, void *stream) {
   }
   ctx->pinned.bytes = 4096;
   // reduction scratch: counters and flags at the front, block partials from +4 KiB
-  if (fvm_ws(ctx, ctx->red, 64 * 1024) != FVM_OK) {
+  if (fvm_ws(ctx, ctx->red, 64 * 1024) != FVM_OK || hipMemset(ctx->red.ptr, 0, 64 * 1024) != hipSuccess) {
     delete ctx;
     return FVM_ERR_NOMEM;
   }
@@ -96,7 +96,11 @@ extern "C" int fvm_ctx_destroy(fvm_ctx *ctx) {
 extern "C" const char *fvm_last_error(const fvm_ctx *ctx) { return ctx ? ctx->err : "no context"; }
 extern "C" void *fvm_ctx_stream(fvm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 extern "C" int fvm_ctx_sync(fvm_ctx *ctx) {
+  int32_t *h = (int32_t *)ctx->pinned.ptr + 1000;  // clear of the other pinned slots
+  FVM_HIP(ctx, hipMemcpyAsync(h, (int32_t *)ctx->red.ptr + FVM_FLAG_SCAN_FAIL, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (*h) return fvm_set_err(ctx, FVM_ERR_HIP, "bucket scan: look-back wait exceeded its bound (sort invalid)");
   return FVM_OK;
 }
 extern "C" size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx) {

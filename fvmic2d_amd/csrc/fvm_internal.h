@@ -27,6 +27,14 @@ struct fvm_ctx {
   fvm_buf payload;     // bucket-ordered copies of the marker columns a projection reads
   fvm_buf payload_xy;  // bucket-ordered X, Y, Mat: valid while xy_serial == sort_serial
   uint64_t sort_serial = 0, xy_serial = 0;
+  // columns held in `payload` (slot k = payload + k * stride doubles) for sort
+  // payload_serial; a fused subgrid pass may take its T column from there only
+  // while reuse_payload is set by a caller that knows the column is unchanged
+  const double *payload_src[24] = {nullptr};
+  int payload_nc = 0;
+  size_t payload_stride = 0;
+  uint64_t payload_serial = 0;
+  bool reuse_payload = false;
   int lanes = 8;  // lanes per lattice cell in the projection partials (8, 16 or 32)
   fvm_buf pinned;  // small pinned host buffer for scalar results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -79,6 +87,10 @@ static inline unsigned fvm_blocks(int64_t n, int bs) { return (unsigned)((n + bs
 // b % 8 and becomes tile (b % 8) * q + min(b % 8, r) + b / 8, q = nb / 8,
 // r = nb % 8 -- a bijection on [0, nb).
 #define FVM_NUM_XCD 8
+
+// int32 slot in ctx->red: sticky flag set when the one-pass bucket scan gave up
+// waiting on a predecessor tile (fvm_ctx_sync reports it)
+#define FVM_FLAG_SCAN_FAIL 2
 __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
   const int64_t q = nb / FVM_NUM_XCD, r = nb % FVM_NUM_XCD;
   const int64_t x = b % FVM_NUM_XCD, k = b / FVM_NUM_XCD;

@@ -268,13 +268,18 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
         if (sl < 0) return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "more than %d distinct marker columns", MAX_PAYLOAD_COLS);
         uses.push_back({i * FVM_MAXF + k, sl});
       }
-  SubT ssub;
-  if (sub) {
-    ssub = *sub;
-    col_slot(sub->MT);
-  }
   const int32_t n = mk->nMark;
   const size_t np = ((size_t)(n > 0 ? n : 1) + 31) / 32 * 32;
+  SubT ssub;
+  const double *cached_mt = nullptr;
+  if (sub) {
+    ssub = *sub;
+    if (ctx->reuse_payload && ctx->payload.ptr && ctx->payload_serial == ctx->sort_serial &&
+        ctx->payload_stride == np)
+      for (int k = 0; k < ctx->payload_nc; k++)
+        if (ctx->payload_src[k] == sub->MT) cached_mt = (const double *)ctx->payload.ptr + np * k;
+    if (!cached_mt) col_slot(sub->MT);
+  }
   // X, Y, Mat in bucket order are kept for the life of the sort
   const bool have_xy = ctx->xy_serial == ctx->sort_serial && ctx->payload_xy.bytes >= np * 20;
   if (!have_xy) FVM_TRY(fvm_ws(ctx, ctx->payload_xy, np * (2 * sizeof(double) + sizeof(int32_t))));
@@ -283,7 +288,13 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
   if (pc.nc > 0) FVM_TRY(fvm_ws(ctx, ctx->payload, np * sizeof(double) * pc.nc));
   for (int k = 0; k < pc.nc; k++) pc.dst[k] = (double *)ctx->payload.ptr + np * k;
   for (auto &u : uses) sg[u.first / FVM_MAXF].src[u.first % FVM_MAXF].col = pc.dst[u.second];
-  if (sub) ssub.MT = pc.dst[0];
+  if (sub) ssub.MT = cached_mt ? cached_mt : pc.dst[0];
+  if (pc.nc > 0) {  // record what the payload now holds
+    ctx->payload_nc = pc.nc;
+    for (int k = 0; k < pc.nc; k++) ctx->payload_src[k] = pc.src[k];
+    ctx->payload_stride = np;
+    ctx->payload_serial = ctx->sort_serial;
+  }
   if (n > 0 && (!have_xy || pc.nc > 0)) {
     FVM_PHASE(ctx, FVM_PHASE_SORT);
     const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);

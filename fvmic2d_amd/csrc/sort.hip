@@ -190,6 +190,105 @@ __global__ void __launch_bounds__(1024) k_scan_sums(int32_t *__restrict__ sums, 
   if (threadIdx.x == 0) *total = carry;
 }
 
+// ---- one-pass scan (decoupled look-back) --------------------------------------------
+// Each block takes the next tile number from a ticket, so every tile it looks
+// back at was started earlier and will publish.  A tile publishes its aggregate
+// (status 1) and then its inclusive prefix (status 2) as one 64-bit word
+// (status << 32 | value); a successor's first wavefront reads its predecessors'
+// words 64 at a time and sums them back to the nearest inclusive prefix.  The wait is bounded: after SCAN_SPIN_MAX polls the tile
+// sets the sticky flag *fail and stops waiting (its output is then invalid);
+// fvm_ctx_sync reports the flag.  state[0] = ticket, state[2 + t] = word of tile t.
+#define SCAN_SPIN_MAX (1 << 20)
+
+__global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restrict__ in, int32_t *__restrict__ out,
+                                                          unsigned long long *__restrict__ state, int64_t n,
+                                                          int32_t *__restrict__ fail) {
+  __shared__ int32_t s[SCAN_BS];
+  __shared__ int32_t s_tile, s_prefix;
+  if (threadIdx.x == 0) s_tile = (int32_t)atomicAdd(&state[0], 1ull);
+  __syncthreads();
+  const int32_t tile = s_tile;
+  const int64_t base = (int64_t)tile * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+  int32_t v[SCAN_IPT];
+  int32_t t = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; i++) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    t += v[i];
+  }
+  s[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < SCAN_BS; off <<= 1) {
+    int32_t a = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
+    __syncthreads();
+    s[threadIdx.x] += a;
+    __syncthreads();
+  }
+  const int32_t agg = s[SCAN_BS - 1];
+  unsigned long long *word = state + 2;
+  if (threadIdx.x == 0)
+    __hip_atomic_store(&word[tile], ((tile == 0 ? 2ull : 1ull) << 32) | (uint32_t)agg, __ATOMIC_RELEASE,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  if (tile > 0 && threadIdx.x < 64) {
+    // wave 0 looks back over a window of 64 predecessors at a time: lane i
+    // reads tile (j - i); the window ends at its nearest inclusive prefix
+    const int lane = threadIdx.x;
+    int32_t prefix = 0;
+    int64_t j = tile - 1;
+    bool failed = false;
+    for (;;) {
+      const int64_t idx = j - lane;
+      unsigned long long w = idx >= 0 ? 0ull : (2ull << 32);  // before tile 0: inclusive 0
+      int spins = 0;
+      for (;;) {
+        if (idx >= 0 && (w >> 32) == 0) w = __hip_atomic_load(&word[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (__ballot((w >> 32) == 0) == 0) break;
+        if (++spins > SCAN_SPIN_MAX) {
+          failed = true;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      if (failed) break;
+      const unsigned long long incl = __ballot((w >> 32) == 2);
+      const int first = incl ? __ffsll((long long)incl) - 1 : 63;  // nearest inclusive lane
+      int32_t v = lane <= first ? (int32_t)(uint32_t)w : 0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      prefix += v;
+      if (incl) break;
+      j -= 64;
+    }
+    if (lane == 0) {
+      if (failed) atomicOr(fail, 1);
+      __hip_atomic_store(&word[tile], (2ull << 32) | (uint32_t)(prefix + agg), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      s_prefix = prefix;
+    }
+  } else if (threadIdx.x == 0) {
+    s_prefix = 0;
+  }
+  __syncthreads();
+  int32_t run = s_prefix + s[threadIdx.x] - t;  // exclusive prefix of this thread
+#pragma unroll
+  for (int i = 0; i < SCAN_IPT; i++) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+  // the thread holding element n-1 ends with the inclusive prefix of all n: the total
+  if (base <= n - 1 && n - 1 < base + SCAN_IPT) out[n] = run;
+}
+
+// counts (n entries, optional), the two location counters and the scan state
+__global__ void __launch_bounds__(256) k_zero_counts(int32_t *__restrict__ count, int64_t n,
+                                                     int32_t *__restrict__ counters,
+                                                     unsigned long long *__restrict__ state, int64_t nstate) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (count && i < n) count[i] = 0;
+  if (state && i < nstate) state[i] = 0ull;
+  if (i < 2) counters[i] = 0;
+}
+
 __global__ void __launch_bounds__(SCAN_BS) k_scan_add(int32_t *__restrict__ out, const int32_t *__restrict__ tile_sums,
                                                       int64_t n) {
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
@@ -205,7 +304,8 @@ __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__res
   if (m >= n) return;
   const int k = key[m];
   if (k < 0) return;
-  perm[start[k] + slot[m]] = (int32_t)m;
+  const int32_t p = start[k] + slot[m];
+  if ((uint32_t)p < (uint32_t)n) perm[p] = (int32_t)m;  // always true for a valid scan
 }
 
 // Put every bucket in ascending marker order: one wavefront per basic cell
@@ -255,14 +355,23 @@ static int sort_buffers(fvm_ctx *ctx, const fvm_grid *g, int32_t n) {
   FVM_TRY(fvm_ws(ctx, ctx->perm, (size_t)(n > 0 ? n : 1) * 4));
   FVM_TRY(fvm_ws(ctx, ctx->count, (size_t)(nk + 1) * 4));
   FVM_TRY(fvm_ws(ctx, ctx->start, (size_t)(nk + 1) * 4));
-  FVM_TRY(fvm_ws(ctx, ctx->blocksums, (size_t)(ntiles + 1) * 4));
+  FVM_TRY(fvm_ws(ctx, ctx->blocksums, (size_t)(ntiles + 2) * 8));  // one-pass scan state
   return FVM_OK;
 }
 
+// Zero the bucket counts, the one-pass scan state and the location counters
+// (one launch); without a grid only the counters.
 static int zero_counts(fvm_ctx *ctx, const fvm_grid *g) {
-  const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+  int32_t *counters = (int32_t *)ctx->red.ptr;
   FVM_PHASE(ctx, FVM_PHASE_SORT);
-  k_zero_i32<<<fvm_blocks(nk + 1, 256), 256, 0, ctx->stream>>>((int32_t *)ctx->count.ptr, nk + 1);
+  if (!g) {
+    k_zero_counts<<<1, 64, 0, ctx->stream>>>(nullptr, 0, counters, nullptr, 0);
+  } else {
+    const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+    const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
+    k_zero_counts<<<fvm_blocks(nk + 1, 256), 256, 0, ctx->stream>>>(
+        (int32_t *)ctx->count.ptr, nk + 1, counters, (unsigned long long *)ctx->blocksums.ptr, ntiles + 2);
+  }
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
@@ -276,12 +385,8 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
   const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
   int32_t *key = (int32_t *)ctx->key.ptr, *slot = (int32_t *)ctx->slot.ptr, *perm = (int32_t *)ctx->perm.ptr;
   int32_t *count = (int32_t *)ctx->count.ptr, *start = (int32_t *)ctx->start.ptr;
-  int32_t *sums = (int32_t *)ctx->blocksums.ptr;
-  k_scan_tiles<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(count, start, sums, nk);
-  FVM_LAUNCH_CHECK(ctx);
-  k_scan_sums<<<1, 1024, 0, ctx->stream>>>(sums, ntiles, start + nk);
-  FVM_LAUNCH_CHECK(ctx);
-  k_scan_add<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(start, sums, nk);
+  k_scan_onepass<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(
+      count, start, (unsigned long long *)ctx->blocksums.ptr, nk, (int32_t *)ctx->red.ptr + FVM_FLAG_SCAN_FAIL);
   FVM_LAUNCH_CHECK(ctx);
   if (n > 0) {
     k_scatter<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, slot, start, perm);
@@ -300,13 +405,10 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
 int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank, const Materials *dens,
                      bool build_order) {
   int32_t *counters = (int32_t *)ctx->red.ptr;
-  FVM_HIP(ctx, hipMemsetAsync(counters, 0, 2 * sizeof(int32_t), ctx->stream));
   const int32_t n = mk->nMark;
   fvm_markers_moved(ctx);
-  if (build_order) {
-    FVM_TRY(sort_buffers(ctx, g, n));
-    FVM_TRY(zero_counts(ctx, g));
-  }
+  if (build_order) FVM_TRY(sort_buffers(ctx, g, n));
+  FVM_TRY(zero_counts(ctx, build_order ? g : nullptr));
   DensityTab dt;
   for (int i = 0; i < FVM_MAXMAT; i++) {
     dt.rho[i] = dens ? dens->materialRho[i] : 0.0;

@@ -15,8 +15,13 @@ extern "C" int fvm_step_thermal(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk
   if (!nf->thisT || !nf->lastT || !nf->rho || !nf->Cp || !nf->kThermal || !nf->vx || !nf->vy || !nf->wxy)
     return fvm_set_err(ctx, FVM_ERR_ARG, "step: missing nodal field");
   double *dTsub = nullptr, *deltaT = nullptr;                   // :415 subgridTemperatureChanges
-  FVM_TRY(fvm_subgrid_t_pass1(ctx, nf->thisT, g, nf, mk, mat, dt, opt->subgridTemperatureDiffusivity, &dTsub,
-                              &deltaT));
+  // marker T is unchanged since the projection above, so its bucket-ordered
+  // copy can be reused by the fused dTsub pass
+  ctx->reuse_payload = true;
+  const int rc = fvm_subgrid_t_pass1(ctx, nf->thisT, g, nf, mk, mat, dt, opt->subgridTemperatureDiffusivity, &dTsub,
+                                     &deltaT);
+  ctx->reuse_payload = false;
+  FVM_TRY(rc);
   // its pass 2 runs inside the advection kernel (:435 advectMarkersRK)
   FVM_TRY(fvm_advect_with_subgrid_t2(ctx, mk, nf, g, bv, dt, deltaT, dTsub, dt));
   return FVM_OK;

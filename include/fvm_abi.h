@@ -81,6 +81,9 @@ int fvm_ctx_create(fvm_ctx **ctx, void *stream);
 int fvm_ctx_destroy(fvm_ctx *ctx);
 const char *fvm_last_error(const fvm_ctx *ctx);
 void *fvm_ctx_stream(fvm_ctx *ctx);
+/* Waits for the context's stream.  Returns FVM_ERR_HIP if a bucket sort since
+ * the context was created gave up its bounded inter-block wait (never expected;
+ * the sort's ordering would then be invalid). */
 int fvm_ctx_sync(fvm_ctx *ctx);
 /* Bytes of device workspace the context currently holds (sort buffers, partials). */
 size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
