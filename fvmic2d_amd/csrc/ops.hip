@@ -11,7 +11,12 @@
 int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
 
 // ---- A6: advectMarkersRK (markers.c:1865-2250), body in marker_common.h ----------
-__global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *__restrict__ MX, double *__restrict__ MY,
+// With LDS the grid coordinates x, y, xc, yc (2 NX + 2 NY + 2 doubles) are staged
+// in LDS once per block, and each block walks ADV_TILES tiles of 256 markers so
+// that the staging is amortised; the per-marker arithmetic is unchanged.
+#define ADV_TILES 2
+template <bool LDS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_advect_rk4(DGrid g, int32_t n, double *__restrict__ MX, double *__restrict__ MY,
                                                     double *__restrict__ MVX, double *__restrict__ MVY,
                                                     double *__restrict__ Mwxy, double *__restrict__ sxx,
                                                     double *__restrict__ syy, double *__restrict__ sxy,
@@ -23,9 +28,51 @@ __global__ void __launch_bounds__(256) k_advect_rk4(DGrid g, int32_t n, double *
                                                     int jaumann, double *__restrict__ MT, double *__restrict__ MTdot,
                                                     const double *__restrict__ dTn, const double *__restrict__ dTsub,
                                                     double dt_thermal) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  advect_marker(m, g, MX, MY, MVX, MVY, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, cellXa, cellYa, vx, vy, wxy, bv, dt, jaumann, MT, MTdot, dTn, dTsub, dt_thermal);
+  if (LDS) {
+    extern __shared__ double sgrid[];
+    for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
+    for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
+    for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
+    for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
+    __syncthreads();
+    DGrid gl = g;
+    gl.x = sgrid;
+    gl.y = sgrid + g.NX;
+    gl.xc = sgrid + g.NX + g.NY;
+    gl.yc = sgrid + 2 * g.NX + g.NY + 1;
+    const int64_t m0 = (int64_t)blockIdx.x * (256 * ADV_TILES) + threadIdx.x;
+#pragma unroll 1
+    for (int t = 0; t < ADV_TILES; t++) {
+      const int64_t m = m0 + 256 * t;
+      if (m < n)
+        advect_marker(m, gl, MX, MY, MVX, MVY, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, cellXa, cellYa, vx, vy, wxy,
+                      bv, dt, jaumann, MT, MTdot, dTn, dTsub, dt_thermal);
+    }
+  } else {
+    const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    advect_marker(m, g, MX, MY, MVX, MVY, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, cellXa, cellYa, vx, vy, wxy, bv,
+                  dt, jaumann, MT, MTdot, dTn, dTsub, dt_thermal);
+  }
+}
+
+static int launch_advect(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const NodalArrays *nf,
+                         const BoundaryValues *bv, double dt, int jaumann, const double *deltaT, const double *dTsub,
+                         double dt_thermal) {
+  const int32_t n = mk->nMark;
+  const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
+  double *T = deltaT ? mk->T : nullptr, *Tdot = deltaT ? mk->Tdot : nullptr;
+  if (lds <= 32 * 1024) {
+    k_advect_rk4<true><<<(unsigned)(((int64_t)n + 256 * ADV_TILES - 1) / (256 * ADV_TILES)), 256, lds, ctx->stream>>>(
+        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy,
+        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal);
+  } else {
+    k_advect_rk4<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(
+        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy,
+        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal);
+  }
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
 }
 
 extern "C" int fvm_advect_markers_rk(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays *nf, const fvm_grid *g,
@@ -35,11 +82,7 @@ extern "C" int fvm_advect_markers_rk(fvm_ctx *ctx, fvm_markers *mk, const NodalA
   if (mk->nMark == 0) return FVM_OK;
   {
     FVM_PHASE(ctx, FVM_PHASE_ADVECT);
-    k_advect_rk4<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(
-        dgrid(g), mk->nMark, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy,
-        mk->exy, mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, nullptr, nullptr,
-      nullptr, nullptr, 0.0);
-    FVM_LAUNCH_CHECK(ctx);
+    FVM_TRY(launch_advect(ctx, g, mk, nf, bv, dt, jaumann, nullptr, nullptr, 0.0));
   }
   fvm_markers_moved(ctx);
   return FVM_OK;
@@ -121,11 +164,7 @@ int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays 
   if (mk->nMark == 0) return FVM_OK;
   {
     FVM_PHASE(ctx, FVM_PHASE_ADVECT);
-    k_advect_rk4<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(
-        dgrid(g), mk->nMark, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy,
-        mk->exy, mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, 0, mk->T, mk->Tdot, deltaT,
-        dTsub, dt_thermal);
-    FVM_LAUNCH_CHECK(ctx);
+    FVM_TRY(launch_advect(ctx, g, mk, nf, bv, dt, 0, deltaT, dTsub, dt_thermal));
   }
   fvm_markers_moved(ctx);
   return FVM_OK;
