@@ -52,6 +52,8 @@ template <int METHOD, int NF, int G>
 __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg, const int32_t *__restrict__ start,
                                                         const double *__restrict__ xs, const double *__restrict__ ys,
                                                         const int32_t *__restrict__ ms) {
+  // blockIdx.y selects the group, so the groups' cells are in flight together
+  const int q = blockIdx.y;
   const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   const int64_t t = (tile * blockDim.x + threadIdx.x) / G;
   const int j = threadIdx.x % G;
@@ -59,16 +61,13 @@ __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg,
   const bool inside = t < nT;
   const int cx = inside ? (int)(t % (g.NX + 1)) - 1 : -2;
   const int cy = inside ? (int)(t / (g.NX + 1)) - 1 : -2;
-#pragma unroll 1
-  for (int q = 0; q < mg.ng; q++) {
-    const ProjGroup &pg = mg.pg[q];
-    const Lat Lx = lattice(pg.stagx, g.NX), Ly = lattice(pg.stagy, g.NY);
-    const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
-    const int64_t nL = (int64_t)Lx.n * Ly.n;
-    const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
-    partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, mg.part[q], nL, tl, live ? cx : 0,
-                                          live ? cy : 0, j, live);
-  }
+  const ProjGroup &pg = mg.pg[q];
+  const Lat Lx = lattice(pg.stagx, g.NX), Ly = lattice(pg.stagy, g.NY);
+  const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
+  const int64_t nL = (int64_t)Lx.n * Ly.n;
+  const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
+  partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, mg.part[q], nL, tl, live ? cx : 0, live ? cy : 0, j,
+                                      live);
 }
 
 template <int G>
@@ -150,7 +149,8 @@ static int launch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, cons
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
   const int G = ctx->lanes;
 #define LAUNCH_M(GG)                                                                                     \
-  k_partials_multi<METHOD, NF, GG><<<fvm_blocks(nT * GG, 256), 256, 0, ctx->stream>>>(                    \
+  k_partials_multi<METHOD, NF, GG><<<dim3((unsigned)fvm_blocks(nT * GG, 256), (unsigned)mg.ng), 256, 0,         \
+                                     ctx->stream>>>(                                                       \
       dgrid(g), mg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms)
   if (G == 8) LAUNCH_M(8);
   else if (G == 32) LAUNCH_M(32);
