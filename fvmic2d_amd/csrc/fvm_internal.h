@@ -49,6 +49,13 @@ struct fvm_ctx {
   int32_t sort_n = -1;
   uint64_t sort_epoch = 0, epoch = 1;
   int32_t sort_nactive = 0;
+  // keys, slots and bucket counts left by a locate pass for (grid, markers,
+  // epoch); the first projection finishes that order (and gathers its payload)
+  bool pend = false;
+  const void *pend_x = nullptr;
+  const void *pend_X = nullptr;
+  int32_t pend_n = -1;
+  uint64_t pend_epoch = 0;
 };
 
 int fvm_set_err(fvm_ctx *ctx, int code, const char *fmt, ...);
@@ -203,7 +210,28 @@ struct ProjGroup {
 int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank, const Materials *dens,
                      bool build_order);
 // Build the (cell, quadrant) bucket order of the active markers (sort.hip).
-int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk);
+// Bucket-ordered payload: position p of the sort gets marker perm[p]'s X, Y,
+// Mat (with_xy) and the listed columns, so the partials passes read contiguous runs.
+#define MAX_PAYLOAD_COLS 24
+struct PayloadCols {
+  int nc;
+  const double *src[MAX_PAYLOAD_COLS];
+  double *dst[MAX_PAYLOAD_COLS];
+};
+struct SortGather {
+  int with_xy;
+  const double *X, *Y;
+  const int32_t *Mat;
+  double *xs, *ys;
+  int32_t *ms;
+  PayloadCols pc;
+};
+// Build (or reuse) the bucket order.  When the order has to be rebuilt and `sg`
+// is given, the payload it names is gathered in the same pass and *gathered is
+// set; a reused order leaves *gathered false.
+int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const SortGather *sg = nullptr,
+                     bool *gathered = nullptr);
+bool fvm_sort_valid(const fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk);
 // Run projection groups over the current bucket order (project.hip).
 int fvm_project_group(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const ProjGroup &pg);
 struct SubT;

@@ -86,15 +86,6 @@ __global__ void __launch_bounds__(256) k_partials_subt(DGrid g, ProjGroup pg, Su
   partials_cell_subt_sorted<G>(g, pg, start, perm, xs, ys, part, nL, t, lx, ly, j, live, sb);
 }
 
-// Bucket-ordered payload: position p of the sort gets marker perm[p]'s X, Y,
-// Mat and the listed columns, so the partials passes read contiguous runs.
-#define MAX_PAYLOAD_COLS 24
-struct PayloadCols {
-  int nc;
-  const double *src[MAX_PAYLOAD_COLS];
-  double *dst[MAX_PAYLOAD_COLS];
-};
-
 __global__ void __launch_bounds__(256) k_gather_payload(int32_t n, const int32_t *__restrict__ total,
                                                         const int32_t *__restrict__ perm, const double *__restrict__ X,
                                                         const double *__restrict__ Y, const int32_t *__restrict__ Mat,
@@ -246,9 +237,10 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
     off[i] = total;
     total += (size_t)Lx.n * Ly.n * 4 * (pg.nf + 1);
   }
-  FVM_TRY(fvm_sort_markers(ctx, g, mk));
   FVM_TRY(fvm_ws(ctx, ctx->partials, total * sizeof(double)));
   double *part = (double *)ctx->partials.ptr;
+  // a rebuilt bucket order invalidates every cached payload column
+  const bool resort = !fvm_sort_valid(ctx, g, mk);
   // payload: X, Y, Mat and every distinct source column, in bucket order
   PayloadCols pc;
   memset(&pc, 0, sizeof(pc));
@@ -274,14 +266,14 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
   const double *cached_mt = nullptr;
   if (sub) {
     ssub = *sub;
-    if (ctx->reuse_payload && ctx->payload.ptr && ctx->payload_serial == ctx->sort_serial &&
+    if (!resort && ctx->reuse_payload && ctx->payload.ptr && ctx->payload_serial == ctx->sort_serial &&
         ctx->payload_stride == np)
       for (int k = 0; k < ctx->payload_nc; k++)
         if (ctx->payload_src[k] == sub->MT) cached_mt = (const double *)ctx->payload.ptr + np * k;
     if (!cached_mt) col_slot(sub->MT);
   }
   // X, Y, Mat in bucket order are kept for the life of the sort
-  const bool have_xy = ctx->xy_serial == ctx->sort_serial && ctx->payload_xy.bytes >= np * 20;
+  const bool have_xy = !resort && ctx->xy_serial == ctx->sort_serial && ctx->payload_xy.bytes >= np * 20;
   if (!have_xy) FVM_TRY(fvm_ws(ctx, ctx->payload_xy, np * (2 * sizeof(double) + sizeof(int32_t))));
   double *xs = (double *)ctx->payload_xy.ptr, *ys = xs + np;
   int32_t *ms = (int32_t *)(ys + np);
@@ -289,13 +281,26 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
   for (int k = 0; k < pc.nc; k++) pc.dst[k] = (double *)ctx->payload.ptr + np * k;
   for (auto &u : uses) sg[u.first / FVM_MAXF].src[u.first % FVM_MAXF].col = pc.dst[u.second];
   if (sub) ssub.MT = cached_mt ? cached_mt : pc.dst[0];
+  // a rebuilt order gathers the payload in its final pass
+  SortGather sgat;
+  memset(&sgat, 0, sizeof(sgat));
+  sgat.with_xy = have_xy ? 0 : 1;
+  sgat.X = mk->X;
+  sgat.Y = mk->Y;
+  sgat.Mat = mk->Mat;
+  sgat.xs = xs;
+  sgat.ys = ys;
+  sgat.ms = ms;
+  sgat.pc = pc;
+  bool gathered = false;
+  FVM_TRY(fvm_sort_markers(ctx, g, mk, &sgat, &gathered));
   if (pc.nc > 0) {  // record what the payload now holds
     ctx->payload_nc = pc.nc;
     for (int k = 0; k < pc.nc; k++) ctx->payload_src[k] = pc.src[k];
     ctx->payload_stride = np;
     ctx->payload_serial = ctx->sort_serial;
   }
-  if (n > 0 && (!have_xy || pc.nc > 0)) {
+  if (!gathered && n > 0 && (!have_xy || pc.nc > 0)) {
     FVM_PHASE(ctx, FVM_PHASE_SORT);
     const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
     k_gather_payload<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, (const int32_t *)ctx->start.ptr + nk,

@@ -310,9 +310,11 @@ __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__res
 
 // Put every bucket in ascending marker order: one wavefront per basic cell
 // ranks the (usually <= 64) entries of the cell's four buckets; a cell with
-// more entries falls back to an insertion sort per bucket.
+// more entries falls back to an insertion sort per bucket.  With GATHER the
+// payload columns of each entry are copied to its final position as well.
+template <bool GATHER>
 __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32_t *__restrict__ start,
-                                                      int32_t *__restrict__ perm) {
+                                                      int32_t *__restrict__ perm, SortGather sg) {
   const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (c >= ncell) return;  // whole wavefront exits together
@@ -331,7 +333,18 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32
       rank += (bj == bk && vj < v) ? 1 : 0;
     }
     const int bs = bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3));
-    if (have) perm[bs + rank] = v;
+    if (have) {
+      const int p = bs + rank;
+      perm[p] = v;
+      if (GATHER) {
+        if (sg.with_xy) {
+          sg.xs[p] = sg.X[v];
+          sg.ys[p] = sg.Y[v];
+          sg.ms[p] = sg.Mat[v];
+        }
+        for (int k = 0; k < sg.pc.nc; k++) sg.pc.dst[k][p] = sg.pc.src[k][v];
+      }
+    }
   } else if (lane < 4) {
     const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));
     const int s1 = lane == 0 ? b1 : (lane == 1 ? b2 : (lane == 2 ? b3 : b4));
@@ -343,6 +356,17 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32
         j--;
       }
       perm[j + 1] = v;
+    }
+    if (GATHER) {  // this lane's own bucket, read back by the lane that wrote it
+      for (int p = s0; p < s1; p++) {
+        const int v = perm[p];
+        if (sg.with_xy) {
+          sg.xs[p] = sg.X[v];
+          sg.ys[p] = sg.Y[v];
+          sg.ms[p] = sg.Mat[v];
+        }
+        for (int k = 0; k < sg.pc.nc; k++) sg.pc.dst[k][p] = sg.pc.src[k][v];
+      }
     }
   }
 }
@@ -377,7 +401,7 @@ static int zero_counts(fvm_ctx *ctx, const fvm_grid *g) {
 }
 
 // scan + scatter + in-bucket order, after the counts and provisional slots exist
-static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
+static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const SortGather *sg) {
   FVM_PHASE(ctx, FVM_PHASE_SORT);
   const int32_t n = mk->nMark;
   const int64_t ncell = (int64_t)(g->NX - 1) * (g->NY - 1);
@@ -392,7 +416,13 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
     k_scatter<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, slot, start, perm);
     FVM_LAUNCH_CHECK(ctx);
   }
-  k_bucket_order<<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm);
+  if (sg && n > 0) {
+    k_bucket_order<true><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm, *sg);
+  } else {
+    SortGather none;
+    memset(&none, 0, sizeof(none));
+    k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm, none);
+  }
   FVM_LAUNCH_CHECK(ctx);
   ctx->sort_epoch = ctx->epoch;
   ctx->sort_serial++;
@@ -434,7 +464,13 @@ int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t r
     }
     FVM_LAUNCH_CHECK(ctx);
   }
-  if (build_order) FVM_TRY(finish_sort(ctx, g, mk));
+  if (build_order) {  // finished by the first projection (fvm_sort_markers)
+    ctx->pend = true;
+    ctx->pend_x = g->x;
+    ctx->pend_X = mk->X;
+    ctx->pend_n = n;
+    ctx->pend_epoch = ctx->epoch;
+  }
   return FVM_OK;
 }
 
@@ -452,10 +488,23 @@ extern "C" int fvm_find_marker_cells(fvm_ctx *ctx, const fvm_grid *g, fvm_marker
   return FVM_OK;
 }
 
-int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
-  if (ctx->sort_epoch == ctx->epoch && ctx->sort_x == (const void *)g->x && ctx->sort_X == (const void *)mk->X &&
-      ctx->sort_n == mk->nMark)
+bool fvm_sort_valid(const fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
+  return ctx->sort_epoch == ctx->epoch && ctx->sort_x == (const void *)g->x && ctx->sort_X == (const void *)mk->X &&
+         ctx->sort_n == mk->nMark;
+}
+
+int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const SortGather *sg,
+                     bool *gathered) {
+  if (gathered) *gathered = false;
+  if (fvm_sort_valid(ctx, g, mk)) return FVM_OK;
+  if (ctx->pend && ctx->pend_epoch == ctx->epoch && ctx->pend_x == (const void *)g->x &&
+      ctx->pend_X == (const void *)mk->X && ctx->pend_n == mk->nMark) {
+    ctx->pend = false;
+    FVM_TRY(finish_sort(ctx, g, mk, sg));
+    if (gathered) *gathered = sg != nullptr && mk->nMark > 0;
     return FVM_OK;
+  }
+  ctx->pend = false;
   const int32_t n = mk->nMark;
   FVM_TRY(sort_buffers(ctx, g, n));
   FVM_TRY(zero_counts(ctx, g));
@@ -466,7 +515,9 @@ int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk) {
                                                         (int32_t *)ctx->count.ptr);
     FVM_LAUNCH_CHECK(ctx);
   }
-  return finish_sort(ctx, g, mk);
+  FVM_TRY(finish_sort(ctx, g, mk, sg));
+  if (gathered) *gathered = sg != nullptr && n > 0;
+  return FVM_OK;
 }
 
 // ---- helpers for the batched ensemble (ensemble.hip) ---------------------------------
@@ -493,7 +544,9 @@ int64_t fvm_scan_tiles(int64_t nk) { return (nk + SCAN_TILE - 1) / SCAN_TILE; }
 
 int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm) {
   FVM_PHASE(ctx, FVM_PHASE_SORT);
-  k_bucket_order<<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm);
+  SortGather none;
+  memset(&none, 0, sizeof(none));
+  k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm, none);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }

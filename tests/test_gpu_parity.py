@@ -306,6 +306,26 @@ def test_step_convection_tdep_full(lanes):
         assert_close(mp.get_nodal(out), st.nodal[out], out, scale=field_scale(st.nodal[out]))
 
 
+def test_step_crowded_cells():
+    """Cells with more than 64 markers take the per-bucket ordering path (and
+    gather their payload there); results still match the oracle."""
+    w = workloads.convection_tdep(N=41, NM=4)
+    r = workloads.Rng(5, 3)
+    m, x, y = w.markers, w.x, w.y
+    a = np.arange(0, 400 * 7, 7)  # 400 markers spread over basic cell (10, 12)
+    m["X"][a] = x[10] + (x[11] - x[10]) * r.uniform(len(a))
+    m["Y"][a] = y[12] + (y[13] - y[12]) * r.uniform(len(a))
+    b = np.arange(2801, 2951)  # 150 markers inside one quadrant of cell (20, 5)
+    m["X"][b] = x[20] + 0.49 * (x[21] - x[20]) * r.uniform(len(b))
+    m["Y"][b] = y[6] - 0.49 * (y[6] - y[5]) * r.uniform(len(b))
+    st, mp = _pair(w)
+    st.step_thermal()
+    mp.step_thermal()
+    _cmp_markers(st, mp, exact=("X", "Y", "VX", "VY", "wxy"), fields=["X", "Y", "T", "Tdot", "rho"])
+    for _, _, _, _, out in w.projections:
+        assert_close(mp.get_nodal(out), st.nodal[out], out, scale=field_scale(st.nodal[out]))
+
+
 def test_step_vep(vep):
     st, mp = _pair(vep)
     y_r, r_r = st.step_vep()
