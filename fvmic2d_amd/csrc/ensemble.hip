@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ ins
     const double xm = I.X[m], ym = I.Y[m];
     int cx = -1, cy = -1;
     if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
-      cx = bisect_grid(gx, g.NX, xm);
-      cy = bisect_grid(gy, g.NY, ym);
+      cx = guess_grid(gx, g.NX, xm, (double)(g.NX - 1) / (gx[g.NX - 1] - gx[0]));
+      cy = guess_grid(gy, g.NY, ym, (double)(g.NY - 1) / (gy[g.NY - 1] - gy[0]));
       if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
         bad = 1;
       } else {

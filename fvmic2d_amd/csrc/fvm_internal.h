@@ -218,6 +218,18 @@ struct PayloadCols {
   const double *src[MAX_PAYLOAD_COLS];
   double *dst[MAX_PAYLOAD_COLS];
 };
+// Copy payload entry p from marker v.
+__device__ __forceinline__ void gather_entry(const PayloadCols &pc, int with_xy, const double *X, const double *Y,
+                                             const int32_t *Mat, double *xs, double *ys, int32_t *ms, int64_t p,
+                                             int v) {
+  if (with_xy) {
+    xs[p] = X[v];
+    ys[p] = Y[v];
+    ms[p] = Mat[v];
+  }
+  for (int k = 0; k < pc.nc; k++) pc.dst[k][p] = pc.src[k][v];
+}
+
 struct SortGather {
   int with_xy;
   const double *X, *Y;
@@ -281,6 +293,24 @@ __device__ __forceinline__ int bisect_grid(const double *__restrict__ x, int N, 
     else hi = mid;
   }
   return lo;
+}
+
+// The same index from a guess: for a non-decreasing x, bisect_grid returns the
+// largest i in [0, N-2] with x[i] <= X (0 when there is none), whatever path
+// the bisection takes.  Start at the cell a uniform spacing would give
+// (inv = (N-1) / (x[N-1] - x[0])), step at most four cells towards the answer,
+// and bisect when that did not settle it (strongly graded grids).
+__device__ __forceinline__ int guess_grid(const double *__restrict__ x, int N, double X, double inv) {
+  double t = (X - x[0]) * inv;
+  t = t > 0.0 ? (t < (double)(N - 2) ? t : (double)(N - 2)) : 0.0;  // NaN -> 0
+  int i = (int)t;
+  bool done = false;
+  for (int s = 0; s < 4 && !done; s++) {
+    if (i > 0 && X < x[i]) i--;
+    else if (i < N - 2 && X >= x[i + 1]) i++;
+    else done = true;  // x[i] <= X < x[i+1], or clamped at an end
+  }
+  return done ? i : bisect_grid(x, N, X);
 }
 
 // deterministic double sum of n values on the device -> *out (host), ops.hip

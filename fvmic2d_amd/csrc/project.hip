@@ -93,13 +93,7 @@ __global__ void __launch_bounds__(256) k_gather_payload(int32_t n, const int32_t
                                                         int32_t *__restrict__ ms, PayloadCols pc, int with_xy) {
   const int64_t p = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n || p >= *total) return;
-  const int m = perm[p];
-  if (with_xy) {
-    xs[p] = X[m];
-    ys[p] = Y[m];
-    ms[p] = Mat[m];
-  }
-  for (int c = 0; c < pc.nc; c++) pc.dst[c][p] = pc.src[c][m];
+  gather_entry(pc, with_xy, X, Y, Mat, xs, ys, ms, p, perm[p]);
 }
 
 // one thread per node; launch row blockIdx.y = (group, field) pair

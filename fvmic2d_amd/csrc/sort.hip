@@ -11,17 +11,8 @@
 #include "fvm_internal.h"
 
 // ---- findMarkerCell (markers.c:41-83) ---------------------------------------
-__device__ __forceinline__ int bisect(const double *__restrict__ x, int N, double X) {
-  int lo = 0, hi = N - 1;
-  while (hi - lo > 1) {
-    int mid = (hi + lo) / 2;
-    if (mid == lo) mid++;
-    else if (mid == hi) mid--;
-    if (X >= x[mid]) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
+// The cell index along each axis is the bisection's (markers.c:48-59), found
+// from a guess (guess_grid) -- the same index for any non-decreasing grid.
 
 // Cell location, fused with what the step does next: the bucket key and its
 // count (when `count` is non-null) and the density loop of main_subduction.c:
@@ -55,14 +46,16 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
     gxc = sgrid + g.NX + g.NY;
     gyc = sgrid + 2 * g.NX + g.NY + 1;
   }
+  const double ginvx = (double)(g.NX - 1) / (gx[g.NX - 1] - gx[0]);
+  const double ginvy = (double)(g.NY - 1) / (gy[g.NY - 1] - gy[0]);
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int out = 0, bad = 0, k = -1;
   if (m < n) {
     const double xm = X[m], ym = Y[m];
     int cx = -1, cy = -1;
     if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
-      cx = bisect(gx, g.NX, xm);
-      cy = bisect(gy, g.NY, ym);
+      cx = guess_grid(gx, g.NX, xm, ginvx);
+      cy = guess_grid(gy, g.NY, ym, ginvy);
       if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
         bad = 1;
       } else {
@@ -336,14 +329,7 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32
     if (have) {
       const int p = bs + rank;
       perm[p] = v;
-      if (GATHER) {
-        if (sg.with_xy) {
-          sg.xs[p] = sg.X[v];
-          sg.ys[p] = sg.Y[v];
-          sg.ms[p] = sg.Mat[v];
-        }
-        for (int k = 0; k < sg.pc.nc; k++) sg.pc.dst[k][p] = sg.pc.src[k][v];
-      }
+      if (GATHER) gather_entry(sg.pc, sg.with_xy, sg.X, sg.Y, sg.Mat, sg.xs, sg.ys, sg.ms, p, v);
     }
   } else if (lane < 4) {
     const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));
@@ -359,13 +345,7 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32
     }
     if (GATHER) {  // this lane's own bucket, read back by the lane that wrote it
       for (int p = s0; p < s1; p++) {
-        const int v = perm[p];
-        if (sg.with_xy) {
-          sg.xs[p] = sg.X[v];
-          sg.ys[p] = sg.Y[v];
-          sg.ms[p] = sg.Mat[v];
-        }
-        for (int k = 0; k < sg.pc.nc; k++) sg.pc.dst[k][p] = sg.pc.src[k][v];
+        gather_entry(sg.pc, sg.with_xy, sg.X, sg.Y, sg.Mat, sg.xs, sg.ys, sg.ms, p, perm[p]);
       }
     }
   }
