@@ -208,11 +208,24 @@ __global__ void __launch_bounds__(256) kb_partials_subt(const InstDev *__restric
 __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ insts, int n,
                                                  const int64_t *__restrict__ blk, const double *__restrict__ deltaT,
                                                  const double *__restrict__ dTsub) {
+  // the instance's grid coordinates are staged in LDS (as in kb_locate)
+  extern __shared__ double sgrid[];
   const int ii = find_inst(blk, n, blockIdx.x);
   const InstDev &I = insts[ii];
+  const DGrid &g = I.g;
+  for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
+  for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
+  for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
+  for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
+  __syncthreads();
+  DGrid gl = g;
+  gl.x = sgrid;
+  gl.y = sgrid + g.NX;
+  gl.xc = sgrid + g.NX + g.NY;
+  gl.yc = sgrid + 2 * g.NX + g.NY + 1;
   const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (m >= I.nMark) return;
-  advect_marker(m, I.g, I.X, I.Y, I.VX, I.VY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.cellX,
+  advect_marker(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.cellX,
                 I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, 0, I.T, I.Tdot, deltaT + I.node_off, dTsub + I.mark_off,
                 I.dt);
 }
@@ -341,15 +354,16 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   const int64_t mblk = E->hblk[0 * (n + 1) + n];
   const int B = E->tune.block;
   // findMarkerCells + density loop + bucket keys (main_subduction.c:243, 255-264)
-  FVM_TRY(fvm_launch_zero_i32(ctx, E->count, E->nbucket + 1));
-  FVM_TRY(fvm_launch_zero_i32(ctx, E->counters, 2));
+  const int64_t ntiles = fvm_scan_tiles(E->nbucket);
+  unsigned long long *scan_state = (unsigned long long *)E->sums;
+  FVM_TRY(fvm_launch_zero_sort(ctx, E->count, E->nbucket + 1, E->counters, scan_state, ntiles + 2));
   if (mblk) {
     FVM_PHASE(ctx, FVM_PHASE_LOCATE);
     kb_locate<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->key, E->slot,
                                                                   E->count, E->counters);
     FVM_LAUNCH_CHECK(ctx);
   }
-  FVM_TRY(fvm_launch_scan(ctx, E->count, E->start, E->sums, E->nbucket));
+  FVM_TRY(fvm_launch_scan_onepass(ctx, E->count, E->start, scan_state, E->nbucket));
   if (mblk) {
     FVM_PHASE(ctx, FVM_PHASE_SORT);
     kb_scatter<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->key, E->slot, E->start, E->perm);
@@ -366,7 +380,7 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   // pass 2 fused into advectMarkersRK (:435)
   if (mblk) {
     FVM_PHASE(ctx, FVM_PHASE_ADVECT);
-    kb_advect<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->deltaT, E->dTsub);
+    kb_advect<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->deltaT, E->dTsub);
     FVM_LAUNCH_CHECK(ctx);
   }
   return FVM_OK;
@@ -501,7 +515,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
   ENS_ALLOC(ctx, E->perm, sizeof(int32_t) * E->nmark);
   ENS_ALLOC(ctx, E->count, sizeof(int32_t) * (E->nbucket + 1));
   ENS_ALLOC(ctx, E->start, sizeof(int32_t) * (E->nbucket + 1));
-  ENS_ALLOC(ctx, E->sums, sizeof(int32_t) * (ntiles + 1));
+  ENS_ALLOC(ctx, E->sums, sizeof(unsigned long long) * (ntiles + 2));  // one-pass scan state
   ENS_ALLOC(ctx, E->counters, sizeof(int32_t) * 4);
   ENS_ALLOC(ctx, E->part, sizeof(double) * parttot);
   ENS_ALLOC(ctx, E->dTsub, sizeof(double) * E->nmark);

@@ -278,8 +278,13 @@ int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays 
                                double dt_thermal);
 
 // sort building blocks reused by the batched ensemble (sort.hip)
-int fvm_launch_zero_i32(fvm_ctx *ctx, int32_t *a, int64_t n);
-int fvm_launch_scan(fvm_ctx *ctx, const int32_t *count, int32_t *start, int32_t *sums, int64_t nk);
+// One launch: count[0..ncount) = 0, counters[0..1] = 0, state[0..nstate) = 0.
+int fvm_launch_zero_sort(fvm_ctx *ctx, int32_t *count, int64_t ncount, int32_t *counters, unsigned long long *state,
+                         int64_t nstate);
+// Exclusive scan of count[0..nk) into start[0..nk], start[nk] = total, in one
+// pass; `state` holds fvm_scan_tiles(nk) + 2 zeroed words.
+int fvm_launch_scan_onepass(fvm_ctx *ctx, const int32_t *count, int32_t *start, unsigned long long *state,
+                            int64_t nk);
 int64_t fvm_scan_tiles(int64_t nk);
 int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm);
 __device__ __forceinline__ int bisect_grid(const double *__restrict__ x, int N, double X) {

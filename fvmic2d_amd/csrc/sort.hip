@@ -80,22 +80,12 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
     const int sl = wave_count_slot(k, count);
     if (k >= 0) slot[m] = sl;
   }
-  // block-wide counts of outside markers and bad cells (integer: order-free)
-  __shared__ int s_out, s_bad;
-  if (threadIdx.x == 0) {
-    s_out = 0;
-    s_bad = 0;
-  }
-  __syncthreads();
-  unsigned long long bo = __ballot(out), bb = __ballot(bad);
+  // counts of outside markers and bad cells (integer: order-free), one atomic
+  // per wavefront that has any
+  const unsigned long long bo = __ballot(out), bb = __ballot(bad);
   if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&s_out, __popcll(bo));
-    atomicAdd(&s_bad, __popcll(bb));
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    if (s_out) atomicAdd(&counters[0], s_out);
-    if (s_bad) atomicAdd(&counters[1], s_bad);
+    if (bo) atomicAdd(&counters[0], __popcll(bo));
+    if (bb) atomicAdd(&counters[1], __popcll(bb));
   }
 }
 
@@ -121,75 +111,19 @@ __global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *
   if (k >= 0) slot[m] = sl;
 }
 
-__global__ void k_zero_i32(int32_t *__restrict__ a, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0;
-}
-
 // exclusive scan of count[0..nk) into start[0..nk], start[nk] = total
 #define SCAN_BS 256
 #define SCAN_IPT 8
 #define SCAN_TILE (SCAN_BS * SCAN_IPT)
-
-__global__ void __launch_bounds__(SCAN_BS) k_scan_tiles(const int32_t *__restrict__ in, int32_t *__restrict__ out,
-                                                        int32_t *__restrict__ tile_sums, int64_t n) {
-  __shared__ int32_t s[SCAN_BS];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
-  int32_t v[SCAN_IPT];
-  int32_t t = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_IPT; i++) {
-    v[i] = (base + i < n) ? in[base + i] : 0;
-    t += v[i];
-  }
-  s[threadIdx.x] = t;
-  __syncthreads();
-  for (int off = 1; off < SCAN_BS; off <<= 1) {
-    int32_t a = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += a;
-    __syncthreads();
-  }
-  int32_t run = s[threadIdx.x] - t;  // exclusive prefix of this thread
-#pragma unroll
-  for (int i = 0; i < SCAN_IPT; i++) {
-    if (base + i < n) out[base + i] = run;
-    run += v[i];
-  }
-  if (threadIdx.x == SCAN_BS - 1) tile_sums[blockIdx.x] = s[SCAN_BS - 1];
-}
-
-__global__ void __launch_bounds__(1024) k_scan_sums(int32_t *__restrict__ sums, int64_t n, int32_t *__restrict__ total) {
-  __shared__ int32_t s[1024];
-  __shared__ int32_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int64_t base = 0; base < n; base += 1024) {
-    const int64_t i = base + threadIdx.x;
-    const int32_t v = i < n ? sums[i] : 0;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      int32_t a = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
-      __syncthreads();
-      s[threadIdx.x] += a;
-      __syncthreads();
-    }
-    if (i < n) sums[i] = carry + s[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 1023) carry += s[1023];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *total = carry;
-}
 
 // ---- one-pass scan (decoupled look-back) --------------------------------------------
 // Each block takes the next tile number from a ticket, so every tile it looks
 // back at was started earlier and will publish.  A tile publishes its aggregate
 // (status 1) and then its inclusive prefix (status 2) as one 64-bit word
 // (status << 32 | value); a successor's first wavefront reads its predecessors'
-// words 64 at a time and sums them back to the nearest inclusive prefix.  The wait is bounded: after SCAN_SPIN_MAX polls the tile
-// sets the sticky flag *fail and stops waiting (its output is then invalid);
+// words 64 at a time and sums them back to the nearest inclusive prefix.  The
+// wait is bounded: after SCAN_SPIN_MAX polls the tile sets the sticky flag
+// *fail and stops waiting (its output is then invalid);
 // fvm_ctx_sync reports the flag.  state[0] = ticket, state[2 + t] = word of tile t.
 #define SCAN_SPIN_MAX (1 << 20)
 
@@ -282,14 +216,6 @@ __global__ void __launch_bounds__(256) k_zero_counts(int32_t *__restrict__ count
   if (i < 2) counters[i] = 0;
 }
 
-__global__ void __launch_bounds__(SCAN_BS) k_scan_add(int32_t *__restrict__ out, const int32_t *__restrict__ tile_sums,
-                                                      int64_t n) {
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
-  const int32_t add = tile_sums[blockIdx.x];
-  for (int i = threadIdx.x; i < SCAN_TILE; i += SCAN_BS)
-    if (base + i < n) out[base + i] += add;
-}
-
 __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__restrict__ key,
                                                  const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
                                                  int32_t *__restrict__ perm) {
@@ -305,11 +231,25 @@ __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__res
 // ranks the (usually <= 64) entries of the cell's four buckets; a cell with
 // more entries falls back to an insertion sort per bucket.  With GATHER the
 // payload columns of each entry are copied to its final position as well.
+// With GATHER the cells are visited in 8 x 8 tiles (y fastest inside a tile),
+// tiles spread over the XCDs in contiguous runs, so that the marker rows a
+// tile's wavefronts read share L2 lines.
 template <bool GATHER>
-__global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, const int32_t *__restrict__ start,
-                                                      int32_t *__restrict__ perm, SortGather sg) {
-  const int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+__global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, int32_t ncx, int32_t ncy,
+                                                      const int32_t *__restrict__ start, int32_t *__restrict__ perm,
+                                                      SortGather sg) {
   const int lane = threadIdx.x & 63;
+  int64_t c;
+  if (GATHER) {
+    const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int ntx = (ncx + 7) >> 3;
+    const int64_t tile = w >> 6;
+    const int l = (int)(w & 63);
+    const int cx = (int)(tile % ntx) * 8 + (l >> 3), cy = (int)(tile / ntx) * 8 + (l & 7);
+    c = (cx < ncx && cy < ncy) ? (int64_t)cy * ncx + cx : ncell;
+  } else {
+    c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  }
   if (c >= ncell) return;  // whole wavefront exits together
   const int b0 = start[4 * c], b1 = start[4 * c + 1], b2 = start[4 * c + 2], b3 = start[4 * c + 3];
   const int b4 = start[4 * c + 4];
@@ -397,11 +337,15 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, c
     FVM_LAUNCH_CHECK(ctx);
   }
   if (sg && n > 0) {
-    k_bucket_order<true><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm, *sg);
+    const int32_t ncx = g->NX - 1, ncy = g->NY - 1;
+    const int64_t ntiles = (int64_t)((ncx + 7) / 8) * ((ncy + 7) / 8);
+    k_bucket_order<true><<<fvm_blocks(ntiles * 64 * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, ncx, ncy, start,
+                                                                                   perm, *sg);
   } else {
     SortGather none;
     memset(&none, 0, sizeof(none));
-    k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm, none);
+    k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, 0, 0, start, perm,
+                                                                                none);
   }
   FVM_LAUNCH_CHECK(ctx);
   ctx->sort_epoch = ctx->epoch;
@@ -501,32 +445,33 @@ int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, con
 }
 
 // ---- helpers for the batched ensemble (ensemble.hip) ---------------------------------
-int fvm_launch_zero_i32(fvm_ctx *ctx, int32_t *a, int64_t n) {
-  FVM_PHASE(ctx, FVM_PHASE_SORT);
-  k_zero_i32<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(a, n);
-  FVM_LAUNCH_CHECK(ctx);
-  return FVM_OK;
-}
-
-int fvm_launch_scan(fvm_ctx *ctx, const int32_t *count, int32_t *start, int32_t *sums, int64_t nk) {
-  FVM_PHASE(ctx, FVM_PHASE_SORT);
-  const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
-  k_scan_tiles<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(count, start, sums, nk);
-  FVM_LAUNCH_CHECK(ctx);
-  k_scan_sums<<<1, 1024, 0, ctx->stream>>>(sums, ntiles, start + nk);
-  FVM_LAUNCH_CHECK(ctx);
-  k_scan_add<<<(unsigned)ntiles, SCAN_BS, 0, ctx->stream>>>(start, sums, nk);
-  FVM_LAUNCH_CHECK(ctx);
-  return FVM_OK;
-}
-
 int64_t fvm_scan_tiles(int64_t nk) { return (nk + SCAN_TILE - 1) / SCAN_TILE; }
+
+int fvm_launch_zero_sort(fvm_ctx *ctx, int32_t *count, int64_t ncount, int32_t *counters, unsigned long long *state,
+                         int64_t nstate) {
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  const int64_t nmax = ncount > nstate ? ncount : nstate;
+  k_zero_counts<<<fvm_blocks(nmax > 2 ? nmax : 2, 256), 256, 0, ctx->stream>>>(count, ncount, counters, state,
+                                                                              nstate);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
+
+int fvm_launch_scan_onepass(fvm_ctx *ctx, const int32_t *count, int32_t *start, unsigned long long *state,
+                            int64_t nk) {
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  k_scan_onepass<<<(unsigned)fvm_scan_tiles(nk), SCAN_BS, 0, ctx->stream>>>(count, start, state, nk,
+                                                                          (int32_t *)ctx->red.ptr + FVM_FLAG_SCAN_FAIL);
+  FVM_LAUNCH_CHECK(ctx);
+  return FVM_OK;
+}
 
 int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm) {
   FVM_PHASE(ctx, FVM_PHASE_SORT);
   SortGather none;
   memset(&none, 0, sizeof(none));
-  k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, start, perm, none);
+  k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, 0, 0, start, perm,
+                                                                              none);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
