@@ -50,33 +50,44 @@ struct fvm_ensemble {
   NodeGroup *dnodes = nullptr;        // [group][instance]
   int64_t *dblk = nullptr;            // block offsets: markers, nodes, then one row per group
   std::vector<int64_t> hblk;
+  // instance of every block of every row (row r starts at bi_off[r]), and of
+  // every 8 x 8 cell tile of the ordering pass
+  int32_t *dbi = nullptr, *dtile_inst = nullptr;
+  int64_t *dbi_off = nullptr;
+  std::vector<int64_t> hbi_off;
+  size_t bi_cap = 0;
   int64_t *dpartoff = nullptr;        // [group][instance] offsets into part
   int32_t *key = nullptr, *slot = nullptr, *perm = nullptr, *count = nullptr, *start = nullptr,
           *sums = nullptr, *counters = nullptr;
   double *part = nullptr, *dTsub = nullptr, *deltaT = nullptr;
+  // bucket-ordered payload at global positions: X, Y, Mat and the distinct
+  // marker columns the projections and the dTsub pass read ("roles"), filled
+  // by the ordering pass; the partials passes read it as contiguous runs
+  int ncol = 0, mt_role = -1;
+  double *xs = nullptr, *ys = nullptr, *pcols = nullptr;
+  int32_t *ms = nullptr;
+  const double **dcolsrc = nullptr;     // [role][instance] source column
+  int64_t *dcell_off = nullptr;         // [instance + 1] first global basic cell, then [instance + 1] first tile
+  int64_t ntile = 0;                    // 8 x 8 cell tiles over all instances
+  ProjGroup *dgroups_sorted = nullptr;  // [group][instance], columns -> payload roles
   size_t lds_locate = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
 
 // ---- block -> instance ----------------------------------------------------------------
-__device__ __forceinline__ int find_inst(const int64_t *__restrict__ blk, int n, int64_t b) {
-  int lo = 0, hi = n;  // blk[lo] <= b < blk[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (blk[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
+// Every kind of work pads each instance's range to whole blocks; a table maps
+// a block (row table `bi`) or an ordering tile to its instance, so a block
+// finds its instance with one load.
 
 // ---- kernels ------------------------------------------------------------------------------
 __global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ insts, int n,
-                                                 const int64_t *__restrict__ blk, int32_t *__restrict__ key,
+                                                 const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
+                                                 int32_t *__restrict__ key,
                                                  int32_t *__restrict__ slot, int32_t *__restrict__ count,
                                                  int32_t *__restrict__ counters) {
   extern __shared__ double sgrid[];
-  const int ii = find_inst(blk, n, blockIdx.x);
+  const int ii = bi[blockIdx.x];
   const InstDev &I = insts[ii];
   const DGrid &g = I.g;
   for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
@@ -124,10 +135,10 @@ __global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ ins
 }
 
 __global__ void __launch_bounds__(512) kb_scatter(const InstDev *__restrict__ insts, int n,
-                                                  const int64_t *__restrict__ blk, const int32_t *__restrict__ key,
-                                                  const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
-                                                  int32_t *__restrict__ perm) {
-  const int ii = find_inst(blk, n, blockIdx.x);
+                                                  const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
+                                                  const int32_t *__restrict__ key, const int32_t *__restrict__ slot,
+                                                  const int32_t *__restrict__ start, int32_t *__restrict__ perm) {
+  const int ii = bi[blockIdx.x];
   const InstDev &I = insts[ii];
   const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (m >= I.nMark) return;
@@ -136,13 +147,62 @@ __global__ void __launch_bounds__(512) kb_scatter(const InstDev *__restrict__ in
   perm[start[k] + slot[I.mark_off + m]] = (int32_t)m;
 }
 
+// Bucket order of every instance's cells (one wavefront per cell) with the
+// payload gathered from the owning instance's columns.  Cells are visited in
+// 8 x 8 tiles per instance (y fastest inside a tile; tile_off = first tile of
+// each instance), tiles spread over the XCDs in contiguous runs, as in the
+// single-instance ordering pass.
+__global__ void __launch_bounds__(256) kb_order_gather(const InstDev *__restrict__ insts, int n,
+                                                       const int64_t *__restrict__ cell_off,
+                                                       const int64_t *__restrict__ tile_off,
+                                                       const int32_t *__restrict__ tile_inst,
+                                                       const int32_t *__restrict__ start, int32_t *__restrict__ perm,
+                                                       double *__restrict__ xs, double *__restrict__ ys,
+                                                       int32_t *__restrict__ ms, int ncol,
+                                                       const double *const *__restrict__ colsrc,
+                                                       double *__restrict__ pcols, int64_t nmark) {
+  const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int64_t tile = w >> 6;
+  if (tile >= tile_off[n]) return;  // whole wavefront exits together
+  const int ii = tile_inst[tile];
+  const InstDev &I = insts[ii];
+  const int ncx = I.g.NX - 1, ncy = I.g.NY - 1, ntx = (ncx + 7) >> 3;
+  const int64_t tl = tile - tile_off[ii];
+  const int l = (int)(w & 63);
+  const int cx = (int)(tl % ntx) * 8 + (l >> 3), cy = (int)(tl / ntx) * 8 + (l & 7);
+  if (cx >= ncx || cy >= ncy) return;
+  const int64_t c = cell_off[ii] + (int64_t)cy * ncx + cx;
+  bucket_order_cell(c, start, perm, [&](int p, int v) {
+    xs[p] = I.X[v];
+    ys[p] = I.Y[v];
+    ms[p] = I.Mat[v];
+    for (int k = 0; k < ncol; k++) pcols[(int64_t)k * nmark + p] = colsrc[(int64_t)k * n + ii][v];
+  });
+}
+
+// Groups of one (method, field count) share a launch: launch row blockIdx.y
+// is group gi[blockIdx.y]; a row's blocks beyond that group's count exit.
+#define KB_MAX_MULTI 8
+struct KbGroups {
+  int gi[KB_MAX_MULTI], sx[KB_MAX_MULTI], sy[KB_MAX_MULTI];
+};
+
 template <int METHOD, int NF, int G>
 __global__ void __launch_bounds__(512) kb_partials(const InstDev *__restrict__ insts, int n,
-                                                   const int64_t *__restrict__ blk, const ProjGroup *__restrict__ groups,
-                                                   const int64_t *__restrict__ partoff, const int32_t *__restrict__ start,
-                                                   const int32_t *__restrict__ perm, double *__restrict__ part, int sx,
-                                                   int sy) {
-  const int ii = find_inst(blk, n, blockIdx.x);
+                                                   const int64_t *__restrict__ blk_rows,
+                                                   const int32_t *__restrict__ bi_all,
+                                                   const int64_t *__restrict__ bi_off,
+                                                   const ProjGroup *__restrict__ all_groups,
+                                                   const int64_t *__restrict__ all_partoff,
+                                                   const int32_t *__restrict__ start, const double *__restrict__ xs,
+                                                   const double *__restrict__ ys, const int32_t *__restrict__ ms,
+                                                   double *__restrict__ part, KbGroups kg) {
+  const int gi = kg.gi[blockIdx.y], sx = kg.sx[blockIdx.y], sy = kg.sy[blockIdx.y];
+  const int64_t *blk = blk_rows + (int64_t)(2 + gi) * (n + 1);
+  if ((int64_t)blockIdx.x >= blk[n]) return;
+  const ProjGroup *groups = all_groups + (int64_t)gi * n;
+  const int64_t *partoff = all_partoff + (int64_t)gi * n;
+  const int ii = bi_all[bi_off[2 + gi] + blockIdx.x];
   const InstDev &I = insts[ii];
   const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
   const int64_t nL = (int64_t)Lx.n * Ly.n;
@@ -152,8 +212,8 @@ __global__ void __launch_bounds__(512) kb_partials(const InstDev *__restrict__ i
   const bool live = t < nL;
   const int lx = live ? (int)(t % Lx.n) + Lx.lo : 0;
   const int ly = live ? (int)(t / Lx.n) + Ly.lo : 0;
-  partials_cell<METHOD, NF, G>(I.g, groups[ii], start + I.bucket_off, perm, I.X, I.Y, I.Mat, part + partoff[ii], nL,
-                               t, lx, ly, j, live);
+  partials_cell_sorted<METHOD, NF, G>(I.g, groups[ii], start + I.bucket_off, xs, ys, ms, part + partoff[ii], nL, t,
+                                      lx, ly, j, live);
 }
 
 // (group, field) pairs of one node launch: launch row blockIdx.y = pair
@@ -162,9 +222,9 @@ struct PairTab {
 };
 
 __global__ void __launch_bounds__(256) kb_nodes(const InstDev *__restrict__ insts, int n,
-                                                const int64_t *__restrict__ blk, const NodeGroup *__restrict__ nodes,
-                                                PairTab pt) {
-  const int ii = find_inst(blk, n, blockIdx.x);
+                                                const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
+                                                const NodeGroup *__restrict__ nodes, PairTab pt) {
+  const int ii = bi[blockIdx.x];
   const InstDev &I = insts[ii];
   const int64_t i = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (i >= (int64_t)I.g.NX * I.g.NY) return;
@@ -176,12 +236,15 @@ __global__ void __launch_bounds__(256) kb_nodes(const InstDev *__restrict__ inst
 template <int G>
 __global__ void __launch_bounds__(256) kb_partials_subt(const InstDev *__restrict__ insts, int n,
                                                         const int64_t *__restrict__ blk,
+                                                        const int32_t *__restrict__ bi,
                                                         const ProjGroup *__restrict__ groups,
                                                         const int64_t *__restrict__ partoff,
                                                         const int32_t *__restrict__ start,
-                                                        const int32_t *__restrict__ perm, double *__restrict__ part,
+                                                        const int32_t *__restrict__ perm,
+                                                        const double *__restrict__ xs, const double *__restrict__ ys,
+                                                        const double *__restrict__ mt, double *__restrict__ part,
                                                         double *__restrict__ dTsub) {
-  const int ii = find_inst(blk, n, blockIdx.x);
+  const int ii = bi[blockIdx.x];
   const InstDev &I = insts[ii];
   const int nLx = I.g.NX - 1, nLy = I.g.NY - 1;
   const int64_t nL = (int64_t)nLx * nLy;
@@ -196,21 +259,21 @@ __global__ void __launch_bounds__(256) kb_partials_subt(const InstDev *__restric
   sb.rho = I.rhon;
   sb.Cp = I.Cpn;
   sb.k = I.kn;
-  sb.MT = I.T;
-  sb.dTsub = dTsub + I.mark_off;
+  sb.MT = mt;                     // payload column of marker T (global positions)
+  sb.dTsub = dTsub + I.mark_off;  // written at perm[p], the instance's marker index
   sb.rho0 = I.rho0;
   sb.dt = I.dt;
   sb.dsub = I.dsub;
-  partials_cell_subt<G>(I.g, groups[ii], start + I.bucket_off, perm, I.X, I.Y, part + partoff[ii], nL, t, lx, ly, j,
-                        live, sb);
+  partials_cell_subt_sorted<G>(I.g, groups[ii], start + I.bucket_off, perm, xs, ys, part + partoff[ii], nL, t, lx,
+                               ly, j, live, sb);
 }
 
 __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ insts, int n,
-                                                 const int64_t *__restrict__ blk, const double *__restrict__ deltaT,
-                                                 const double *__restrict__ dTsub) {
+                                                 const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
+                                                 const double *__restrict__ deltaT, const double *__restrict__ dTsub) {
   // the instance's grid coordinates are staged in LDS (as in kb_locate)
   extern __shared__ double sgrid[];
-  const int ii = find_inst(blk, n, blockIdx.x);
+  const int ii = bi[blockIdx.x];
   const InstDev &I = insts[ii];
   const DGrid &g = I.g;
   for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
@@ -233,6 +296,7 @@ __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ ins
 // ---- host side ----------------------------------------------------------------------------
 // block offsets rows: 0 markers (tune.block), 1 nodes (256), 2 + g lattice cells of group g
 static int64_t *blk_row(fvm_ensemble *E, int r) { return E->dblk + (int64_t)r * (E->n + 1); }
+static const int32_t *bi_row(fvm_ensemble *E, int r) { return E->dbi + E->hbi_off[r]; }
 
 static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
   const int n = E->n, rows = 2 + E->ng + 1;
@@ -250,58 +314,96 @@ static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
       E->hblk[r * (n + 1) + i + 1] = E->hblk[r * (n + 1) + i] + (th + 255) / 256;
     }
   }
+  E->hbi_off.assign(rows + 1, 0);
+  for (int r = 0; r < rows; r++) E->hbi_off[r + 1] = E->hbi_off[r] + E->hblk[(size_t)r * (n + 1) + n];
+  std::vector<int32_t> hbi((size_t)E->hbi_off[rows]);
+  for (int r = 0; r < rows; r++)
+    for (int i = 0; i < n; i++)
+      for (int64_t b = E->hblk[(size_t)r * (n + 1) + i]; b < E->hblk[(size_t)r * (n + 1) + i + 1]; b++)
+        hbi[(size_t)(E->hbi_off[r] + b)] = i;
+  const size_t need = hbi.size() * sizeof(int32_t) + 8;
+  if (need > E->bi_cap) {
+    FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (E->dbi) (void)hipFree(E->dbi);
+    E->dbi = nullptr;
+    E->bi_cap = 0;
+    FVM_HIP(ctx, hipMalloc((void **)&E->dbi, need));
+    E->bi_cap = need;
+  }
+  if (!E->dbi_off) FVM_HIP(ctx, hipMalloc((void **)&E->dbi_off, sizeof(int64_t) * (rows + 1)));
   FVM_HIP(ctx, hipMemcpyAsync(E->dblk, E->hblk.data(), E->hblk.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dbi, hbi.data(), hbi.size() * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dbi_off, E->hbi_off.data(), sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice,
                               ctx->stream));
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return FVM_OK;
 }
 
 template <int METHOD, int NF>
-static int launch_kb_partials(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
+static int launch_kb_partials(fvm_ctx *ctx, fvm_ensemble *E, const KbGroups &kg, int nrow) {
   const int n = E->n;
-  const int64_t nblk = E->hblk[(2 + gi) * (n + 1) + n];
+  int64_t nblk = 0;
+  for (int r = 0; r < nrow; r++) nblk = std::max(nblk, E->hblk[(2 + kg.gi[r]) * (n + 1) + n]);
   if (nblk == 0) return FVM_OK;
-  const ProjGroup *groups = E->dgroups + (int64_t)gi * n;
-  const int64_t *po = E->dpartoff + (int64_t)gi * n;
-  const int64_t *blk = blk_row(E, 2 + gi);
-  const int sx = E->gstag_x[gi], sy = E->gstag_y[gi];
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
+  const dim3 grid((unsigned)nblk, (unsigned)nrow);
   switch (E->tune.lanes) {
     case 8:
-      kb_partials<METHOD, NF, 8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start,
-                                                                          E->perm, E->part, sx, sy);
+      kb_partials<METHOD, NF, 8><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, E->dblk, E->dbi, E->dbi_off,
+                                                                E->dgroups_sorted, E->dpartoff,
+                                                                E->start, E->xs, E->ys, E->ms, E->part, kg);
       break;
     case 32:
-      kb_partials<METHOD, NF, 32><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start,
-                                                                           E->perm, E->part, sx, sy);
+      kb_partials<METHOD, NF, 32><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, E->dblk, E->dbi, E->dbi_off,
+                                                                E->dgroups_sorted, E->dpartoff,
+                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg);
       break;
     default:
-      kb_partials<METHOD, NF, 16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start,
-                                                                           E->perm, E->part, sx, sy);
+      kb_partials<METHOD, NF, 16><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, E->dblk, E->dbi, E->dbi_off,
+                                                                E->dgroups_sorted, E->dpartoff,
+                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg);
   }
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
 
 template <int METHOD>
-static int kb_partials_nf(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
-  switch (E->gnf[gi]) {
-    case 1: return launch_kb_partials<METHOD, 1>(ctx, E, gi);
-    case 2: return launch_kb_partials<METHOD, 2>(ctx, E, gi);
-    case 3: return launch_kb_partials<METHOD, 3>(ctx, E, gi);
-    case 4: return launch_kb_partials<METHOD, 4>(ctx, E, gi);
-    case 5: return launch_kb_partials<METHOD, 5>(ctx, E, gi);
-    case 6: return launch_kb_partials<METHOD, 6>(ctx, E, gi);
-    case 7: return launch_kb_partials<METHOD, 7>(ctx, E, gi);
-    case 8: return launch_kb_partials<METHOD, 8>(ctx, E, gi);
-    default: return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "ensemble group with %d fields", E->gnf[gi]);
+static int kb_partials_nf(fvm_ctx *ctx, fvm_ensemble *E, int nf, const KbGroups &kg, int nrow) {
+  switch (nf) {
+    case 1: return launch_kb_partials<METHOD, 1>(ctx, E, kg, nrow);
+    case 2: return launch_kb_partials<METHOD, 2>(ctx, E, kg, nrow);
+    case 3: return launch_kb_partials<METHOD, 3>(ctx, E, kg, nrow);
+    case 4: return launch_kb_partials<METHOD, 4>(ctx, E, kg, nrow);
+    case 5: return launch_kb_partials<METHOD, 5>(ctx, E, kg, nrow);
+    case 6: return launch_kb_partials<METHOD, 6>(ctx, E, kg, nrow);
+    case 7: return launch_kb_partials<METHOD, 7>(ctx, E, kg, nrow);
+    case 8: return launch_kb_partials<METHOD, 8>(ctx, E, kg, nrow);
+    default: return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "ensemble group with %d fields", nf);
   }
 }
 
-static int kb_partials_dispatch(fvm_ctx *ctx, fvm_ensemble *E, int gi) {
-  if (E->gmethod[gi] == PM_ARITH) return kb_partials_nf<PM_ARITH>(ctx, E, gi);
-  if (E->gmethod[gi] == PM_LOCAL) return kb_partials_nf<PM_LOCAL>(ctx, E, gi);
-  return fvm_set_err(ctx, FVM_ERR_BAD_METHOD, "ensemble projections support ARITHMETIC and ARITHMETIC_LOCAL");
+// every projection group, groups of one (method, field count) in one launch
+static int kb_partials_all(fvm_ctx *ctx, fvm_ensemble *E) {
+  std::vector<bool> done(E->ng, false);
+  for (int g0 = 0; g0 < E->ng; g0++) {
+    if (done[g0]) continue;
+    KbGroups kg;
+    memset(&kg, 0, sizeof(kg));
+    int nrow = 0;
+    for (int g = g0; g < E->ng && nrow < KB_MAX_MULTI; g++) {
+      if (done[g] || E->gmethod[g] != E->gmethod[g0] || E->gnf[g] != E->gnf[g0]) continue;
+      kg.gi[nrow] = g;
+      kg.sx[nrow] = E->gstag_x[g];
+      kg.sy[nrow] = E->gstag_y[g];
+      nrow++;
+      done[g] = true;
+    }
+    if (E->gmethod[g0] == PM_ARITH) FVM_TRY(kb_partials_nf<PM_ARITH>(ctx, E, E->gnf[g0], kg, nrow));
+    else if (E->gmethod[g0] == PM_LOCAL) FVM_TRY(kb_partials_nf<PM_LOCAL>(ctx, E, E->gnf[g0], kg, nrow));
+    else return fvm_set_err(ctx, FVM_ERR_BAD_METHOD, "ensemble projections support ARITHMETIC and ARITHMETIC_LOCAL");
+  }
+  return FVM_OK;
 }
 
 static int launch_kb_partials_subt(fvm_ctx *ctx, fvm_ensemble *E) {
@@ -311,19 +413,21 @@ static int launch_kb_partials_subt(fvm_ctx *ctx, fvm_ensemble *E) {
   const ProjGroup *groups = E->dgroups + (int64_t)gi * n;
   const int64_t *po = E->dpartoff + (int64_t)gi * n;
   const int64_t *blk = blk_row(E, 2 + gi);
+  const double *mt = E->pcols + (int64_t)E->mt_role * E->nmark;
+  const int32_t *bi = bi_row(E, 2 + gi);
   FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
   switch (E->tune.lanes) {
     case 8:
-      kb_partials_subt<8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start, E->perm,
-                                                                   E->part, E->dTsub);
+      kb_partials_subt<8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
+                                                                   E->xs, E->ys, mt, E->part, E->dTsub);
       break;
     case 32:
-      kb_partials_subt<32><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start, E->perm,
-                                                                    E->part, E->dTsub);
+      kb_partials_subt<32><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
+                                                                    E->xs, E->ys, mt, E->part, E->dTsub);
       break;
     default:
-      kb_partials_subt<16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, groups, po, E->start, E->perm,
-                                                                    E->part, E->dTsub);
+      kb_partials_subt<16><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
+                                                                    E->xs, E->ys, mt, E->part, E->dTsub);
   }
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
@@ -344,7 +448,8 @@ static int launch_kb_nodes(fvm_ctx *ctx, fvm_ensemble *E, int g0, int g1) {
     }
   if (np == 0) return FVM_OK;
   FVM_PHASE(ctx, FVM_PHASE_NODES);
-  kb_nodes<<<dim3((unsigned)nblk, np), 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1), E->dnodes, pt);
+  kb_nodes<<<dim3((unsigned)nblk, np), 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1), bi_row(E, 1), E->dnodes,
+                                                              pt);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
@@ -359,19 +464,26 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   FVM_TRY(fvm_launch_zero_sort(ctx, E->count, E->nbucket + 1, E->counters, scan_state, ntiles + 2));
   if (mblk) {
     FVM_PHASE(ctx, FVM_PHASE_LOCATE);
-    kb_locate<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->key, E->slot,
-                                                                  E->count, E->counters);
+    kb_locate<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), bi_row(E, 0), E->key,
+                                                                  E->slot, E->count, E->counters);
     FVM_LAUNCH_CHECK(ctx);
   }
   FVM_TRY(fvm_launch_scan_onepass(ctx, E->count, E->start, scan_state, E->nbucket));
   if (mblk) {
     FVM_PHASE(ctx, FVM_PHASE_SORT);
-    kb_scatter<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->key, E->slot, E->start, E->perm);
+    kb_scatter<<<(unsigned)mblk, B, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 0), bi_row(E, 0), E->key, E->slot,
+                                                      E->start, E->perm);
     FVM_LAUNCH_CHECK(ctx);
   }
-  FVM_TRY(fvm_launch_bucket_order(ctx, E->ncell, E->start, E->perm));
+  {
+    FVM_PHASE(ctx, FVM_PHASE_SORT);
+    kb_order_gather<<<fvm_blocks(E->ntile * 64 * 64, 256), 256, 0, ctx->stream>>>(
+        E->dinst, n, E->dcell_off, E->dcell_off + n + 1, E->dtile_inst, E->start, E->perm, E->xs, E->ys, E->ms, E->ncol,
+        E->dcolsrc, E->pcols, E->nmark);
+    FVM_LAUNCH_CHECK(ctx);
+  }
   // projections (:268)
-  for (int gi = 0; gi < E->ng; gi++) FVM_TRY(kb_partials_dispatch(ctx, E, gi));
+  FVM_TRY(kb_partials_all(ctx, E));
   FVM_TRY(launch_kb_nodes(ctx, E, 0, E->ng));
   // subgridTemperatureChanges (:415): pass 1 inside the dTsub projection, whose
   // node pass applies the VecAXPBYPCZ epilogue
@@ -380,7 +492,8 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   // pass 2 fused into advectMarkersRK (:435)
   if (mblk) {
     FVM_PHASE(ctx, FVM_PHASE_ADVECT);
-    kb_advect<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), E->deltaT, E->dTsub);
+    kb_advect<<<(unsigned)mblk, B, E->lds_locate, ctx->stream>>>(E->dinst, n, blk_row(E, 0), bi_row(E, 0), E->deltaT,
+                                                                  E->dTsub);
     FVM_LAUNCH_CHECK(ctx);
   }
   return FVM_OK;
@@ -394,8 +507,10 @@ extern "C" int fvm_ensemble_destroy(fvm_ensemble *E) {
   if (!E) return FVM_OK;
   if (E->exec) (void)hipGraphExecDestroy(E->exec);
   if (E->graph) (void)hipGraphDestroy(E->graph);
-  void *bufs[] = {E->dinst, E->dgroups, E->dnodes, E->dblk, E->dpartoff, E->key,   E->slot,  E->perm,
-                  E->count, E->start,   E->sums,   E->counters, E->part, E->dTsub, E->deltaT};
+  void *bufs[] = {E->dinst, E->dgroups, E->dnodes, E->dblk,    E->dpartoff, E->key,       E->slot,
+                  E->perm,  E->count,   E->start,  E->sums,    E->counters, E->part,      E->dTsub,
+                  E->deltaT, E->xs,     E->ys,     E->pcols,   E->ms,       (void *)E->dcolsrc, E->dcell_off,
+                  E->dgroups_sorted, E->dbi, E->dtile_inst, E->dbi_off};
   for (void *b : bufs) free_dev(b);
   delete E;
   return FVM_OK;
@@ -571,6 +686,64 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
       q.eb = pg.eb;
     }
   }
+  // payload roles: fields whose column pointers agree in every instance share one
+  std::vector<std::vector<const double *>> roles;
+  auto role_of = [&](const std::vector<const double *> &v) -> int {
+    for (size_t r = 0; r < roles.size(); r++)
+      if (roles[r] == v) return (int)r;
+    roles.push_back(v);
+    return (int)roles.size() - 1;
+  };
+  std::vector<int> role_gk((size_t)E->ng * FVM_MAXF, -1);  // role of (group, field), -1 = material table
+  for (int gi = 0; gi < E->ng; gi++)
+    for (int k = 0; k < E->gnf[gi]; k++) {
+      if (E->hgroups[(size_t)gi * n].src[k].use_mat) continue;
+      std::vector<const double *> v(n);
+      for (int i = 0; i < n; i++) v[i] = E->hgroups[(size_t)gi * n + i].src[k].col;
+      role_gk[(size_t)gi * FVM_MAXF + k] = role_of(v);
+    }
+  {
+    std::vector<const double *> v(n);
+    for (int i = 0; i < n; i++) v[i] = E->hinst[i].T;
+    E->mt_role = role_of(v);
+  }
+  E->ncol = (int)roles.size();
+  ENS_ALLOC(ctx, E->xs, sizeof(double) * E->nmark);
+  ENS_ALLOC(ctx, E->ys, sizeof(double) * E->nmark);
+  ENS_ALLOC(ctx, E->ms, sizeof(int32_t) * E->nmark);
+  ENS_ALLOC(ctx, E->pcols, sizeof(double) * E->nmark * E->ncol);
+  ENS_ALLOC(ctx, E->dcolsrc, sizeof(const double *) * E->ncol * n);
+  ENS_ALLOC(ctx, E->dcell_off, sizeof(int64_t) * 2 * (n + 1));
+  ENS_ALLOC(ctx, E->dgroups_sorted, sizeof(ProjGroup) * NG * n);
+  std::vector<ProjGroup> hsorted = E->hgroups;  // the same groups reading the payload columns
+  for (int gi = 0; gi < E->ng; gi++)
+    for (int k = 0; k < E->gnf[gi]; k++) {
+      const int r = role_gk[(size_t)gi * FVM_MAXF + k];
+      if (r < 0) continue;
+      for (int i = 0; i < n; i++) hsorted[(size_t)gi * n + i].src[k].col = E->pcols + (int64_t)r * E->nmark;
+    }
+  std::vector<const double *> hcolsrc((size_t)E->ncol * n);
+  for (int r = 0; r < E->ncol; r++)
+    for (int i = 0; i < n; i++) hcolsrc[(size_t)r * n + i] = roles[r][i];
+  std::vector<int64_t> hcell(2 * (n + 1), 0);
+  for (int i = 0; i < n; i++) {
+    const int64_t ncx = inst[i].grid.NX - 1, ncy = inst[i].grid.NY - 1;
+    hcell[i + 1] = hcell[i] + ncx * ncy;
+    hcell[n + 1 + i + 1] = hcell[n + 1 + i] + ((ncx + 7) / 8) * ((ncy + 7) / 8);
+  }
+  E->ntile = hcell[2 * n + 1];
+  std::vector<int32_t> htile((size_t)E->ntile);
+  for (int i = 0; i < n; i++)
+    for (int64_t t = hcell[n + 1 + i]; t < hcell[n + 1 + i + 1]; t++) htile[(size_t)t] = i;
+  ENS_ALLOC(ctx, E->dtile_inst, sizeof(int32_t) * E->ntile);
+  FVM_HIP(ctx, hipMemcpyAsync(E->dtile_inst, htile.data(), sizeof(int32_t) * E->ntile, hipMemcpyHostToDevice,
+                              ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dcolsrc, hcolsrc.data(), sizeof(const double *) * E->ncol * n,
+                              hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dcell_off, hcell.data(), sizeof(int64_t) * 2 * (n + 1), hipMemcpyHostToDevice,
+                              ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->dgroups_sorted, hsorted.data(), sizeof(ProjGroup) * NG * n, hipMemcpyHostToDevice,
+                              ctx->stream));
   FVM_HIP(ctx, hipMemcpyAsync(E->dinst, E->hinst.data(), sizeof(InstDev) * n, hipMemcpyHostToDevice, ctx->stream));
   FVM_HIP(ctx, hipMemcpyAsync(E->dgroups, E->hgroups.data(), sizeof(ProjGroup) * NG * n, hipMemcpyHostToDevice,
                               ctx->stream));

@@ -230,6 +230,50 @@ __device__ __forceinline__ void gather_entry(const PayloadCols &pc, int with_xy,
   for (int k = 0; k < pc.nc; k++) pc.dst[k][p] = pc.src[k][v];
 }
 
+// Put the four buckets of one basic cell (global bucket index 4c..4c+3) in
+// ascending marker order and call emit(p, v) for every final position p and
+// its marker v.  One wavefront per cell: the (usually <= 64) entries are ranked
+// across lanes; a fuller cell falls back to an insertion sort per bucket whose
+// lane then visits its own bucket (reading back its own writes).
+template <class Emit>
+__device__ __forceinline__ void bucket_order_cell(int64_t c, const int32_t *__restrict__ start,
+                                                  int32_t *__restrict__ perm, Emit &&emit) {
+  const int lane = threadIdx.x & 63;
+  const int b0 = start[4 * c], b1 = start[4 * c + 1], b2 = start[4 * c + 2], b3 = start[4 * c + 3];
+  const int b4 = start[4 * c + 4];
+  const int n = b4 - b0;
+  if (n <= 64) {
+    const int pos = b0 + lane;
+    const bool have = lane < n;
+    const int v = have ? perm[pos] : 0x7fffffff;
+    const int bk = (pos >= b1) + (pos >= b2) + (pos >= b3);
+    int rank = 0;
+    for (int j = 0; j < n; j++) {
+      const int vj = read_lane(v, j);
+      const int bj = read_lane(bk, j);
+      rank += (bj == bk && vj < v) ? 1 : 0;
+    }
+    const int bs = bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3));
+    if (have) {
+      perm[bs + rank] = v;
+      emit(bs + rank, v);
+    }
+  } else if (lane < 4) {
+    const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));
+    const int s1 = lane == 0 ? b1 : (lane == 1 ? b2 : (lane == 2 ? b3 : b4));
+    for (int i = s0 + 1; i < s1; i++) {
+      const int32_t v = perm[i];
+      int j = i - 1;
+      while (j >= s0 && perm[j] > v) {
+        perm[j + 1] = perm[j];
+        j--;
+      }
+      perm[j + 1] = v;
+    }
+    for (int p = s0; p < s1; p++) emit(p, perm[p]);
+  }
+}
+
 struct SortGather {
   int with_xy;
   const double *X, *Y;

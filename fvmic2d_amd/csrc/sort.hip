@@ -227,18 +227,16 @@ __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__res
   if ((uint32_t)p < (uint32_t)n) perm[p] = (int32_t)m;  // always true for a valid scan
 }
 
-// Put every bucket in ascending marker order: one wavefront per basic cell
-// ranks the (usually <= 64) entries of the cell's four buckets; a cell with
-// more entries falls back to an insertion sort per bucket.  With GATHER the
-// payload columns of each entry are copied to its final position as well.
-// With GATHER the cells are visited in 8 x 8 tiles (y fastest inside a tile),
-// tiles spread over the XCDs in contiguous runs, so that the marker rows a
-// tile's wavefronts read share L2 lines.
+// Put every bucket in ascending marker order (bucket_order_cell, one
+// wavefront per basic cell).  With GATHER the payload columns of each entry
+// are copied to its final position as well, and the cells are visited in
+// 8 x 8 tiles (y fastest inside a tile), tiles spread over the XCDs in
+// contiguous runs, so that the marker rows a tile's wavefronts read share L2
+// lines.
 template <bool GATHER>
 __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, int32_t ncx, int32_t ncy,
                                                       const int32_t *__restrict__ start, int32_t *__restrict__ perm,
                                                       SortGather sg) {
-  const int lane = threadIdx.x & 63;
   int64_t c;
   if (GATHER) {
     const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -251,44 +249,9 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, int32_t ncx
     c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   }
   if (c >= ncell) return;  // whole wavefront exits together
-  const int b0 = start[4 * c], b1 = start[4 * c + 1], b2 = start[4 * c + 2], b3 = start[4 * c + 3];
-  const int b4 = start[4 * c + 4];
-  const int n = b4 - b0;
-  if (n <= 64) {
-    const int pos = b0 + lane;
-    const bool have = lane < n;
-    const int v = have ? perm[pos] : 0x7fffffff;
-    const int bk = (pos >= b1) + (pos >= b2) + (pos >= b3);
-    int rank = 0;
-    for (int j = 0; j < n; j++) {
-      const int vj = read_lane(v, j);
-      const int bj = read_lane(bk, j);
-      rank += (bj == bk && vj < v) ? 1 : 0;
-    }
-    const int bs = bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3));
-    if (have) {
-      const int p = bs + rank;
-      perm[p] = v;
-      if (GATHER) gather_entry(sg.pc, sg.with_xy, sg.X, sg.Y, sg.Mat, sg.xs, sg.ys, sg.ms, p, v);
-    }
-  } else if (lane < 4) {
-    const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));
-    const int s1 = lane == 0 ? b1 : (lane == 1 ? b2 : (lane == 2 ? b3 : b4));
-    for (int i = s0 + 1; i < s1; i++) {
-      const int32_t v = perm[i];
-      int j = i - 1;
-      while (j >= s0 && perm[j] > v) {
-        perm[j + 1] = perm[j];
-        j--;
-      }
-      perm[j + 1] = v;
-    }
-    if (GATHER) {  // this lane's own bucket, read back by the lane that wrote it
-      for (int p = s0; p < s1; p++) {
-        gather_entry(sg.pc, sg.with_xy, sg.X, sg.Y, sg.Mat, sg.xs, sg.ys, sg.ms, p, perm[p]);
-      }
-    }
-  }
+  bucket_order_cell(c, start, perm, [&](int p, int v) {
+    if (GATHER) gather_entry(sg.pc, sg.with_xy, sg.X, sg.Y, sg.Mat, sg.xs, sg.ys, sg.ms, p, v);
+  });
 }
 
 static int sort_buffers(fvm_ctx *ctx, const fvm_grid *g, int32_t n) {
