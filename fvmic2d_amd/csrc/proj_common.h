@@ -33,6 +33,31 @@ __host__ __device__ __forceinline__ Lat lattice(int stag, int N) {
 // (stag 0) or xc[l+1], xc[l+2] (stag +1); partials_cell_impl loads lo and hi
 // once per lattice cell.
 
+// ---- lane-group sums -------------------------------------------------------------
+// A DPP move of a double between lanes of the same row (both 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Butterfly step s of a G-lane group sum: lanes pair with lane ^ 1, then
+// lane ^ 2 (quad permutes), then the mirror lane inside 8 and inside 16
+// (DPP row_half_mirror / row_mirror), then lane ^ 16 (cross-row shuffle).
+// After step s every lane of a 2^(s+1)-lane block holds the same bits (IEEE
+// addition commutes), so each step adds two equal-pattern partial sums and
+// the group ends with one order-fixed total in every lane.
+template <int S>
+__device__ __forceinline__ double group_step(double v) {
+  if constexpr (S == 0) return v + dpp_f64<0xB1>(v);       // quad_perm [1,0,3,2]
+  else if constexpr (S == 1) return v + dpp_f64<0x4E>(v);  // quad_perm [2,3,0,1]
+  else if constexpr (S == 2) return v + dpp_f64<0x141>(v); // row_half_mirror
+  else if constexpr (S == 3) return v + dpp_f64<0x140>(v); // row_mirror
+  else return v + __shfl_xor(v, 16, 32);
+}
+
 // Accumulate one marker's contribution into the four corner sums.
 template <int METHOD, int NF>
 __device__ __forceinline__ void accumulate(double mdx, double mdy, const double (&f)[NF], double (&sw)[4],
@@ -269,15 +294,22 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
       accumulate<METHOD, NF>(mdx, mdy, f, sw, sf);
     }
   }
-  // fixed butterfly over the G lanes of the group (a + b == b + a exactly,
-  // so every lane ends with the same, order-fixed total)
+  // fixed butterfly over the G lanes of the group (group_step)
+  static_assert(G == 8 || G == 16 || G == 32, "lane groups of 8, 16 or 32");
 #pragma unroll
-  for (int off = G / 2; off > 0; off >>= 1) {
+  for (int c = 0; c < 4; c++) {
+    sw[c] = group_step<2>(group_step<1>(group_step<0>(sw[c])));
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      sw[c] += __shfl_xor(sw[c], off, G);
+    for (int k = 0; k < NF; k++) sf[c][k] = group_step<2>(group_step<1>(group_step<0>(sf[c][k])));
+    if constexpr (G >= 16) {
+      sw[c] = group_step<3>(sw[c]);
 #pragma unroll
-      for (int k = 0; k < NF; k++) sf[c][k] += __shfl_xor(sf[c][k], off, G);
+      for (int k = 0; k < NF; k++) sf[c][k] = group_step<3>(sf[c][k]);
+    }
+    if constexpr (G == 32) {
+      sw[c] = group_step<4>(sw[c]);
+#pragma unroll
+      for (int k = 0; k < NF; k++) sf[c][k] = group_step<4>(sf[c][k]);
     }
   }
   if (live && j == 0) {
