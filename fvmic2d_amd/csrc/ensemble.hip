@@ -675,6 +675,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
       const Lat Lx = lattice(pg.stagx, S.grid.NX), Ly = lattice(pg.stagy, S.grid.NY);
       q.part = E->part + partoff[(size_t)gi * n + i];
       q.nf = pg.nf;
+      q.pst = pg.nf + 1;
       q.harm = pg.method == PM_HARM;
       q.nLx = Lx.n;
       q.nLy = Ly.n;

@@ -375,6 +375,7 @@ __device__ __forceinline__ void partials_cell_subt_sorted(const DGrid &g, const 
 struct NodeGroup {
   const double *part;
   int nf, harm, nLx, nLy, lox, loy, epilogue;
+  int pst;  // partials per corner: stored fields + 1 (>= nf + 1)
   double *out[FVM_MAXF];
   const double *ea, *eb;
 };
@@ -388,7 +389,7 @@ struct NodeGroups {
 // Pass 2 for node (ix, jy) (flat index i), field k of one group.
 __device__ __forceinline__ void node_field(const NodeGroup &q, int ix, int jy, int64_t i, int k) {
   const int64_t nL = (int64_t)q.nLx * q.nLy;
-  const int st = q.nf + 1;
+  const int st = q.pst;
   // (lattice cell, corner) pairs touching node (ix, jy), in a fixed order:
   // (ix-1, jy-1) corner 3, (ix, jy-1) corner 2, (ix-1, jy) corner 1, (ix, jy) corner 0
   const int ax0 = ix - 1 - q.lox, ax1 = ix - q.lox, ay0 = jy - 1 - q.loy, ay1 = jy - q.loy;

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const i
 // every group whose lattice contains it.  Neighbouring families' cells at one
 // coordinate cover nearly the same markers, so the later families read them
 // from cache.
-#define MAX_MULTI 3
+#define MAX_MULTI 4
 struct MultiGroups {
   int ng;
   ProjGroup pg[MAX_MULTI];
@@ -52,9 +52,12 @@ template <int METHOD, int NF, int G>
 __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg, const int32_t *__restrict__ start,
                                                         const double *__restrict__ xs, const double *__restrict__ ys,
                                                         const int32_t *__restrict__ ms) {
-  // blockIdx.y selects the group, so the groups' cells are in flight together
-  const int q = blockIdx.y;
-  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  // consecutive blocks (one XCD, after xcd_tile) take the same range of cells
+  // for each group in turn, so every group after the first reads the markers
+  // from L2
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  const int q = (int)(b % mg.ng);
+  const int64_t tile = b / mg.ng;
   const int64_t t = (tile * blockDim.x + threadIdx.x) / G;
   const int j = threadIdx.x % G;
   const int64_t nT = (int64_t)(g.NX + 1) * (g.NY + 1);
@@ -134,7 +137,7 @@ static int launch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, cons
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
   const int G = ctx->lanes;
 #define LAUNCH_M(GG)                                                                                     \
-  k_partials_multi<METHOD, NF, GG><<<dim3((unsigned)fvm_blocks(nT * GG, 256), (unsigned)mg.ng), 256, 0,         \
+  k_partials_multi<METHOD, NF, GG><<<(unsigned)fvm_blocks(nT * GG, 256) * (unsigned)mg.ng, 256, 0,              \
                                      ctx->stream>>>(                                                       \
       dgrid(g), mg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms)
   if (G == 8) LAUNCH_M(8);
@@ -227,9 +230,29 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
       return fvm_set_err(ctx, FVM_ERR_BAD_STAG, "invalid stagger (%d,%d)", pg.stagx, pg.stagy);
     if (pg.nf < 1 || pg.nf > FVM_MAXF)
       return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "projection group with %d fields", pg.nf);
-    const Lat Lx = lattice(pg.stagx, g->NX), Ly = lattice(pg.stagy, g->NY);
+  }
+  // ARITHMETIC groups one field short of the largest (<= 5) get a zero field,
+  // so that all of them share one launch (blocks of one cell range take every
+  // group in turn and read the markers once from HBM); the padded group's
+  // partials carry the extra field, which no node pass reads
+  std::vector<ProjGroup> sg(pgs, pgs + ng);
+  if (!sub) {
+    int nmax = 0;
+    for (int i = 0; i < ng; i++)
+      if (sg[i].method == PM_ARITH && sg[i].nf <= 5) nmax = sg[i].nf > nmax ? sg[i].nf : nmax;
+    for (int i = 0; i < ng; i++)
+      if (sg[i].method == PM_ARITH && sg[i].nf == nmax - 1) {
+        ProjSrc &z = sg[i].src[sg[i].nf];
+        memset(&z, 0, sizeof(z));
+        z.use_mat = 1;  // material table of zeros: no load, adds 0
+        sg[i].out[sg[i].nf] = nullptr;
+        sg[i].nf++;
+      }
+  }
+  for (int i = 0; i < ng; i++) {
+    const Lat Lx = lattice(sg[i].stagx, g->NX), Ly = lattice(sg[i].stagy, g->NY);
     off[i] = total;
-    total += (size_t)Lx.n * Ly.n * 4 * (pg.nf + 1);
+    total += (size_t)Lx.n * Ly.n * 4 * (sg[i].nf + 1);
   }
   FVM_TRY(fvm_ws(ctx, ctx->partials, total * sizeof(double)));
   double *part = (double *)ctx->partials.ptr;
@@ -245,7 +268,6 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
     pc.src[pc.nc] = c;
     return pc.nc++;
   };
-  std::vector<ProjGroup> sg(pgs, pgs + ng);
   std::vector<std::pair<int, int>> uses;  // (group * FVM_MAXF + field, slot)
   for (int i = 0; i < ng; i++)
     for (int k = 0; k < sg[i].nf; k++)
@@ -338,6 +360,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
       NodeGroup &q = G.q[G.ng++];
       q.part = part + off[i];
       q.nf = pg.nf;
+      q.pst = sg[i].nf + 1;
       q.harm = pg.method == PM_HARM;
       q.nLx = Lx.n;
       q.nLy = Ly.n;
