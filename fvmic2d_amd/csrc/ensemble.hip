@@ -43,6 +43,7 @@ struct fvm_ensemble {
   std::vector<InstDev> hinst;
   std::vector<ProjGroup> hgroups;   // [group][instance]; group ng is the dTsub projection
   std::vector<int> gstag_x, gstag_y, gmethod, gnf;
+  std::vector<int> gsnf;  // fields stored in a group's partials (gnf, or gnf + 1 when padded)
   int64_t nmark = 0, nbucket = 0, nnode = 0, ncell = 0;
   // device memory
   InstDev *dinst = nullptr;
@@ -187,33 +188,40 @@ struct KbGroups {
   int gi[KB_MAX_MULTI], sx[KB_MAX_MULTI], sy[KB_MAX_MULTI];
 };
 
+// Every group of one (method, stored field count) in one launch over the
+// universal lattice ((NX+1) x (NY+1) cell coordinates per instance, block row
+// `urow`): consecutive blocks take the same cell range for each group in turn
+// (group fastest, one XCD after xcd_tile), so the groups after the first read
+// the payload from L2.
 template <int METHOD, int NF, int G>
-__global__ void __launch_bounds__(512) kb_partials(const InstDev *__restrict__ insts, int n,
-                                                   const int64_t *__restrict__ blk_rows,
-                                                   const int32_t *__restrict__ bi_all,
-                                                   const int64_t *__restrict__ bi_off,
+__global__ void __launch_bounds__(256) kb_partials(const InstDev *__restrict__ insts, int n,
+                                                   const int64_t *__restrict__ ublk, const int32_t *__restrict__ ubi,
                                                    const ProjGroup *__restrict__ all_groups,
                                                    const int64_t *__restrict__ all_partoff,
                                                    const int32_t *__restrict__ start, const double *__restrict__ xs,
                                                    const double *__restrict__ ys, const int32_t *__restrict__ ms,
-                                                   double *__restrict__ part, KbGroups kg) {
-  const int gi = kg.gi[blockIdx.y], sx = kg.sx[blockIdx.y], sy = kg.sy[blockIdx.y];
-  const int64_t *blk = blk_rows + (int64_t)(2 + gi) * (n + 1);
-  if ((int64_t)blockIdx.x >= blk[n]) return;
-  const ProjGroup *groups = all_groups + (int64_t)gi * n;
-  const int64_t *partoff = all_partoff + (int64_t)gi * n;
-  const int ii = bi_all[bi_off[2 + gi] + blockIdx.x];
+                                                   double *__restrict__ part, KbGroups kg, int nrow) {
+  const int64_t b = xcd_tile(blockIdx.x, gridDim.x);
+  const int q = (int)(b % nrow);
+  const int64_t ub = b / nrow;
+  if (ub >= ublk[n]) return;
+  const int gi = kg.gi[q], sx = kg.sx[q], sy = kg.sy[q];
+  const int ii = ubi[ub];
   const InstDev &I = insts[ii];
-  const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
-  const int64_t nL = (int64_t)Lx.n * Ly.n;
-  const int64_t lt = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t lt = (ub - ublk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   const int64_t t = lt / G;
   const int j = (int)(lt % G);
-  const bool live = t < nL;
-  const int lx = live ? (int)(t % Lx.n) + Lx.lo : 0;
-  const int ly = live ? (int)(t / Lx.n) + Ly.lo : 0;
-  partials_cell_sorted<METHOD, NF, G>(I.g, groups[ii], start + I.bucket_off, xs, ys, ms, part + partoff[ii], nL, t,
-                                      lx, ly, j, live);
+  const int64_t nT = (int64_t)(I.g.NX + 1) * (I.g.NY + 1);
+  const bool inside = t < nT;
+  const int cx = inside ? (int)(t % (I.g.NX + 1)) - 1 : -2;
+  const int cy = inside ? (int)(t / (I.g.NX + 1)) - 1 : -2;
+  const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
+  const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
+  const int64_t nL = (int64_t)Lx.n * Ly.n;
+  const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
+  partials_cell_sorted<METHOD, NF, G>(I.g, all_groups[(int64_t)gi * n + ii], start + I.bucket_off, xs, ys, ms,
+                                      part + all_partoff[(int64_t)gi * n + ii], nL, tl, live ? cx : 0,
+                                      live ? cy : 0, j, live);
 }
 
 // (group, field) pairs of one node launch: launch row blockIdx.y = pair
@@ -294,12 +302,16 @@ __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ ins
 }
 
 // ---- host side ----------------------------------------------------------------------------
-// block offsets rows: 0 markers (tune.block), 1 nodes (256), 2 + g lattice cells of group g
+// block offsets rows: 0 markers (tune.block), 1 nodes (256), 2 + g lattice cells of
+// group g, then the universal lattice ((NX+1) x (NY+1) cells) of the shared
+// partials launches
 static int64_t *blk_row(fvm_ensemble *E, int r) { return E->dblk + (int64_t)r * (E->n + 1); }
 static const int32_t *bi_row(fvm_ensemble *E, int r) { return E->dbi + E->hbi_off[r]; }
 
+static int urow(const fvm_ensemble *E) { return 2 + E->ng + 1; }
+
 static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
-  const int n = E->n, rows = 2 + E->ng + 1;
+  const int n = E->n, rows = urow(E) + 1;
   E->hblk.assign((size_t)rows * (n + 1), 0);
   const int B = E->tune.block, G = E->tune.lanes;
   for (int i = 0; i < n; i++) {
@@ -313,6 +325,8 @@ static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
       const int r = 2 + gi;
       E->hblk[r * (n + 1) + i + 1] = E->hblk[r * (n + 1) + i] + (th + 255) / 256;
     }
+    const int64_t tu = (int64_t)(I.g.NX + 1) * (I.g.NY + 1) * G, u = urow(E);
+    E->hblk[u * (n + 1) + i + 1] = E->hblk[u * (n + 1) + i] + (tu + 255) / 256;
   }
   E->hbi_off.assign(rows + 1, 0);
   for (int r = 0; r < rows; r++) E->hbi_off[r + 1] = E->hbi_off[r] + E->hblk[(size_t)r * (n + 1) + n];
@@ -342,27 +356,25 @@ static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
 
 template <int METHOD, int NF>
 static int launch_kb_partials(fvm_ctx *ctx, fvm_ensemble *E, const KbGroups &kg, int nrow) {
-  const int n = E->n;
-  int64_t nblk = 0;
-  for (int r = 0; r < nrow; r++) nblk = std::max(nblk, E->hblk[(2 + kg.gi[r]) * (n + 1) + n]);
+  const int n = E->n, u = urow(E);
+  const int64_t nblk = E->hblk[(size_t)u * (n + 1) + n];
   if (nblk == 0) return FVM_OK;
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
-  const dim3 grid((unsigned)nblk, (unsigned)nrow);
+  const unsigned grid = (unsigned)(nblk * nrow);
+  const int64_t *ub = blk_row(E, u);
+  const int32_t *ubi = bi_row(E, u);
   switch (E->tune.lanes) {
     case 8:
-      kb_partials<METHOD, NF, 8><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, E->dblk, E->dbi, E->dbi_off,
-                                                                E->dgroups_sorted, E->dpartoff,
-                                                                E->start, E->xs, E->ys, E->ms, E->part, kg);
+      kb_partials<METHOD, NF, 8><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, ub, ubi, E->dgroups_sorted, E->dpartoff,
+                                                                E->start, E->xs, E->ys, E->ms, E->part, kg, nrow);
       break;
     case 32:
-      kb_partials<METHOD, NF, 32><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, E->dblk, E->dbi, E->dbi_off,
-                                                                E->dgroups_sorted, E->dpartoff,
-                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg);
+      kb_partials<METHOD, NF, 32><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, ub, ubi, E->dgroups_sorted, E->dpartoff,
+                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg, nrow);
       break;
     default:
-      kb_partials<METHOD, NF, 16><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, E->dblk, E->dbi, E->dbi_off,
-                                                                E->dgroups_sorted, E->dpartoff,
-                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg);
+      kb_partials<METHOD, NF, 16><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, ub, ubi, E->dgroups_sorted, E->dpartoff,
+                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg, nrow);
   }
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
@@ -392,15 +404,15 @@ static int kb_partials_all(fvm_ctx *ctx, fvm_ensemble *E) {
     memset(&kg, 0, sizeof(kg));
     int nrow = 0;
     for (int g = g0; g < E->ng && nrow < KB_MAX_MULTI; g++) {
-      if (done[g] || E->gmethod[g] != E->gmethod[g0] || E->gnf[g] != E->gnf[g0]) continue;
+      if (done[g] || E->gmethod[g] != E->gmethod[g0] || E->gsnf[g] != E->gsnf[g0]) continue;
       kg.gi[nrow] = g;
       kg.sx[nrow] = E->gstag_x[g];
       kg.sy[nrow] = E->gstag_y[g];
       nrow++;
       done[g] = true;
     }
-    if (E->gmethod[g0] == PM_ARITH) FVM_TRY(kb_partials_nf<PM_ARITH>(ctx, E, E->gnf[g0], kg, nrow));
-    else if (E->gmethod[g0] == PM_LOCAL) FVM_TRY(kb_partials_nf<PM_LOCAL>(ctx, E, E->gnf[g0], kg, nrow));
+    if (E->gmethod[g0] == PM_ARITH) FVM_TRY(kb_partials_nf<PM_ARITH>(ctx, E, E->gsnf[g0], kg, nrow));
+    else if (E->gmethod[g0] == PM_LOCAL) FVM_TRY(kb_partials_nf<PM_LOCAL>(ctx, E, E->gsnf[g0], kg, nrow));
     else return fvm_set_err(ctx, FVM_ERR_BAD_METHOD, "ensemble projections support ARITHMETIC and ARITHMETIC_LOCAL");
   }
   return FVM_OK;
@@ -572,6 +584,16 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
   E->gmethod.push_back(PM_ARITH);
   E->gnf.push_back(1);
   const int NG = E->ng + 1;
+  // ARITHMETIC groups one field short of the largest (<= 5) store a zero field
+  // so that they share the largest group's launch
+  E->gsnf = E->gnf;
+  {
+    int nmax = 0;
+    for (int gi = 0; gi < E->ng; gi++)
+      if (E->gmethod[gi] == PM_ARITH && E->gnf[gi] <= 5) nmax = std::max(nmax, E->gnf[gi]);
+    for (int gi = 0; gi < E->ng; gi++)
+      if (E->gmethod[gi] == PM_ARITH && E->gnf[gi] == nmax - 1) E->gsnf[gi] = nmax;
+  }
 
   // per-instance device views and offsets
   E->hinst.resize(n);
@@ -611,7 +633,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
     for (int gi = 0; gi < NG; gi++) {
       const Lat Lx = lattice(E->gstag_x[gi], S.grid.NX), Ly = lattice(E->gstag_y[gi], S.grid.NY);
       partoff[(size_t)gi * n + i] = parttot;
-      parttot += (int64_t)Lx.n * Ly.n * 4 * (E->gnf[gi] + 1);
+      parttot += (int64_t)Lx.n * Ly.n * 4 * (E->gsnf[gi] + 1);
     }
   }
   if (max_lds > 64 * 1024) {
@@ -623,7 +645,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
   ENS_ALLOC(ctx, E->dinst, sizeof(InstDev) * n);
   ENS_ALLOC(ctx, E->dgroups, sizeof(ProjGroup) * NG * n);
   ENS_ALLOC(ctx, E->dnodes, sizeof(NodeGroup) * NG * n);
-  ENS_ALLOC(ctx, E->dblk, sizeof(int64_t) * (2 + NG) * (n + 1));
+  ENS_ALLOC(ctx, E->dblk, sizeof(int64_t) * (3 + NG) * (n + 1));
   ENS_ALLOC(ctx, E->dpartoff, sizeof(int64_t) * NG * n);
   ENS_ALLOC(ctx, E->key, sizeof(int32_t) * E->nmark);
   ENS_ALLOC(ctx, E->slot, sizeof(int32_t) * E->nmark);
@@ -647,7 +669,8 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
       pg.stagx = E->gstag_x[gi];
       pg.stagy = E->gstag_y[gi];
       pg.method = E->gmethod[gi];
-      pg.nf = E->gnf[gi];
+      pg.nf = E->gsnf[gi];
+      for (int k = E->gnf[gi]; k < E->gsnf[gi]; k++) pg.src[k].use_mat = 1;  // zero field (padding)
       pg.epilogue = EPI_DIVIDE;
     }
     for (int p = 0; p < I0.nproj; p++) {
@@ -674,15 +697,15 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
       memset(&q, 0, sizeof(q));
       const Lat Lx = lattice(pg.stagx, S.grid.NX), Ly = lattice(pg.stagy, S.grid.NY);
       q.part = E->part + partoff[(size_t)gi * n + i];
-      q.nf = pg.nf;
-      q.pst = pg.nf + 1;
+      q.nf = E->gnf[gi];
+      q.pst = E->gsnf[gi] + 1;
       q.harm = pg.method == PM_HARM;
       q.nLx = Lx.n;
       q.nLy = Ly.n;
       q.lox = Lx.lo;
       q.loy = Ly.lo;
       q.epilogue = pg.epilogue;
-      for (int k = 0; k < pg.nf; k++) q.out[k] = pg.out[k];
+      for (int k = 0; k < q.nf; k++) q.out[k] = pg.out[k];
       q.ea = pg.ea;
       q.eb = pg.eb;
     }
