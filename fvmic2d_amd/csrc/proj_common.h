@@ -235,7 +235,7 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
   // Lane j takes entries j, j+G, j+2G, ... of the concatenated ranges and adds
   // them in that order.  U entries are fetched together (their loads in
   // flight at once) before they are added.
-  constexpr int U = (SUBT || NF > 3) ? 2 : 4;
+  constexpr int U = (NF > 3 && !SUBT) ? 2 : 4;
   constexpr int NV = SUBT ? 1 : NF;  // loaded values per marker
   for (int e0 = j; e0 < tot; e0 += U * G) {
     int mm[U];  // marker index (SORTED: bucket position) of entry e0 + u*G, -1 past the end
