@@ -337,6 +337,21 @@ def test_step_vep(vep):
         assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
 
 
+def test_step_vep_wide_grid():
+    """A 3100 x 12 graded grid: its coordinates do not fit the LDS staging of the
+    location (48 KB) or advection (32 KB) kernels, which then read them from
+    global memory; strongly graded cells take the bisection fallback."""
+    w = workloads.vep_subduction(NX=3100, NY=12, NM=2)
+    st, mp = _pair(w)
+    y_r, r_r = st.step_vep()
+    y_g, r_g = mp.step_vep()
+    assert y_r == y_g
+    assert abs(r_g - r_r) <= RTOL * abs(r_r)
+    _cmp_markers(st, mp, exact=("X", "Y", "VX", "VY", "wxy"))
+    for n in oracle.VEP_PROJ + ["dsxx", "dsyy", "dsxy"]:
+        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
+
+
 def test_step_is_deterministic():
     w = workloads.convection_tdep(N=101, NM=5)
     a = MarkerPath(w)
