@@ -352,6 +352,28 @@ def test_step_vep_wide_grid():
         assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
 
 
+@pytest.mark.timeout(600)
+def test_step_vep_full_size():
+    """The visco-elasto-plastic step at full size: 801 x 401 tanh-stretched grid,
+    25 markers per cell (8M markers), five materials with Drucker-Prager
+    yielding, against the oracle."""
+    import time
+    t0 = time.time()
+    w = workloads.vep_subduction()
+    assert (w.NX, w.NY, w.n_markers) == (801, 401, 8_000_000)
+    st, mp = _pair(w)
+    print("  workload and both states in %.1f s" % (time.time() - t0), flush=True)
+    y_r, r_r = st.step_vep()
+    y_g, r_g = mp.step_vep()
+    print("  oracle and GPU steps done at %.1f s" % (time.time() - t0), flush=True)
+    assert y_r == y_g
+    assert abs(r_g - r_r) <= RTOL * abs(r_r)
+    _cmp_markers(st, mp, exact=("X", "Y", "VX", "VY", "wxy"))
+    for n in oracle.VEP_PROJ + ["dsxx", "dsyy", "dsxy"]:
+        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
+    print("  compared at %.1f s" % (time.time() - t0), flush=True)
+
+
 def test_step_is_deterministic():
     w = workloads.convection_tdep(N=101, NM=5)
     a = MarkerPath(w)
