@@ -72,6 +72,32 @@ def test_ensemble_equals_single_instance():
     ens.close()
 
 
+@pytest.mark.timeout(600)
+def test_ensemble_full_size_equals_single_instance():
+    """The full 64-instance ensemble (grids 65..257 nodes a side, 16-36 markers
+    per cell, ~47M markers), graph-replayed: every instance bit-identical to
+    the same instance stepped alone."""
+    import time
+    t0 = time.time()
+    print("CASE ensemble full size, 64 instances", flush=True)
+    ctx = Context()
+    ws = workloads.ensemble(64)
+    alone = _paths(ws, ctx)
+    batch = _paths(ws, ctx)
+    print("  %d markers generated and uploaded twice in %.1f s" % (sum(w.n_markers for w in ws), time.time() - t0),
+          flush=True)
+    ens = Ensemble(batch)
+    ens.capture()
+    for p in alone:
+        p.step_thermal()
+    ens.launch()
+    ctx.sync()
+    for i, (a, b) in enumerate(zip(alone, batch)):
+        _assert_same(a, b, "instance %d" % i)
+    print("  compared at %.1f s" % (time.time() - t0), flush=True)
+    ens.close()
+
+
 @pytest.mark.parametrize("block,lanes", [(128, 8), (256, 16), (512, 32), (128, 32), (512, 8)])
 def test_ensemble_vs_oracle(block, lanes):
     print("CASE ensemble vs oracle block=%d lanes=%d" % (block, lanes), flush=True)
