@@ -167,8 +167,9 @@ __global__ void __launch_bounds__(256) kb_order_gather(const InstDev *__restrict
   if (tile >= tile_off[n]) return;  // whole wavefront exits together
   const int ii = tile_inst[tile];
   const InstDev &I = insts[ii];
-  const int ncx = I.g.NX - 1, ncy = I.g.NY - 1, ntx = (ncx + 7) >> 3;
-  const int64_t tl = tile - tile_off[ii];
+  const int ncx = I.g.NX - 1, ncy = I.g.NY - 1;
+  const uint32_t ntx = (uint32_t)(ncx + 7) >> 3;
+  const uint32_t tl = (uint32_t)(tile - tile_off[ii]);
   const int l = (int)(w & 63);
   const int cx = (int)(tl % ntx) * 8 + (l >> 3), cy = (int)(tl / ntx) * 8 + (l & 7);
   if (cx >= ncx || cy >= ncy) return;

@@ -239,24 +239,44 @@ template <class Emit>
 __device__ __forceinline__ void bucket_order_cell(int64_t c, const int32_t *__restrict__ start,
                                                   int32_t *__restrict__ perm, Emit &&emit) {
   const int lane = threadIdx.x & 63;
+  // c is uniform across the wavefront: make that explicit so that the bucket
+  // starts and the loop bounds below live in scalar registers
+  c = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(c >> 32)) << 32) |
+                (uint32_t)__builtin_amdgcn_readfirstlane((int)c));
   const int b0 = start[4 * c], b1 = start[4 * c + 1], b2 = start[4 * c + 2], b3 = start[4 * c + 3];
   const int b4 = start[4 * c + 4];
-  const int n = b4 - b0;
+  const int n = __builtin_amdgcn_readfirstlane(b4 - b0);
   if (n <= 64) {
     const int pos = b0 + lane;
     const bool have = lane < n;
     const int v = have ? perm[pos] : 0x7fffffff;
     const int bk = (pos >= b1) + (pos >= b2) + (pos >= b3);
-    int rank = 0;
-    for (int j = 0; j < n; j++) {
-      const int vj = read_lane(v, j);
-      const int bj = read_lane(bk, j);
-      rank += (bj == bk && vj < v) ? 1 : 0;
+    int p;
+    if (__ballot(have && v >= (1 << 30)) == 0) {
+      // (bucket, marker) packed in 32 bits: the final position is the cell's
+      // start plus the number of the cell's entries with a smaller pair
+      const uint32_t key = have ? ((uint32_t)bk << 30) | (uint32_t)v : 0xffffffffu;
+      // lanes >= n hold the largest key, so the count may run to a multiple of 4
+      int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+      for (int j = 0; j < n; j += 4) {
+        r0 += (uint32_t)read_lane((int)key, j) < key ? 1 : 0;
+        r1 += (uint32_t)read_lane((int)key, j + 1) < key ? 1 : 0;
+        r2 += (uint32_t)read_lane((int)key, j + 2) < key ? 1 : 0;
+        r3 += (uint32_t)read_lane((int)key, j + 3) < key ? 1 : 0;
+      }
+      p = b0 + ((r0 + r1) + (r2 + r3));
+    } else {
+      int rank = 0;
+      for (int j = 0; j < n; j++) {
+        const int vj = read_lane(v, j);
+        const int bj = read_lane(bk, j);
+        rank += (bj == bk && vj < v) ? 1 : 0;
+      }
+      p = (bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3))) + rank;
     }
-    const int bs = bk == 0 ? b0 : (bk == 1 ? b1 : (bk == 2 ? b2 : b3));
     if (have) {
-      perm[bs + rank] = v;
-      emit(bs + rank, v);
+      perm[p] = v;
+      emit(p, v);
     }
   } else if (lane < 4) {
     const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));

@@ -240,8 +240,8 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, int32_t ncx
   int64_t c;
   if (GATHER) {
     const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const int ntx = (ncx + 7) >> 3;
-    const int64_t tile = w >> 6;
+    const uint32_t ntx = (uint32_t)(ncx + 7) >> 3;
+    const uint32_t tile = (uint32_t)(w >> 6);
     const int l = (int)(w & 63);
     const int cx = (int)(tile % ntx) * 8 + (l >> 3), cy = (int)(tile / ntx) * 8 + (l & 7);
     c = (cx < ncx && cy < ncy) ? (int64_t)cy * ncx + cx : ncell;
