@@ -387,6 +387,45 @@ struct NodeGroups {
 };
 
 // Pass 2 for node (ix, jy) (flat index i), field k of one group.
+// Pass 2 for node (ix, jy), every field of one group: the corner weights are
+// read once; each field's result is the one node_field gives.
+__device__ __forceinline__ void node_group(const NodeGroup &q, int ix, int jy, int64_t i) {
+  const int64_t nL = (int64_t)q.nLx * q.nLy;
+  const int st = q.pst;
+  const int ax0 = ix - 1 - q.lox, ax1 = ix - q.lox, ay0 = jy - 1 - q.loy, ay1 = jy - q.loy;
+  const bool vx0 = ax0 >= 0 && ax0 < q.nLx, vx1 = ax1 >= 0 && ax1 < q.nLx;
+  const bool vy0 = ay0 >= 0 && ay0 < q.nLy, vy1 = ay1 >= 0 && ay1 < q.nLy;
+  const bool v3 = vx0 && vy0, v2 = vx1 && vy0, v1 = vx0 && vy1, v0 = vx1 && vy1;
+  const int64_t L3 = v3 ? (int64_t)ay0 * q.nLx + ax0 : 0, L2 = v2 ? (int64_t)ay0 * q.nLx + ax1 : 0;
+  const int64_t L1 = v1 ? (int64_t)ay1 * q.nLx + ax0 : 0, L0 = v0 ? (int64_t)ay1 * q.nLx + ax1 : 0;
+  const double *P = q.part;
+  const double w3 = P[(int64_t)(3 * st) * nL + L3], w2 = P[(int64_t)(2 * st) * nL + L2];
+  const double w1 = P[(int64_t)(1 * st) * nL + L1], w0 = P[L0];
+  double den = 0.0;
+  if (v3) den += w3;
+  if (v2) den += w2;
+  if (v1) den += w1;
+  if (v0) den += w0;
+  den = 0.0 + den;
+  double ea = 0.0, eb = 0.0;
+  if (q.epilogue == EPI_SUBGRID_T) ea = q.ea[i], eb = q.eb[i];
+  for (int k = 0; k < q.nf; k++) {
+    const double f3 = P[(int64_t)(3 * st + 1 + k) * nL + L3], f2 = P[(int64_t)(2 * st + 1 + k) * nL + L2];
+    const double f1 = P[(int64_t)(1 * st + 1 + k) * nL + L1], f0 = P[(int64_t)(1 + k) * nL + L0];
+    double num = 0.0;
+    if (v3) num += f3;
+    if (v2) num += f2;
+    if (v1) num += f1;
+    if (v0) num += f0;
+    num = 0.0 + num;
+    const double r = q.harm ? den / num : num / den;
+    double *o = q.out[k];
+    if (q.epilogue == EPI_DIVIDE) o[i] = r;
+    else if (q.epilogue == EPI_SUBGRID_T) o[i] = ea + -1.0 * eb + -1.0 * r;
+    else o[i] = o[i] + -1.0 * r;
+  }
+}
+
 __device__ __forceinline__ void node_field(const NodeGroup &q, int ix, int jy, int64_t i, int k) {
   const int64_t nL = (int64_t)q.nLx * q.nLy;
   const int st = q.pst;

@@ -99,13 +99,12 @@ __global__ void __launch_bounds__(256) k_gather_payload(int32_t n, const int32_t
   gather_entry(pc, with_xy, X, Y, Mat, xs, ys, ms, p, perm[p]);
 }
 
-// one thread per node; launch row blockIdx.y = (group, field) pair
+// one thread per node; launch row blockIdx.y = group (all its fields)
 __global__ void __launch_bounds__(256) k_nodes(int NX, int NY, NodeGroups G) {
   const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= (int64_t)NX * NY) return;
   const int ix = (int)(i % NX), jy = (int)(i / NX);
-  const int p = blockIdx.y;
-  node_field(G.q[G.pg[p]], ix, jy, i, G.pk[p]);
+  node_group(G.q[blockIdx.y], ix, jy, i);
 }
 
 struct Payload {
@@ -377,7 +376,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
       q.eb = pg.eb;
     }
     FVM_PHASE(ctx, FVM_PHASE_NODES);
-    k_nodes<<<dim3(fvm_blocks(nn, 256), G.npair), 256, 0, ctx->stream>>>(g->NX, g->NY, G);
+    k_nodes<<<dim3(fvm_blocks(nn, 256), G.ng), 256, 0, ctx->stream>>>(g->NX, g->NY, G);
     FVM_LAUNCH_CHECK(ctx);
   }
   return FVM_OK;
