@@ -225,21 +225,16 @@ __global__ void __launch_bounds__(256) kb_partials(const InstDev *__restrict__ i
                                       live ? cy : 0, j, live);
 }
 
-// (group, field) pairs of one node launch: launch row blockIdx.y = pair
-struct PairTab {
-  unsigned char g[64], k[64];
-};
-
+// node pass: launch row blockIdx.y = group g0 + blockIdx.y, every field of it
 __global__ void __launch_bounds__(256) kb_nodes(const InstDev *__restrict__ insts, int n,
                                                 const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
-                                                const NodeGroup *__restrict__ nodes, PairTab pt) {
+                                                const NodeGroup *__restrict__ nodes, int g0) {
   const int ii = bi[blockIdx.x];
   const InstDev &I = insts[ii];
   const int64_t i = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (i >= (int64_t)I.g.NX * I.g.NY) return;
   const int ix = (int)(i % I.g.NX), jy = (int)(i / I.g.NX);
-  const int p = blockIdx.y;
-  node_field(nodes[(int64_t)pt.g[p] * n + ii], ix, jy, i, pt.k[p]);
+  node_group(nodes[(int64_t)(g0 + blockIdx.y) * n + ii], ix, jy, i);
 }
 
 template <int G>
@@ -449,20 +444,10 @@ static int launch_kb_partials_subt(fvm_ctx *ctx, fvm_ensemble *E) {
 static int launch_kb_nodes(fvm_ctx *ctx, fvm_ensemble *E, int g0, int g1) {
   const int n = E->n;
   const int64_t nblk = E->hblk[1 * (n + 1) + n];
-  if (nblk == 0) return FVM_OK;
-  PairTab pt;
-  int np = 0;
-  for (int gi = g0; gi < g1; gi++)
-    for (int k = 0; k < E->gnf[gi]; k++) {
-      if (np == 64) return fvm_set_err(ctx, FVM_ERR_TOO_MANY_FIELDS, "ensemble: more than 64 projected fields");
-      pt.g[np] = (unsigned char)gi;
-      pt.k[np] = (unsigned char)k;
-      np++;
-    }
-  if (np == 0) return FVM_OK;
+  if (nblk == 0 || g1 <= g0) return FVM_OK;
   FVM_PHASE(ctx, FVM_PHASE_NODES);
-  kb_nodes<<<dim3((unsigned)nblk, np), 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1), bi_row(E, 1), E->dnodes,
-                                                              pt);
+  kb_nodes<<<dim3((unsigned)nblk, (unsigned)(g1 - g0)), 256, 0, ctx->stream>>>(E->dinst, n, blk_row(E, 1),
+                                                                               bi_row(E, 1), E->dnodes, g0);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }

@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *
 __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restrict__ in, int32_t *__restrict__ out,
                                                           unsigned long long *__restrict__ state, int64_t n,
                                                           int32_t *__restrict__ fail) {
-  __shared__ int32_t s[SCAN_BS];
+  __shared__ int32_t s_wave[SCAN_BS / 64];
   __shared__ int32_t s_tile, s_prefix;
   if (threadIdx.x == 0) s_tile = (int32_t)atomicAdd(&state[0], 1ull);
   __syncthreads();
@@ -143,15 +143,24 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restr
     v[i] = (base + i < n) ? in[base + i] : 0;
     t += v[i];
   }
-  s[threadIdx.x] = t;
-  __syncthreads();
-  for (int off = 1; off < SCAN_BS; off <<= 1) {
-    int32_t a = (threadIdx.x >= (unsigned)off) ? s[threadIdx.x - off] : 0;
-    __syncthreads();
-    s[threadIdx.x] += a;
-    __syncthreads();
+  // inclusive scan of the thread sums: inside each wavefront by shuffles, then
+  // across the block's wavefronts through LDS
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int32_t incl = t;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
   }
-  const int32_t agg = s[SCAN_BS - 1];
+  if (lane == 63) s_wave[wv] = incl;
+  __syncthreads();
+  int32_t before = 0, agg = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_BS / 64; k++) {
+    const int32_t wt = s_wave[k];
+    before += k < wv ? wt : 0;
+    agg += wt;
+  }
   unsigned long long *word = state + 2;
   if (threadIdx.x == 0)
     __hip_atomic_store(&word[tile], ((tile == 0 ? 2ull : 1ull) << 32) | (uint32_t)agg, __ATOMIC_RELEASE,
@@ -196,7 +205,7 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restr
     s_prefix = 0;
   }
   __syncthreads();
-  int32_t run = s_prefix + s[threadIdx.x] - t;  // exclusive prefix of this thread
+  int32_t run = s_prefix + before + incl - t;  // exclusive prefix of this thread
 #pragma unroll
   for (int i = 0; i < SCAN_IPT; i++) {
     if (base + i < n) out[base + i] = run;
