@@ -350,7 +350,6 @@ int fvm_launch_zero_sort(fvm_ctx *ctx, int32_t *count, int64_t ncount, int32_t *
 int fvm_launch_scan_onepass(fvm_ctx *ctx, const int32_t *count, int32_t *start, unsigned long long *state,
                             int64_t nk);
 int64_t fvm_scan_tiles(int64_t nk);
-int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm);
 __device__ __forceinline__ int bisect_grid(const double *__restrict__ x, int N, double X) {
   // findMarkerCell's bisection (markers.c:48-59): largest i in [0, N-2] with x[i] <= X
   int lo = 0, hi = N - 1;

@@ -438,12 +438,3 @@ int fvm_launch_scan_onepass(fvm_ctx *ctx, const int32_t *count, int32_t *start, 
   return FVM_OK;
 }
 
-int fvm_launch_bucket_order(fvm_ctx *ctx, int64_t ncell, const int32_t *start, int32_t *perm) {
-  FVM_PHASE(ctx, FVM_PHASE_SORT);
-  SortGather none;
-  memset(&none, 0, sizeof(none));
-  k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, 0, 0, start, perm,
-                                                                              none);
-  FVM_LAUNCH_CHECK(ctx);
-  return FVM_OK;
-}
