@@ -5,9 +5,10 @@
 // "energy_" KSP.  Here the operator is applied matrix-free, row for row as
 // assembled there (dummy rows for ix == 0 or jy == 0, the pinned cell (3, 3),
 // zero-flux sides, dp/dy = rho g on top and bottom), and solved with
-// Jacobi-preconditioned BiCGStab whose scalars stay on the device.  Dot
-// products reduce in a fixed order (per-block trees, then one ordered pass),
-// so the iteration is bitwise reproducible.
+// Jacobi-preconditioned BiCGStab in five kernels per iteration whose scalars
+// stay on the device.  Dot products reduce in a fixed order (per-block trees,
+// then the same ordered sum in every block of the next kernel), so the
+// iteration is bitwise reproducible.
 #include "fvm_internal.h"
 
 namespace {
@@ -96,128 +97,173 @@ __global__ void __launch_bounds__(256) k_pg_apply(PGrid g, const double *__restr
   y[i] = s;
 }
 
-// ---- deterministic dot products ---------------------------------------------
-#define DOT_BS 256
-#define DOT_ITEMS 8
+// ---- BiCGStab, five kernels per iteration -----------------------------------------
+// Every vector kernel runs one thread per row in blocks of 256 (block k holds
+// rows 256k ..).  Dot products are per-block partial sums (fixed trees) that
+// the next kernel reduces: every block of it sums the same partials in the same
+// order, so every thread holds the same scalar and the iteration is bitwise
+// reproducible.  The stencil coefficients are tabulated once.
+#define BCG_BS 256
 
-struct DotArgs {
-  int nd;
-  const double *a[2], *b[2];
+enum { S_RHO = 0, S_RHO_OLD, S_ALPHA, S_OMEGA, S_NSC };
+
+struct Coef {
+  const double *d, *r, *l, *dn, *up;
+};
+struct BcgVecs {
+  double *b, *dinv, *x, *r, *rh, *p, *v, *s, *t, *y, *z;
+  double *cd, *cr, *cl, *cdn, *cup;          // stencil coefficients
+  double *prv, *pts, *ptt, *prho, *prr;      // per-block partial sums
+  double *sc;                                // scalars (S_*)
 };
 
-// partial[blk * nd + k] = sum over block blk of a_k . b_k (k < nd)
-__global__ void __launch_bounds__(DOT_BS) k_dots(int64_t n, DotArgs d, double *__restrict__ partial) {
-  __shared__ double sh[DOT_BS];
-  for (int k = 0; k < d.nd; k++) {
-    const double *a = d.a[k], *b = d.b[k];
-    double s = 0.0;
-    const int64_t base = (int64_t)blockIdx.x * DOT_BS * DOT_ITEMS + threadIdx.x;
+// sum over the block's threads with a fixed tree; every thread gets the total
+__device__ __forceinline__ double block_total(double v) {
+  __shared__ double sh[BCG_BS / 64];
 #pragma unroll
-    for (int it = 0; it < DOT_ITEMS; it++) {
-      const int64_t i = base + (int64_t)it * DOT_BS;
-      if (i < n) s = fma(a[i], b[i], s);
-    }
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = DOT_BS / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[(int64_t)blockIdx.x * d.nd + k] = sh[0];
-    __syncthreads();
-  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();  // sh may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int k = 0; k < BCG_BS / 64; k++) t += sh[k];
+  return t;
 }
 
-// out[k] = sum over blocks: strided per lane in block order, then a fixed tree
-__global__ void __launch_bounds__(DOT_BS) k_dots_final(int nblk, int nd, const double *__restrict__ partial,
-                                                       double *__restrict__ out) {
-  __shared__ double sh[DOT_BS];
-  for (int k = 0; k < nd; k++) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += DOT_BS) s += partial[(int64_t)b * nd + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = DOT_BS / 2; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = sh[0];
-    __syncthreads();
-  }
+// sum of nblk per-block partials, identical in every block
+__device__ __forceinline__ double partial_total(const double *__restrict__ part, int nblk) {
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nblk; k += BCG_BS) s += part[k];
+  return block_total(s);
 }
 
-// ---- BiCGStab with the scalars on the device ---------------------------------
-enum { S_RHO = 0, S_RR, S_RV, S_TS, S_TT, S_BB, S_RHO_OLD, S_ALPHA, S_OMEGA, S_BETA, S_NSC };
+__device__ __forceinline__ double coef_apply(const Coef &c, const PGrid &g, const double *__restrict__ x, int64_t i) {
+  double s = c.d[i] * x[i];
+  const double r = c.r[i], l = c.l[i], dn = c.dn[i], up = c.up[i];
+  if (r != 0.0) s += r * x[i + 1];
+  if (l != 0.0) s += l * x[i - 1];
+  if (dn != 0.0) s += dn * x[i + g.NX];
+  if (up != 0.0) s += up * x[i - g.NX];
+  return s;
+}
 
-__global__ void k_bcg_init(double *sc) {
+// right-hand side, Jacobi scale, coefficients; x = 0, r = rhat = b, p = v = 0;
+// partials of (rhat, r) and (r, r)
+__global__ void __launch_bounds__(BCG_BS) k_bcg_setup(PGrid g, const double *__restrict__ rho, double gy, double gx,
+                                                      BcgVecs V) {
+  const int64_t n = (int64_t)g.NX * g.NY;
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  double bb = 0.0;
+  if (i < n) {
+    const int ix = (int)(i % g.NX), jy = (int)(i / g.NX);
+    const Row w = pg_row(g, ix, jy);
+    V.cd[i] = w.d, V.cr[i] = w.r, V.cl[i] = w.l, V.cdn[i] = w.dn, V.cup[i] = w.up;
+    const double bi = V.b[i];
+    V.x[i] = 0.0;  // x0 = 0 (initialPressureGuess.c:148-149), so r0 = b
+    V.r[i] = bi;
+    V.rh[i] = bi;
+    V.p[i] = 0.0;
+    V.v[i] = 0.0;
+    bb = bi * bi;
+  }
+  const double tot = block_total(bb);
   if (threadIdx.x == 0) {
-    sc[S_RHO_OLD] = 1.0;
-    sc[S_ALPHA] = 1.0;
-    sc[S_OMEGA] = 1.0;
-    sc[S_BETA] = 0.0;
+    V.prho[blockIdx.x] = tot;
+    V.prr[blockIdx.x] = tot;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    V.sc[S_RHO_OLD] = 1.0;
+    V.sc[S_ALPHA] = 1.0;
+    V.sc[S_OMEGA] = 1.0;
   }
 }
 
-__global__ void k_bcg_beta(double *sc, int first) {
-  if (threadIdx.x == 0)
-    sc[S_BETA] = first ? 0.0 : (sc[S_RHO] / sc[S_RHO_OLD]) * (sc[S_ALPHA] / sc[S_OMEGA]);
+// rho = (rhat, r); beta; p = r + beta (p - omega v); y = Dinv p
+__global__ void __launch_bounds__(BCG_BS) k_bcg_p(int64_t n, int nblk, int first, BcgVecs V) {
+  const double rho = partial_total(V.prho, nblk);
+  const double omega = V.sc[S_OMEGA];
+  const double beta = first ? 0.0 : (rho / V.sc[S_RHO_OLD]) * (V.sc[S_ALPHA] / omega);
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  if (i < n) {
+    const double pi = V.r[i] + beta * (V.p[i] - omega * V.v[i]);
+    V.p[i] = pi;
+    V.y[i] = V.dinv[i] * pi;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) V.sc[S_RHO] = rho;
 }
 
-__global__ void __launch_bounds__(256) k_bcg_p(int64_t n, const double *__restrict__ sc, const double *__restrict__ r,
-                                               const double *__restrict__ v, double *__restrict__ p,
-                                               const double *__restrict__ dinv, double *__restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double beta = sc[S_BETA], omega = sc[S_OMEGA];
-  const double pi = r[i] + beta * (p[i] - omega * v[i]);
-  p[i] = pi;
-  y[i] = dinv[i] * pi;
+// v = A y; partials of (rhat, v)
+__global__ void __launch_bounds__(BCG_BS) k_bcg_v(PGrid g, Coef c, BcgVecs V) {
+  const int64_t n = (int64_t)g.NX * g.NY;
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  double d = 0.0;
+  if (i < n) {
+    const double vi = coef_apply(c, g, V.y, i);
+    V.v[i] = vi;
+    d = V.rh[i] * vi;
+  }
+  const double tot = block_total(d);
+  if (threadIdx.x == 0) V.prv[blockIdx.x] = tot;
 }
 
-__global__ void k_bcg_alpha(double *sc) {
-  if (threadIdx.x == 0) sc[S_ALPHA] = sc[S_RV] != 0.0 ? sc[S_RHO] / sc[S_RV] : 0.0;
+// alpha = rho / (rhat, v); x += alpha y; s = r - alpha v; z = Dinv s
+__global__ void __launch_bounds__(BCG_BS) k_bcg_s(int64_t n, int nblk, BcgVecs V) {
+  const double rv = partial_total(V.prv, nblk);
+  const double alpha = rv != 0.0 ? V.sc[S_RHO] / rv : 0.0;
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  if (i < n) {
+    V.x[i] = V.x[i] + alpha * V.y[i];
+    const double si = V.r[i] - alpha * V.v[i];
+    V.s[i] = si;
+    V.z[i] = V.dinv[i] * si;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) V.sc[S_ALPHA] = alpha;
 }
 
-__global__ void __launch_bounds__(256) k_bcg_s(int64_t n, const double *__restrict__ sc, const double *__restrict__ r,
-                                               const double *__restrict__ v, const double *__restrict__ y,
-                                               double *__restrict__ x, double *__restrict__ s,
-                                               const double *__restrict__ dinv, double *__restrict__ z) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double alpha = sc[S_ALPHA];
-  x[i] = x[i] + alpha * y[i];
-  const double si = r[i] - alpha * v[i];
-  s[i] = si;
-  z[i] = dinv[i] * si;
-}
-
-__global__ void k_bcg_omega(double *sc) {
+// t = A z; partials of (t, s) and (t, t)
+__global__ void __launch_bounds__(BCG_BS) k_bcg_t(PGrid g, Coef c, BcgVecs V) {
+  const int64_t n = (int64_t)g.NX * g.NY;
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  double ts = 0.0, tt = 0.0;
+  if (i < n) {
+    const double ti = coef_apply(c, g, V.z, i);
+    V.t[i] = ti;
+    ts = ti * V.s[i];
+    tt = ti * ti;
+  }
+  const double a = block_total(ts);
+  const double b = block_total(tt);
   if (threadIdx.x == 0) {
-    sc[S_OMEGA] = sc[S_TT] != 0.0 ? sc[S_TS] / sc[S_TT] : 0.0;
-    sc[S_RHO_OLD] = sc[S_RHO];
+    V.pts[blockIdx.x] = a;
+    V.ptt[blockIdx.x] = b;
   }
 }
 
-__global__ void __launch_bounds__(256) k_bcg_x(int64_t n, const double *__restrict__ sc, const double *__restrict__ z,
-                                               const double *__restrict__ s, const double *__restrict__ t,
-                                               double *__restrict__ x, double *__restrict__ r) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double omega = sc[S_OMEGA];
-  x[i] = x[i] + omega * z[i];
-  r[i] = s[i] - omega * t[i];
-}
-
-__global__ void __launch_bounds__(256) k_init_vecs(int64_t n, const double *__restrict__ b, double *__restrict__ x,
-                                                   double *__restrict__ r, double *__restrict__ rh,
-                                                   double *__restrict__ p, double *__restrict__ v) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  x[i] = 0.0;  // x0 = 0 (initialPressureGuess.c:148-149), so r0 = b
-  r[i] = b[i];
-  rh[i] = b[i];
-  p[i] = 0.0;
-  v[i] = 0.0;
+// omega = (t, s) / (t, t); x += omega z; r = s - omega t; partials of (rhat, r), (r, r)
+__global__ void __launch_bounds__(BCG_BS) k_bcg_x(int64_t n, int nblk, BcgVecs V) {
+  const double ts = partial_total(V.pts, nblk);
+  const double tt = partial_total(V.ptt, nblk);
+  const double omega = tt != 0.0 ? ts / tt : 0.0;
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  double rr0 = 0.0, rr = 0.0;
+  if (i < n) {
+    V.x[i] = V.x[i] + omega * V.z[i];
+    const double ri = V.s[i] - omega * V.t[i];
+    V.r[i] = ri;
+    rr0 = V.rh[i] * ri;
+    rr = ri * ri;
+  }
+  const double a = block_total(rr0);
+  const double b = block_total(rr);
+  if (threadIdx.x == 0) {
+    V.prho[blockIdx.x] = a;
+    V.prr[blockIdx.x] = b;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    V.sc[S_OMEGA] = omega;
+    V.sc[S_RHO_OLD] = V.sc[S_RHO];
+  }
 }
 
 __global__ void __launch_bounds__(256) k_store_p(int64_t n, const double *__restrict__ p, double *__restrict__ S,
@@ -257,70 +303,105 @@ extern "C" int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const
   if (g->NX < 5 || g->NY < 5) return fvm_set_err(ctx, FVM_ERR_ARG, "pressure guess needs NX, NY >= 5 (pinned cell (3,3))");
   const int64_t n = (int64_t)g->NX * g->NY;
   const int64_t np = (n + 63) / 64 * 64;
-  const int nblk = (int)((n + DOT_BS * DOT_ITEMS - 1) / (DOT_BS * DOT_ITEMS));
-  // b, dinv, x, r, rh, p, v, s, t, y, z; per-block partials (2 per block); scalars
-  FVM_TRY(fvm_ws(ctx, ctx->scratch, (11 * np + 2 * (size_t)nblk + 64) * sizeof(double)));
+  const int nblk = (int)((n + BCG_BS - 1) / BCG_BS);
+  const int64_t npart = ((int64_t)nblk + 63) / 64 * 64;
+  // 11 vectors, 5 coefficient arrays, 5 partial arrays, scalars
+  FVM_TRY(fvm_ws(ctx, ctx->scratch, (16 * np + 5 * npart + 64) * sizeof(double)));
   double *w = (double *)ctx->scratch.ptr;
-  double *b = w, *dinv = w + np, *x = w + 2 * np, *r = w + 3 * np, *rh = w + 4 * np, *p = w + 5 * np;
-  double *v = w + 6 * np, *s = w + 7 * np, *t = w + 8 * np, *y = w + 9 * np, *z = w + 10 * np;
-  double *partial = w + 11 * np;
-  double *sc = partial + 2 * (size_t)nblk;
+  BcgVecs V;
+  double **vecs[] = {&V.b, &V.dinv, &V.x, &V.r, &V.rh, &V.p, &V.v, &V.s, &V.t, &V.y, &V.z,
+                     &V.cd, &V.cr, &V.cl, &V.cdn, &V.cup};
+  for (int k = 0; k < 16; k++) *vecs[k] = w + (int64_t)k * np;
+  double *pw = w + 16 * np;
+  V.prv = pw, V.pts = pw + npart, V.ptt = pw + 2 * npart, V.prho = pw + 3 * npart, V.prr = pw + 4 * npart;
+  V.sc = pw + 5 * npart;
+  const Coef c = {V.cd, V.cr, V.cl, V.cdn, V.cup};
   const PGrid pg = pgrid(g);
-  const unsigned nb = fvm_blocks(n, 256);
-  auto dots = [&](int nd, const double *a0, const double *b0, const double *a1, const double *b1,
-                  double *out) -> int {
-    DotArgs d;
-    d.nd = nd;
-    d.a[0] = a0;
-    d.b[0] = b0;
-    d.a[1] = a1;
-    d.b[1] = b1;
-    k_dots<<<nblk, DOT_BS, 0, ctx->stream>>>(n, d, partial);
-    FVM_LAUNCH_CHECK(ctx);
-    k_dots_final<<<1, DOT_BS, 0, ctx->stream>>>(nblk, nd, partial, out);
+  const unsigned nb = (unsigned)nblk;
+  FVM_PHASE(ctx, FVM_PHASE_OTHER);
+  k_pg_rhs<<<nb, BCG_BS, 0, ctx->stream>>>(pg, rho, gy, gx, V.b, V.dinv);
+  k_bcg_setup<<<nb, BCG_BS, 0, ctx->stream>>>(pg, rho, gy, gx, V);
+  FVM_LAUNCH_CHECK(ctx);
+  // ||r|| from the (r, r) partials, summed on the host in block order
+  std::vector<double> hp((size_t)nblk);
+  auto rnorm = [&](double *out) -> int {
+    FVM_HIP(ctx, hipMemcpyAsync(hp.data(), V.prr, sizeof(double) * nblk, hipMemcpyDeviceToHost, ctx->stream));
+    FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double t = 0.0;
+    for (int k = 0; k < nblk; k++) t += hp[(size_t)k];
+    *out = sqrt(t);
+    return FVM_OK;
+  };
+  double bnorm = 0.0;
+  FVM_TRY(rnorm(&bnorm));
+  auto iteration = [&](int first) -> int {
+    k_bcg_p<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, first, V);
+    k_bcg_v<<<nb, BCG_BS, 0, ctx->stream>>>(pg, c, V);
+    k_bcg_s<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, V);
+    k_bcg_t<<<nb, BCG_BS, 0, ctx->stream>>>(pg, c, V);
+    k_bcg_x<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, V);
     FVM_LAUNCH_CHECK(ctx);
     return FVM_OK;
   };
-  FVM_PHASE(ctx, FVM_PHASE_OTHER);
-  k_pg_rhs<<<nb, 256, 0, ctx->stream>>>(pg, rho, gy, gx, b, dinv);
-  k_init_vecs<<<nb, 256, 0, ctx->stream>>>(n, b, x, r, rh, p, v);
-  k_bcg_init<<<1, 64, 0, ctx->stream>>>(sc);
-  FVM_LAUNCH_CHECK(ctx);
-  FVM_TRY(dots(1, b, b, nullptr, nullptr, sc + S_BB));
-  double hs[S_NSC];
-  FVM_HIP(ctx, hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  const double bnorm = sqrt(hs[S_BB]);
+  // After the first eight iterations, eight more at a time are replayed from a
+  // captured graph (same kernels, same order); a stream that cannot be
+  // captured keeps launching them one by one.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool try_graph = ctx->profiling == 0;
+  auto release = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  };
   int it = 0;
   double rel = 0.0;
+  int rc = FVM_OK;
   if (bnorm > 0.0) {
     rel = 1.0;
     for (;;) {
-      // (rhat, r) for this iteration and (r, r) of the current residual
-      FVM_TRY(dots(2, rh, r, r, r, sc + S_RHO));
       if (it % 8 == 0 || it == maxit) {
-        FVM_HIP(ctx, hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, ctx->stream));
-        FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        rel = sqrt(hs[S_RR]) / bnorm;
+        double rn = 0.0;
+        if ((rc = rnorm(&rn)) != FVM_OK) break;
+        rel = rn / bnorm;
         if (!(rel > rtol) || it == maxit) break;  // also stops on NaN
       }
-      k_bcg_beta<<<1, 64, 0, ctx->stream>>>(sc, it == 0 ? 1 : 0);
-      k_bcg_p<<<nb, 256, 0, ctx->stream>>>(n, sc, r, v, p, dinv, y);
-      k_pg_apply<<<nb, 256, 0, ctx->stream>>>(pg, y, v);
-      FVM_LAUNCH_CHECK(ctx);
-      FVM_TRY(dots(1, rh, v, nullptr, nullptr, sc + S_RV));
-      k_bcg_alpha<<<1, 64, 0, ctx->stream>>>(sc);
-      k_bcg_s<<<nb, 256, 0, ctx->stream>>>(n, sc, r, v, y, x, s, dinv, z);
-      k_pg_apply<<<nb, 256, 0, ctx->stream>>>(pg, z, t);
-      FVM_LAUNCH_CHECK(ctx);
-      FVM_TRY(dots(2, t, s, t, t, sc + S_TS));
-      k_bcg_omega<<<1, 64, 0, ctx->stream>>>(sc);
-      k_bcg_x<<<nb, 256, 0, ctx->stream>>>(n, sc, z, s, t, x, r);
-      FVM_LAUNCH_CHECK(ctx);
+      if (it >= 8 && it % 8 == 0 && it + 8 <= maxit && try_graph) {
+        if (!exec) {
+          bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+          if (ok) {
+            for (int k = 0; k < 8; k++) {
+              k_bcg_p<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, 0, V);
+              k_bcg_v<<<nb, BCG_BS, 0, ctx->stream>>>(pg, c, V);
+              k_bcg_s<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, V);
+              k_bcg_t<<<nb, BCG_BS, 0, ctx->stream>>>(pg, c, V);
+              k_bcg_x<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, V);
+            }
+            ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && graph != nullptr &&
+                 hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
+          }
+          (void)hipGetLastError();
+          if (!ok) {
+            release();
+            try_graph = false;
+            continue;
+          }
+        }
+        if (hipGraphLaunch(exec, ctx->stream) != hipSuccess) {
+          rc = fvm_set_err(ctx, FVM_ERR_HIP, "pressure guess: graph launch failed");
+          break;
+        }
+        it += 8;
+        continue;
+      }
+      if ((rc = iteration(it == 0 ? 1 : 0)) != FVM_OK) break;
       it++;
     }
   }
-  k_store_p<<<nb, 256, 0, ctx->stream>>>(n, x, S, p_out);
+  release();
+  FVM_TRY(rc);
+  k_store_p<<<nb, BCG_BS, 0, ctx->stream>>>(n, V.x, S, p_out);
   FVM_LAUNCH_CHECK(ctx);
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (iters) *iters = it;
