@@ -256,6 +256,66 @@ def bench_learned_guess(a):
     return 0
 
 
+def bench_pressure_guess(a):
+    """The plain initial pressure guess (initialPressureGuess): GPU BiCGStab to
+    rtol 1e-10 on seeded blob densities, against the host direct solve of the
+    same assembled system (the oracle's assembly, SuperLU) on the same states."""
+    import scipy.sparse.linalg as sla
+    from fvmic2d_amd import learned_guess as lg
+    from fvmic2d_amd.host import initial_pressure_guess
+    from fvmic2d_amd.lib import Context, DeviceArray, DeviceGrid
+    ctx = Context()
+    rows = []
+    for N in a.pg_grids:
+        g = lg.BoxGrid(N)
+        rho = lg.fields(lg.blob_field(N, 11))[3]
+        dg = DeviceGrid(ctx, g)
+        d_rho = DeviceArray(ctx, rho.ravel())
+        d_p = DeviceArray(ctx, np.zeros(N * N))
+        initial_pressure_guess(ctx, dg, d_rho, lg.G, rtol=1e-10, maxit=200000, p_out=d_p)  # warm-up
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            its, rel = initial_pressure_guess(ctx, dg, d_rho, lg.G, rtol=1e-10, maxit=200000, p_out=d_p)
+        gpu_ms = (time.perf_counter() - t0) / reps * 1e3
+        row = {"grid": "%dx%d" % (N, N), "iterations": its, "relres": rel, "gpu_ms": gpu_ms}
+        if not a.no_cpu:
+            import oracle.stokes as OS  # the host restatement: assembly of the same system
+            A, b = OS.pressure_guess_system(g, rho, lg.G)
+            Ac = A.tocsc()
+            t0 = time.perf_counter()
+            ref = sla.spsolve(Ac, b)
+            row["cpu_direct_ms"] = (time.perf_counter() - t0) * 1e3
+            got = d_p.get()
+            row["max_rel_diff_vs_direct"] = float(np.abs(got - ref).max() / np.abs(ref).max())
+        rows.append(row)
+        print("pressure guess %s: %d iterations, %.2f ms on the GPU%s" % (
+            row["grid"], its, gpu_ms, ", %.1f ms host direct" % row["cpu_direct_ms"] if "cpu_direct_ms" in row
+            else ""), file=sys.stderr, flush=True)
+    main_row = next((r for r in rows if r["grid"] == "257x257"), rows[-1])
+    out = {"metric": "plain initial pressure guess (initialPressureGuess) solve time", "value": main_row["gpu_ms"],
+           "unit": "ms", "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic", "scaling": "weak", "steps": 5, "warmup": 1,
+           "config": {"workload": "pressure_guess_box", "grid": main_row["grid"],
+                      "solver": "Jacobi BiCGStab, rtol 1e-10, five kernels per iteration"},
+           "grids": rows,
+           # one iteration streams 37 FP64 values per node through its five kernels
+           # (p: r, p, v, Dinv -> p, y; v: y, 5 coefficients, rhat -> v; s: r, v, y, x,
+           # Dinv -> x, s, z; t: z, 5 coefficients, s -> t; x: x, z, s, t, rhat -> x, r)
+           "roofline": {"bound": "hbm", "kernel": "BiCGStab iteration (5 kernels)",
+                        "achieved": 37 * 8 * int(main_row["grid"].split("x")[0]) ** 2 * main_row["iterations"]
+                        / (main_row["gpu_ms"] * 1e-3) / 1e9,
+                        "peak": 8000.0, "unit": "GB/s",
+                        "frac": 37 * 8 * int(main_row["grid"].split("x")[0]) ** 2 * main_row["iterations"]
+                        / (main_row["gpu_ms"] * 1e-3) / 1e9 / 8000.0, "traffic": None},
+           "cpu_baseline": None if a.no_cpu else {
+               "value": main_row["cpu_direct_ms"], "unit": "ms", "cores": 1, "kind": "port",
+               "sample": "the same assembled system solved by scipy sparse LU (SuperLU), as the reference "
+                         "solves it with a direct LU"}}
+    print(json.dumps(out))
+    return 0
+
+
 def model_flops(net, H):
     """Multiply-add flops (x2) of one forward pass of `net` on one H x H sample."""
     f = 0
@@ -288,16 +348,20 @@ def main():
     ap.add_argument("--lg-test", type=int, default=24)
     ap.add_argument("--lg-epochs", type=int, default=40)
     ap.add_argument("--lg-workers", type=int, default=12)
+    ap.add_argument("--pressure-guess", action="store_true",
+                    help="time the plain initial pressure guess (own JSON line)")
+    ap.add_argument("--pg-grids", type=int, nargs="+", default=[129, 257, 513])
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
-              "and this build has no multi-GPU path (see DESIGN.md for how it would shard).",
-              file=sys.stderr)
+              "and this build has no multi-GPU path.", file=sys.stderr)
         return 2
     if a.copy_calibration:
         return copy_calibration()
     if a.learned_guess:
         return bench_learned_guess(a)
+    if a.pressure_guess:
+        return bench_pressure_guess(a)
     if a.ensemble:
         return bench_ensemble(a)
 
