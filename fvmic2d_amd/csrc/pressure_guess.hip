@@ -343,64 +343,21 @@ extern "C" int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const
     FVM_LAUNCH_CHECK(ctx);
     return FVM_OK;
   };
-  // After the first eight iterations, eight more at a time are replayed from a
-  // captured graph (same kernels, same order); a stream that cannot be
-  // captured keeps launching them one by one.
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  bool try_graph = ctx->profiling == 0;
-  auto release = [&]() {
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
-    exec = nullptr;
-    graph = nullptr;
-  };
   int it = 0;
   double rel = 0.0;
-  int rc = FVM_OK;
   if (bnorm > 0.0) {
     rel = 1.0;
     for (;;) {
       if (it % 8 == 0 || it == maxit) {
         double rn = 0.0;
-        if ((rc = rnorm(&rn)) != FVM_OK) break;
+        FVM_TRY(rnorm(&rn));
         rel = rn / bnorm;
         if (!(rel > rtol) || it == maxit) break;  // also stops on NaN
       }
-      if (it >= 8 && it % 8 == 0 && it + 8 <= maxit && try_graph) {
-        if (!exec) {
-          bool ok = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
-          if (ok) {
-            for (int k = 0; k < 8; k++) {
-              k_bcg_p<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, 0, V);
-              k_bcg_v<<<nb, BCG_BS, 0, ctx->stream>>>(pg, c, V);
-              k_bcg_s<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, V);
-              k_bcg_t<<<nb, BCG_BS, 0, ctx->stream>>>(pg, c, V);
-              k_bcg_x<<<nb, BCG_BS, 0, ctx->stream>>>(n, nblk, V);
-            }
-            ok = hipStreamEndCapture(ctx->stream, &graph) == hipSuccess && graph != nullptr &&
-                 hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess;
-          }
-          (void)hipGetLastError();
-          if (!ok) {
-            release();
-            try_graph = false;
-            continue;
-          }
-        }
-        if (hipGraphLaunch(exec, ctx->stream) != hipSuccess) {
-          rc = fvm_set_err(ctx, FVM_ERR_HIP, "pressure guess: graph launch failed");
-          break;
-        }
-        it += 8;
-        continue;
-      }
-      if ((rc = iteration(it == 0 ? 1 : 0)) != FVM_OK) break;
+      FVM_TRY(iteration(it == 0 ? 1 : 0));
       it++;
     }
   }
-  release();
-  FVM_TRY(rc);
   k_store_p<<<nb, BCG_BS, 0, ctx->stream>>>(n, V.x, S, p_out);
   FVM_LAUNCH_CHECK(ctx);
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
