@@ -316,6 +316,103 @@ def bench_pressure_guess(a):
     return 0
 
 
+VEP_STEP_BYTES_PER_MARKER = 640  # P1 76 + A11 132 + A10/A12 80 + A8 80 + A9 64 + P2 32 + P3 176
+VEP_STEP_BYTES_PER_NODE = 376
+
+
+def vep_algorithmic_bytes(w):
+    """Compulsory state traffic per launch of the single-kernel phases of one
+    vep_subduction step (FP64 = 8 B, int32 = 4 B); phases made of several kernels
+    are left out."""
+    n, nodes = w.n_markers, w.NX * w.NY
+    return {
+        # findMarkerCells + density loop: read X, Y, T, Mat; write cellX, cellY, cpu, rho, rhodot
+        "locate": n * (16 + 8 + 4 + 12 + 16),
+        # subgrid T pass 2: read X, Y, cellX, cellY, T, dTsub; write T, Tdot; gather deltaT
+        "subgrid_t2": n * (16 + 8 + 16 + 16) + nodes * 8,
+        # RK4: read X, Y, cellX, cellY, s(3), e(3), Eii; write X, Y, VX, VY, wxy, s(3), Eii;
+        # gather vx, vy, wxy
+        "advect": n * (80 + 72) + nodes * 24,
+    }
+
+
+def cpu_baseline_vep(w, budget_s=12.0):
+    """The oracle's visco-elasto-plastic step on the same workload, one thread,
+    whole steps until budget_s (at least one)."""
+    import oracle
+    st = oracle.State(w, extra=oracle.VEP_PROJ)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        st.step_vep()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": w.n_markers * steps / el, "unit": "markers/s", "cores": 1, "kind": "port",
+            "sample": "%d full vep_subduction steps (%d markers each) of the C restatement, "
+                      "gcc -O3 -ffp-contract=off, one thread, %.1f s" % (steps, w.n_markers, el),
+            "cpu": platform.processor() or platform.machine()}
+
+
+def bench_vep(a):
+    """The visco-elasto-plastic marker step at full size: 801 x 401 tanh-stretched
+    grid, 25 markers per cell (8M markers), five materials with Drucker-Prager
+    yielding -- locate + density, yielding, viscosity limits, projections,
+    strain/pressure update and residual, subgrid stress, stress update, subgrid
+    temperature and RK4 advection.  The two scalar results (yield flag,
+    residual) are read back by the host every step, as the reference does."""
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import MarkerPath
+    t0 = time.perf_counter()
+    w = workloads.vep_subduction()
+    print("vep: workload generated in %.0f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    mp = MarkerPath(w)
+    n0 = w.n_markers
+    for _ in range(a.warmup):
+        mp.step_vep()
+    mp.ctx.sync()
+    t0 = time.perf_counter()
+    mp.ctx.timer_start()
+    for _ in range(a.steps):
+        y, r = mp.step_vep()
+    ms = mp.ctx.timer_stop()
+    wall = time.perf_counter() - t0
+    nout = mp.find_marker_cells()
+    active = n0 - nout
+    value = active * a.steps / (ms / 1e3)
+    print("vep: %.3f ms per step" % (ms / a.steps), file=sys.stderr, flush=True)
+    phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_vep, 3)
+    ab = vep_algorithmic_bytes(w)
+    cands = {k: v / phase_launches[k] for k, v in phase_ms.items() if ab.get(k)}
+    dom = max(cands, key=cands.get)
+    per_launch_ms = cands[dom]
+    achieved = ab[dom] / (per_launch_ms / 1e3) / 1e9
+    step_bytes = n0 * VEP_STEP_BYTES_PER_MARKER + w.NX * w.NY * VEP_STEP_BYTES_PER_NODE
+    out = {
+        "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "vep_subduction", "grid": "801x401 tanh-stretched", "markers": n0,
+                   "markers_per_cell": 25, "active_markers_after": active, "materials": 5,
+                   "plasticity": "Drucker-Prager", "yielding_last_step": y, "residual_last_step": r},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": per_launch_ms},
+        "step": {"algorithmic_bytes": step_bytes,
+                 "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
+                 "wall_ms_per_step": wall / a.steps * 1e3,
+                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
+    }
+    if not a.no_cpu:
+        cb = cpu_baseline_vep(w, a.cpu_budget)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
 def model_flops(net, H):
     """Multiply-add flops (x2) of one forward pass of `net` on one H x H sample."""
     f = 0
@@ -339,6 +436,8 @@ def main():
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--ens-nmin", type=int, default=65)
     ap.add_argument("--ens-nmax", type=int, default=257)
+    ap.add_argument("--vep", action="store_true",
+                    help="the visco-elasto-plastic step at 801x401, 8M markers")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--learned-guess", action="store_true",
@@ -364,6 +463,8 @@ def main():
         return bench_pressure_guess(a)
     if a.ensemble:
         return bench_ensemble(a)
+    if a.vep:
+        return bench_vep(a)
 
     from fvmic2d_amd import workloads
     from fvmic2d_amd.host import MarkerPath

@@ -429,6 +429,10 @@ struct YieldParams {
   double etamin, etamax, fractionalEtamin;
   double slabAngle, decoupleThickness, minDecoupleDepth, maxDecoupleDepth, decoupleEtaScaler;
   double dt;
+  // sin(F[m][k] / 180 * pi) and tan(slabAngle), evaluated once on the host with
+  // the C library the reference calls (markers.c:1444-1462, viscosity.c:55)
+  double sinF[FVM_MAXMAT][2];
+  double tanSlab;
 };
 
 __device__ __forceinline__ int rheology(const YieldParams &P, int mm, double T, double X, double Y, double *eta,
@@ -451,8 +455,8 @@ __device__ __forceinline__ int rheology(const YieldParams &P, int mm, double T, 
   } else if (law == 6) {
     const double depth = Y, distance = X;
     thiseta = mat.materialEta[mm];
-    if (distance > depth / tan(P.slabAngle)) {
-      if (distance < (depth / tan(P.slabAngle) + P.decoupleThickness)) {
+    if (distance > depth / P.tanSlab) {
+      if (distance < (depth / P.tanSlab + P.decoupleThickness)) {
         if (depth > P.minDecoupleDepth && depth < P.maxDecoupleDepth) thiseta *= P.decoupleEtaScaler;
       }
     }
@@ -464,37 +468,32 @@ __device__ __forceinline__ int rheology(const YieldParams &P, int mm, double T, 
   return 0;
 }
 
-__global__ void __launch_bounds__(256) k_yield(int32_t n, YieldParams P, const double *__restrict__ MX,
-                                               const double *__restrict__ MY, const double *__restrict__ MT,
-                                               const int32_t *__restrict__ Mat, double *__restrict__ eta,
-                                               double *__restrict__ mu, double *__restrict__ sxx,
-                                               double *__restrict__ syy, double *__restrict__ sxy,
-                                               const double *__restrict__ exx, const double *__restrict__ eyy,
-                                               const double *__restrict__ exy, const double *__restrict__ Mp,
-                                               const double *__restrict__ MEii, const int32_t *__restrict__ cX,
-                                               int32_t *__restrict__ flags) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
-  if (cX[m] == -1) return;
+// checkPlasticYielding's per-marker body (markers.c:1381-1505 with the viscosity
+// law of viscosity.c:12-104): 0, 1 when the marker yields, 2 when its material
+// or viscosity law is undefined.
+__device__ __forceinline__ int yield_marker(int64_t m, const YieldParams &P, const double *__restrict__ MX,
+                                            const double *__restrict__ MY, const double *__restrict__ MT,
+                                            const int32_t *__restrict__ Mat, double *__restrict__ eta,
+                                            double *__restrict__ mu, double *__restrict__ sxx,
+                                            double *__restrict__ syy, double *__restrict__ sxy,
+                                            const double *__restrict__ exx, const double *__restrict__ eyy,
+                                            const double *__restrict__ exy, const double *__restrict__ Mp,
+                                            const double *__restrict__ MEii, const int32_t *__restrict__ cX) {
+  if (cX[m] == -1) return 0;
   const int mm = Mat[m];
-  if ((unsigned)mm >= FVM_MAXMAT) {
-    atomicOr(&flags[1], 1);
-    return;
-  }
+  if ((unsigned)mm >= FVM_MAXMAT) return 2;
   const double eta0 = eta[m], mu0 = mu[m];
   double s11 = sxx[m], s22 = syy[m], s12 = sxy[m];
   const double e11 = exx[m], e22 = eyy[m], e12 = exy[m];
   double etanew = eta0;
+  int yielded = 0;
   const double X = eta0 / (mu0 * P.dt + eta0);
   const double sxxnew = s11 * X + 2 * eta0 * e11 * (1.0 - X);
   const double syynew = s22 * X + 2 * eta0 * e22 * (1.0 - X);
   const double sxynew = s12 * X + 2 * eta0 * e12 * (1.0 - X);
   const double siinew = sqrt(0.5 * (sxxnew * sxxnew + syynew * syynew) + sxynew * sxynew);
   double etaL, muL;
-  if (rheology(P, mm, MT[m], MX[m], MY[m], &etaL, &muL)) {
-    atomicOr(&flags[1], 1);
-    return;
-  }
+  if (rheology(P, mm, MT[m], MX[m], MY[m], &etaL, &muL)) return 2;
   mu[m] = muL;
   double etaOut = etaL;
   const Materials &mat = P.mat;
@@ -511,14 +510,14 @@ __global__ void __launch_bounds__(256) k_yield(int32_t n, YieldParams P, const d
       double c, f;
       if (mEii < mat.gamma[mm][0]) {
         c = mat.C[mm][0];
-        f = sin(mat.F[mm][0] / 180.0 * M_PI);
+        f = P.sinF[mm][0];
       } else if (mEii >= mat.gamma[mm][0] && mEii <= mat.gamma[mm][1]) {
         const double dg = (mEii - mat.gamma[mm][0]) / (mat.gamma[mm][1] - mat.gamma[mm][0]);
         c = (mat.C[mm][1] - mat.C[mm][0]) * dg + mat.C[mm][0];
-        f = (sin(mat.F[mm][1] / 180.0 * M_PI) - sin(mat.F[mm][0] / 180.0 * M_PI)) * dg + sin(mat.F[mm][0] / 180.0 * M_PI);
+        f = (P.sinF[mm][1] - P.sinF[mm][0]) * dg + P.sinF[mm][0];
       } else {
         c = mat.C[mm][1];
-        f = sin(mat.F[mm][1] / 180.0 * M_PI);
+        f = P.sinF[mm][1];
       }
       siiyield = c + f * pm;
     } else {
@@ -541,11 +540,35 @@ __global__ void __launch_bounds__(256) k_yield(int32_t n, YieldParams P, const d
         syy[m] = s22;
         sxy[m] = s12;
       }
-      atomicOr(&flags[0], 1);
+      yielded = 1;
     }
     etaOut = etanew;  // markers.c:1493 (the pre-call eta when not yielding)
   }
   eta[m] = etaOut;
+  return yielded;
+}
+
+__global__ void __launch_bounds__(256) k_yield(int32_t n, YieldParams P, const double *__restrict__ MX,
+                                               const double *__restrict__ MY, const double *__restrict__ MT,
+                                               const int32_t *__restrict__ Mat, double *__restrict__ eta,
+                                               double *__restrict__ mu, double *__restrict__ sxx,
+                                               double *__restrict__ syy, double *__restrict__ sxy,
+                                               const double *__restrict__ exx, const double *__restrict__ eyy,
+                                               const double *__restrict__ exy, const double *__restrict__ Mp,
+                                               const double *__restrict__ MEii, const int32_t *__restrict__ cX,
+                                               int32_t *__restrict__ flags) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int r = yield_marker(m, P, MX, MY, MT, Mat, eta, mu, sxx, syy, sxy, exx, eyy, exy, Mp, MEii, cX);
+  // The flags only ever go from 0 to 1, so a wavefront that has any raises them
+  // with a plain store: every writer stores the same value and the host reads
+  // after the kernel.  (A device-scope atomic per wavefront on one word is
+  // serialised where it executes: 125k of them cost ~0.7 ms at 8M markers.)
+  const unsigned long long yb = __ballot(r == 1), bb = __ballot(r == 2);
+  if ((yb | bb) && __lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1) {
+    if (yb) __hip_atomic_store(&flags[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (bb) __hip_atomic_store(&flags[1], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
 }
 
 extern "C" int fvm_check_plastic_yielding(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const Materials *mat,
@@ -564,6 +587,9 @@ extern "C" int fvm_check_plastic_yielding(fvm_ctx *ctx, const fvm_grid *g, fvm_m
   P.maxDecoupleDepth = opt->maxDecoupleDepth;
   P.decoupleEtaScaler = opt->decoupleEtaScaler;
   P.dt = dt;
+  for (int i = 0; i < FVM_MAXMAT; i++)
+    for (int k = 0; k < 2; k++) P.sinF[i][k] = sin(mat->F[i][k] / 180.0 * M_PI);
+  P.tanSlab = tan(opt->slabAngle);
   FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
   int32_t *flags = (int32_t *)ctx->red.ptr + 16;
   FVM_HIP(ctx, hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), ctx->stream));
@@ -754,21 +780,32 @@ extern "C" int fvm_update_marker_density(fvm_ctx *ctx, fvm_markers *mk, const Ma
   return FVM_OK;
 }
 
-// max of non-negative doubles via their ordered bit patterns (order-free, exact)
-__global__ void k_max_pos(int32_t n, const int32_t *__restrict__ cX, const double *__restrict__ v, int absval,
-                          unsigned long long *__restrict__ out) {
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double x = 0.0;
-  if (m < n && cX[m] != -1) {
-    x = absval ? fabs(v[m]) : v[m];
+// max of non-negative doubles via their ordered bit patterns (order-free, exact).
+// A bounded grid strides over the markers and each block issues one atomic, so
+// the number of atomics on the result word stays at the block count.
+#define MAXPOS_BLOCKS 1024
+__global__ void __launch_bounds__(256) k_max_pos(int32_t n, const int32_t *__restrict__ cX,
+                                                 const double *__restrict__ v, int absval,
+                                                 unsigned long long *__restrict__ out) {
+  __shared__ unsigned long long s_w[4];
+  unsigned long long b = 0;
+  for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n; m += (int64_t)gridDim.x * blockDim.x) {
+    if (cX[m] == -1) continue;
+    double x = absval ? fabs(v[m]) : v[m];
     if (!(x > 0.0)) x = 0.0;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    b = u > b ? u : b;
   }
-  unsigned long long b = (unsigned long long)__double_as_longlong(x);
   for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(b, off);
+    const unsigned long long o = __shfl_xor(b, off);
     b = o > b ? o : b;
   }
-  if ((threadIdx.x & 63) == 0 && b) atomicMax(out, b);
+  if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = b;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); k++) b = s_w[k] > b ? s_w[k] : b;
+    if (b) atomicMax(out, b);
+  }
 }
 
 static int max_pos(fvm_ctx *ctx, const fvm_markers *mk, const double *col, int absval, double *res) {
@@ -778,7 +815,8 @@ static int max_pos(fvm_ctx *ctx, const fvm_markers *mk, const double *col, int a
   if (mk->nMark > 0) {
     {
       FVM_PHASE(ctx, FVM_PHASE_OTHER);
-      k_max_pos<<<fvm_blocks(mk->nMark, 256), 256, 0, ctx->stream>>>(mk->nMark, mk->cellX, col, absval, d);
+      const unsigned nb = std::min(fvm_blocks(mk->nMark, 256), (unsigned)MAXPOS_BLOCKS);
+      k_max_pos<<<nb, 256, 0, ctx->stream>>>(mk->nMark, mk->cellX, col, absval, d);
       FVM_LAUNCH_CHECK(ctx);
     }
   }

@@ -162,8 +162,11 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restr
     agg += wt;
   }
   unsigned long long *word = state + 2;
+  // status and value travel in one 64-bit word, so relaxed agent-scope atomics
+  // suffice: a reader sees a whole word and reads nothing else the writer made
+  // (an acquire would add a cache invalidation to every poll)
   if (threadIdx.x == 0)
-    __hip_atomic_store(&word[tile], ((tile == 0 ? 2ull : 1ull) << 32) | (uint32_t)agg, __ATOMIC_RELEASE,
+    __hip_atomic_store(&word[tile], ((tile == 0 ? 2ull : 1ull) << 32) | (uint32_t)agg, __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
   if (tile > 0 && threadIdx.x < 64) {
     // wave 0 looks back over a window of 64 predecessors at a time: lane i
@@ -177,7 +180,7 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restr
       unsigned long long w = idx >= 0 ? 0ull : (2ull << 32);  // before tile 0: inclusive 0
       int spins = 0;
       for (;;) {
-        if (idx >= 0 && (w >> 32) == 0) w = __hip_atomic_load(&word[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (idx >= 0 && (w >> 32) == 0) w = __hip_atomic_load(&word[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (__ballot((w >> 32) == 0) == 0) break;
         if (++spins > SCAN_SPIN_MAX) {
           failed = true;
@@ -197,7 +200,7 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restr
     }
     if (lane == 0) {
       if (failed) atomicOr(fail, 1);
-      __hip_atomic_store(&word[tile], (2ull << 32) | (uint32_t)(prefix + agg), __ATOMIC_RELEASE,
+      __hip_atomic_store(&word[tile], (2ull << 32) | (uint32_t)(prefix + agg), __ATOMIC_RELAXED,
                          __HIP_MEMORY_SCOPE_AGENT);
       s_prefix = prefix;
     }
