@@ -389,7 +389,8 @@ __global__ void __launch_bounds__(256) k_strain_pressure(DGrid g, int32_t n, con
   const double d11 = t11 - mexx, d12 = t12 - mexy;
   const double r = sqrt(d11 * d11 + d12 * d12);
   srr[m] = r;
-  if (r != r) atomicOr(&flags[0], 1);
+  // the flag only goes from 0 to 1: a plain store, never an atomic per marker
+  if (r != r) __hip_atomic_store(&flags[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 extern "C" int fvm_update_marker_strain_pressure(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,

@@ -23,6 +23,7 @@ struct DensityTab {
   double rho[FVM_MAXMAT], alpha[FVM_MAXMAT];
 };
 
+#define LOC_TILE_BYTES 4096
 template <bool LDS>
 __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double *__restrict__ X,
                                                 const double *__restrict__ Y, int32_t *__restrict__ cellX,
@@ -31,7 +32,7 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
                                                 int32_t *__restrict__ slot, int32_t *__restrict__ count,
                                                 DensityTab dens, int do_dens, const int32_t *__restrict__ Mat,
                                                 const double *__restrict__ T, double *__restrict__ rho,
-                                                double *__restrict__ rhodot) {
+                                                double *__restrict__ rhodot, int tiles) {
   extern __shared__ double sgrid[];
   const double *gx = g.x, *gy = g.y, *gxc = g.xc, *gyc = g.yc;
   if (LDS) {
@@ -48,44 +49,50 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
   }
   const double ginvx = (double)(g.NX - 1) / (gx[g.NX - 1] - gx[0]);
   const double ginvy = (double)(g.NY - 1) / (gy[g.NY - 1] - gy[0]);
-  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int out = 0, bad = 0, k = -1;
-  if (m < n) {
-    const double xm = X[m], ym = Y[m];
-    int cx = -1, cy = -1;
-    if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
-      cx = guess_grid(gx, g.NX, xm, ginvx);
-      cy = guess_grid(gy, g.NY, ym, ginvy);
-      if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
-        bad = 1;
+  // each block locates `tiles` consecutive tiles of blockDim.x markers, so that
+  // the grid staged in LDS serves more than one tile
+  int nout = 0, nbad = 0;
+  for (int t = 0; t < tiles; t++) {
+    const int64_t m = ((int64_t)blockIdx.x * tiles + t) * blockDim.x + threadIdx.x;
+    int out = 0, bad = 0, k = -1;
+    if (m < n) {
+      const double xm = X[m], ym = Y[m];
+      int cx = -1, cy = -1;
+      if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
+        cx = guess_grid(gx, g.NX, xm, ginvx);
+        cy = guess_grid(gy, g.NY, ym, ginvy);
+        if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
+          bad = 1;
+        } else {
+          const int bx = xm >= gxc[cx + 1] ? 1 : 0;
+          const int by = ym >= gyc[cy + 1] ? 1 : 0;
+          k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+        }
       } else {
-        const int bx = xm >= gxc[cx + 1] ? 1 : 0;
-        const int by = ym >= gyc[cy + 1] ? 1 : 0;
-        k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+        out = 1;
       }
-    } else {
-      out = 1;
+      cellX[m] = cx;
+      cellY[m] = cy;
+      cpu[m] = rank;
+      if (do_dens && cx != -1) {
+        const int mm = Mat[m] & (FVM_MAXMAT - 1);
+        rho[m] = -dens.rho[mm] * (dens.alpha[mm] * T[m]);
+        rhodot[m] = 0.0;
+      }
+      if (count) key[m] = k;
     }
-    cellX[m] = cx;
-    cellY[m] = cy;
-    cpu[m] = rank;
-    if (do_dens && cx != -1) {
-      const int mm = Mat[m] & (FVM_MAXMAT - 1);
-      rho[m] = -dens.rho[mm] * (dens.alpha[mm] * T[m]);
-      rhodot[m] = 0.0;
+    if (count) {
+      const int sl = wave_count_slot(k, count);
+      if (k >= 0) slot[m] = sl;
     }
-    if (count) key[m] = k;
-  }
-  if (count) {
-    const int sl = wave_count_slot(k, count);
-    if (k >= 0) slot[m] = sl;
+    nout += __popcll(__ballot(out));
+    nbad += __popcll(__ballot(bad));
   }
   // counts of outside markers and bad cells (integer: order-free), one atomic
   // per wavefront that has any
-  const unsigned long long bo = __ballot(out), bb = __ballot(bad);
   if ((threadIdx.x & 63) == 0) {
-    if (bo) atomicAdd(&counters[0], __popcll(bo));
-    if (bb) atomicAdd(&counters[1], __popcll(bb));
+    if (nout) atomicAdd(&counters[0], nout);
+    if (nbad) atomicAdd(&counters[1], nbad);
   }
 }
 
@@ -351,15 +358,16 @@ int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t r
     int32_t *count = build_order ? (int32_t *)ctx->count.ptr : nullptr;
     const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
     if (lds <= 48 * 1024) {
-      k_locate<true><<<fvm_blocks(n, 256), 256, lds, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
-                                                                     mk->cpu, rank, counters, key, slot, count, dt,
-                                                                     dens ? 1 : 0, mk->Mat, mk->T, mk->rho,
-                                                                     mk->rhodot);
+      // one tile of 256 markers per LOC_TILE_BYTES of staged grid (1 to 8)
+      const int tiles = std::max(1, std::min(8, (int)(lds / LOC_TILE_BYTES)));
+      k_locate<true><<<fvm_blocks(n, 256 * tiles), 256, lds, ctx->stream>>>(
+          dg, n, mk->X, mk->Y, mk->cellX, mk->cellY, mk->cpu, rank, counters, key, slot, count, dt, dens ? 1 : 0,
+          mk->Mat, mk->T, mk->rho, mk->rhodot, tiles);
     } else {
       k_locate<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(dg, n, mk->X, mk->Y, mk->cellX, mk->cellY,
                                                                     mk->cpu, rank, counters, key, slot, count, dt,
                                                                     dens ? 1 : 0, mk->Mat, mk->T, mk->rho,
-                                                                    mk->rhodot);
+                                                                    mk->rhodot, 1);
     }
     FVM_LAUNCH_CHECK(ctx);
   }
