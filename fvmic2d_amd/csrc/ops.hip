@@ -12,9 +12,10 @@ int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
 
 // ---- A6: advectMarkersRK (markers.c:1865-2250), body in marker_common.h ----------
 // With LDS the grid coordinates x, y, xc, yc (2 NX + 2 NY + 2 doubles) are staged
-// in LDS once per block, and each block walks ADV_TILES tiles of 256 markers so
-// that the staging is amortised; the per-marker arithmetic is unchanged.
-#define ADV_TILES 2
+// in LDS once per block, and each block walks `tiles` tiles of 256 markers so
+// that the staging is amortised (2 for small grids, one more per ADV_TILE_BYTES
+// of staged grid beyond that, at most 8); the per-marker arithmetic is unchanged.
+#define ADV_TILE_BYTES 4096
 template <bool LDS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_advect_rk4(DGrid g, int32_t n, double *__restrict__ MX, double *__restrict__ MY,
                                                     double *__restrict__ MVX, double *__restrict__ MVY,
@@ -27,7 +28,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                                                     const double *__restrict__ wxy, BoundaryValues bv, double dt,
                                                     int jaumann, double *__restrict__ MT, double *__restrict__ MTdot,
                                                     const double *__restrict__ dTn, const double *__restrict__ dTsub,
-                                                    double dt_thermal) {
+                                                    double dt_thermal, int tiles) {
   if (LDS) {
     extern __shared__ double sgrid[];
     for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
@@ -40,9 +41,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     gl.y = sgrid + g.NX;
     gl.xc = sgrid + g.NX + g.NY;
     gl.yc = sgrid + 2 * g.NX + g.NY + 1;
-    const int64_t m0 = (int64_t)blockIdx.x * (256 * ADV_TILES) + threadIdx.x;
+    const int64_t m0 = (int64_t)blockIdx.x * (256 * tiles) + threadIdx.x;
 #pragma unroll 1
-    for (int t = 0; t < ADV_TILES; t++) {
+    for (int t = 0; t < tiles; t++) {
       const int64_t m = m0 + 256 * t;
       if (m < n)
         advect_marker(m, gl, MX, MY, MVX, MVY, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, cellXa, cellYa, vx, vy, wxy,
@@ -63,13 +64,16 @@ static int launch_advect(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const
   const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
   double *T = deltaT ? mk->T : nullptr, *Tdot = deltaT ? mk->Tdot : nullptr;
   if (lds <= 32 * 1024) {
-    k_advect_rk4<true><<<(unsigned)(((int64_t)n + 256 * ADV_TILES - 1) / (256 * ADV_TILES)), 256, lds, ctx->stream>>>(
+    const int tiles = std::max(2, std::min(8, (int)(lds / ADV_TILE_BYTES)));
+    k_advect_rk4<true><<<fvm_blocks(n, 256 * tiles), 256, lds, ctx->stream>>>(
         dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy,
-        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal);
+        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal,
+        tiles);
   } else {
     k_advect_rk4<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(
         dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy,
-        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal);
+        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal,
+        1);
   }
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
