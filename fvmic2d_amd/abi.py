@@ -118,7 +118,7 @@ class Options(C.Structure):
 
 NODAL_NAMES = ["rho", "rhodot", "etaS", "etaN", "thisT", "lastT", "kThermal", "Cp", "edotxx",
                "edotxy", "edotyy", "soxx", "soxy", "soyy", "dsxx", "dsyy", "dsxy", "muS", "muN",
-               "vx", "vy", "p", "wxy", "strainRateResidual"]
+               "vx", "vy", "p", "wxy", "strainRateResidual", "eii", "sii"]
 
 
 class NodalArrays(C.Structure):

@@ -114,6 +114,26 @@ class MarkerPath:
         self.call("fvm_update_marker_stress", self.grid.ref, self.nodal.ref, self.markers.ref,
                   C.byref(self.w.materials))
 
+    # nodal step between the Stokes solve and the marker updates ------------
+    def retrieve_solutions(self, S, Kcont):
+        """retrieveSolutions: S a device array of NX*NY*3 (vx, vy, p interleaved),
+        its pressure entries scaled in place."""
+        fn = I32(0)
+        self.call("fvm_retrieve_solutions", self.grid.ref, self.nodal.ref, C.c_void_p(S.ptr), Kcont,
+                  C.byref(fn))
+        return fn.value
+
+    def nodal_stress_strain(self, dt, heating):
+        """nodalStressStrain: heating a device array of NX*NY (shear heating out)."""
+        self.call("fvm_nodal_stress_strain", self.grid.ref, self.nodal.ref, C.byref(self.w.bv), dt,
+                  C.c_void_p(heating.ptr))
+
+    def limit_displacement_timestep(self, dt):
+        d = C.c_double(dt)
+        self.call("fvm_limit_displacement_timestep", self.grid.ref, self.nodal.ref, C.byref(d),
+                  C.byref(self.w.options))
+        return d.value
+
     def update_marker_strain_pressure(self, dt):
         self.call("fvm_update_marker_strain_pressure", self.grid.ref, self.nodal.ref,
                   self.markers.ref, C.byref(self.w.materials), C.byref(self.w.options), dt)

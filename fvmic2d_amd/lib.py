@@ -91,6 +91,9 @@ _SIGS = {
     "fvm_update_marker_density": [P, P, P],
     "fvm_max_marker_T": [P, P, C.POINTER(D)],
     "fvm_max_abs_sxx": [P, P, C.POINTER(D)],
+    "fvm_retrieve_solutions": [P, P, P, P, D, C.POINTER(I32)],
+    "fvm_nodal_stress_strain": [P, P, P, P, D, P],
+    "fvm_limit_displacement_timestep": [P, P, P, C.POINTER(D), P],
     "fvm_step_thermal": [P, P, P, P, P, P, P, D, I32, P],
     "fvm_step_vep": [P, P, P, P, P, P, P, P, D, C.POINTER(I32), C.POINTER(D)],
     "fvm_profile_enable": [P, I32],

@@ -506,3 +506,40 @@ WORKLOADS = {
     "convection_tdep": convection_tdep,
     "vep_subduction": vep_subduction,
 }
+
+
+def nodal_box(NX=801, NY=401, LX=2.0, LY=1.0, seed=5, graded=True):
+    """Nodal fields for the step between the Stokes solve and the marker updates
+    (retrieveSolutions, nodalStressStrain, limitDisplacementTimestep): a
+    tanh-stretched (or uniform) grid, smooth seeded velocities of either sign,
+    viscosities 1e19..1e23, shear moduli 1e10..3e10 (mu = 0 on a few nodes),
+    stresses of either sign, densities and density rates.  No markers."""
+    rng = Rng(seed, 7)
+    x = spacing_tanh_band(NX, LX) if graded else spacing_uniform(NX, LX)
+    y = spacing_tanh_band(NY, LY) if graded else spacing_uniform(NY, LY)
+    xp, xcp = pad_grid(x, LX)
+    yp, ycp = pad_grid(y, LY)
+    X, Y = np.meshgrid(x / LX, y / LY)
+
+    def u(lo, hi):
+        return lo + (hi - lo) * rng.uniform(NY * NX).reshape(NY, NX)
+
+    a, b = u(0.5, 2.0)[0, 0], u(0.5, 2.0)[0, 0]
+    nodal = {
+        "vx": 1e-9 * (a * np.sin(np.pi * X) * np.cos(2 * np.pi * Y) + 0.1 * u(-1.0, 1.0)),
+        "vy": -1e-9 * (b * np.cos(np.pi * X) * np.sin(2 * np.pi * Y) + 0.1 * u(-1.0, 1.0)),
+        "etaN": 10.0 ** u(19.0, 23.0), "etaS": 10.0 ** u(19.0, 23.0),
+        "muN": u(1e10, 3e10), "muS": u(1e10, 3e10),
+        "soxx": u(-1e7, 1e7), "soyy": u(-1e7, 1e7), "soxy": u(-1e7, 1e7),
+        "rho": u(3000.0, 3300.0), "rhodot": u(-1e-12, 1e-12),
+    }
+    nodal["muN"][::7, ::5] = 0.0
+    nodal["muS"][::5, ::7] = 0.0
+    nodal["vx"][3, 4] = 0.0
+    nodal["rhodot"][2, 2] = 0.0
+    mk = _new_markers(0)
+    opt = abi.default_options()
+    opt.displacementStepLimit = 0.1
+    return Workload(name="nodal_box", NX=NX, NY=NY, LX=LX, LY=LY, xp=xp, yp=yp, xcp=xcp, ycp=ycp, markers=mk,
+                    nodal=nodal, materials=abi.Materials(), options=opt, bv=_bc_free_slip(), dt=3.15e10,
+                    kind="plain", projections=[], seed=seed)

@@ -48,7 +48,8 @@ enum {
   FVM_ERR_HIP = 8,               /* a HIP runtime call failed; see fvm_last_error */
   FVM_ERR_ARG = 9,               /* invalid argument (NULL pointer, size) */
   FVM_ERR_TOO_MANY_FIELDS = 10,  /* more projected fields in one group than supported */
-  FVM_ERR_NOT_CONVERGED = 11     /* an iterative solve missed its tolerance */
+  FVM_ERR_NOT_CONVERGED = 11,    /* an iterative solve missed its tolerance */
+  FVM_ERR_BC = 12                /* a boundary type whose ghost value the reference leaves undefined */
 };
 
 /* ---- device marker store: one device column per Marker field --------------- */
@@ -195,6 +196,36 @@ int fvm_update_marker_density(fvm_ctx *ctx, fvm_markers *mk, const Materials *ma
 int fvm_max_marker_T(fvm_ctx *ctx, const fvm_markers *mk, double *maxT);
 /* :437-448 max |sxx| over active markers. */
 int fvm_max_abs_sxx(fvm_ctx *ctx, const fvm_markers *mk, double *sxxmax);
+
+/* ---- nodal step between the Stokes solve and the marker updates ---------- */
+/* retrieveSolutions.c:4-82 retrieveSolutions(grid, nodalFields, S, Kcont, &foundnan):
+ * S (device, NX*NY*3 doubles) is the interleaved Stokes solution [jy][ix][vx, vy, p]
+ * (fdcode.h:242-245); its pressure entries are scaled by Kcont in place, vx, vy, p
+ * are written to nf, and the Kcont-scaled row-1 pressure mean (weights dx/LX,
+ * summed in index order) is removed from every node with jy > 0, ix > 0.
+ * *foundnan (optional) is set to 0: the reference checks only when built with
+ * TEXTURE.  Called at main_subduction.c:320. */
+int fvm_retrieve_solutions(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, double *S,
+                           double Kcont, int32_t *foundnan);
+/* nodalStressStrain.c:6-485 nodalStressStrain(grid, nodalFields, options, bv, dt,
+ * nodalHeating, gy): from nf->vx, vy, etaN, muN, etaS, muS and the old stresses
+ * soxx, soyy, soxy, writes edotxx, edotyy (cell centres, 0 on row/column 0),
+ * edotxy and wxy (basic nodes; left/top ghost velocities by mechBCLeft.type[1] /
+ * mechBCTop.type[0], which must be 0 or 1, else FVM_ERR_BC), the new stresses
+ * soxx, soyy, soxy and their changes dsxx, dsyy, dsxy, and on the interior basic
+ * nodes the shear heating (nodalHeating, zero elsewhere) and the second
+ * invariants eii, sii (zero elsewhere).  options and gy are not read by the
+ * reference either.  Called at main_subduction.c:335. */
+int fvm_nodal_stress_strain(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
+                            const BoundaryValues *bv, double dt, double *nodalHeating);
+/* limitTimestep.c:6-133 limitDisplacementTimestep(grid, nodalFields, &dt, options):
+ * *displacementdt (host, in/out) becomes the minimum of itself and, over every
+ * node, dx/|vx|*displacementStepLimit, dy/|vy|*displacementStepLimit (nonzero
+ * velocities; dx, dy the smaller adjacent spacing) and |0.01 rho/rhodot| (nonzero
+ * rhodot), then is capped at 0.01/max|rhodot|.  Exact (a minimum is order-free);
+ * synchronises.  Called at main_subduction.c:349. */
+int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
+                                    double *displacementdt, const Options *opt);
 
 /* ---- whole marker step, in the reference's call order (main_subduction.c) -- */
 /* Thermal-convection step: findMarkerCells (:243), density loop (:255-264),

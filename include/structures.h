@@ -168,6 +168,7 @@ typedef struct {
   double *vx, *vy, *p;
   double *wxy;
   double *strainRateResidual;
+  double *eii, *sii; /* second invariants (nodalStressStrain.c:375-389) */
 } NodalArrays;
 
 #ifdef __cplusplus

@@ -37,6 +37,9 @@ _SIGS = {
     "orc_update_marker_density": [P, P],
     "orc_max_marker_T": [P],
     "orc_max_abs_sxx": [P],
+    "orc_nodal_stress_strain": [P, P, P, D, P],
+    "orc_retrieve_solutions": [P, P, P, D, C.POINTER(I)],
+    "orc_limit_displacement_timestep": [P, P, C.POINTER(D), D],
 }
 _RES = {"orc_max_marker_T": D, "orc_max_abs_sxx": D, "orc_limit_viscosity": None,
         "orc_update_marker_density": None}
@@ -149,6 +152,22 @@ class State:
         self._c("orc_check_plastic_yielding", C.byref(self.grid), C.byref(self.ms),
                 C.byref(self.w.materials), dt, C.byref(y), C.byref(self.w.options))
         return y.value
+
+    def nodal_stress_strain(self, dt, heating):
+        self._c("orc_nodal_stress_strain", C.byref(self.grid), C.byref(self.nf), C.byref(self.w.bv), dt,
+                heating.ctypes.data)
+
+    def retrieve_solutions(self, S, Kcont):
+        fn = I(0)
+        self._c("orc_retrieve_solutions", C.byref(self.grid), C.byref(self.nf), S.ctypes.data, Kcont,
+                C.byref(fn))
+        return fn.value
+
+    def limit_displacement_timestep(self, dt):
+        d = D(dt)
+        self._c("orc_limit_displacement_timestep", C.byref(self.grid), C.byref(self.nf), C.byref(d),
+                self.w.options.displacementStepLimit)
+        return d.value
 
     def limit_viscosity(self):
         self.lib.orc_limit_viscosity(C.byref(self.ms), C.byref(self.w.options))

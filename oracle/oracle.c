@@ -913,3 +913,195 @@ double orc_max_abs_sxx(const MarkerSet *ms) {
     if (ms->markers[m].cellX != -1 && fabs(ms->markers[m].s.T11) > v) v = fabs(ms->markers[m].s.T11);
   return v;
 }
+
+/* ---- nodal step between the Stokes solve and the marker updates ----------- */
+/* nodalStressStrain.c:6-485.  Serial restatement over the whole grid (one rank:
+ * every node owned; the ghosted local arrays read only owned nodes here).
+ * PETSc vector operations are written out in their element order: VecAXPY
+ * y = y + a*x, VecPointwiseDivide/Mult, VecScale, VecShift, VecAYPX(y,-1,x)
+ * y = x - y.  Nodes that the interior loop (:370-394) does not visit get
+ * eii = sii = 0 (the freshly duplicated local vectors are zero) and the heating
+ * vector is zeroed first (:355), so it ends as the shear heating alone. */
+static int vep_node_update(double eta, double mu, double dt, double *so, double e, double *ds) {
+  double X = eta + dt * mu;           /* :93-94 VecCopy, VecAXPY */
+  X = eta / X;                        /* :95 VecPointwiseDivide */
+  double X1 = X * -2.0;               /* :97-98 VecCopy, VecScale */
+  X1 = X1 + 2.0;                      /* :99 VecShift */
+  X1 = eta * X1;                      /* :100 VecPointwiseMult */
+  const double et = X1 * e;           /* :108 */
+  double st = X * (*so);              /* :109 */
+  st = st + 1.0 * et;                 /* :110 VecAXPY(stemp, 1.0, etemp) */
+  *ds = st - *so;                     /* :112-113 VecCopy, VecAYPX(ds, -1, stemp) */
+  *so = st;                           /* :114 */
+  return 0;
+}
+
+int orc_nodal_stress_strain(const GridData *g, NodalArrays *nf, const BoundaryValues *bv, double dt,
+                            double *nodalHeating) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  /* the ghost velocities of :162-180 are defined for prescribed (0) and free-slip (1) sides only */
+  if ((bv->mechBCLeft.type[1] != 0 && bv->mechBCLeft.type[1] != 1) ||
+      (bv->mechBCTop.type[0] != 0 && bv->mechBCTop.type[0] != 1))
+    return ORC_ERR_BC;
+#define N2(a, jy, ix) ((a)[(size_t)(jy) * NX + (ix)])
+  /* cell-centre deviatoric strain rates (:54-75) */
+  for (int ix = 0; ix < NX; ix++)
+    for (int jy = 0; jy < NY; jy++) {
+      if (ix == 0 || jy == 0) {
+        N2(nf->edotxx, jy, ix) = 0.0;
+        N2(nf->edotyy, jy, ix) = 0.0;
+      } else {
+        const double dx = g->x[ix] - g->x[ix - 1];
+        const double dy = g->y[jy] - g->y[jy - 1];
+        const double divv = (N2(nf->vx, jy - 1, ix) - N2(nf->vx, jy - 1, ix - 1)) / dx +
+                            (N2(nf->vy, jy, ix - 1) - N2(nf->vy, jy - 1, ix - 1)) / dy;
+        N2(nf->edotxx, jy, ix) = (N2(nf->vx, jy - 1, ix) - N2(nf->vx, jy - 1, ix - 1)) / dx - divv / 3.0;
+        N2(nf->edotyy, jy, ix) = (N2(nf->vy, jy, ix - 1) - N2(nf->vy, jy - 1, ix - 1)) / dy - divv / 3.0;
+      }
+    }
+  /* normal stresses on the cell-centre nodes (:88-121) */
+  for (size_t i = 0; i < (size_t)NX * NY; i++) {
+    vep_node_update(nf->etaN[i], nf->muN[i], dt, &nf->soxx[i], nf->edotxx[i], &nf->dsxx[i]);
+    vep_node_update(nf->etaN[i], nf->muN[i], dt, &nf->soyy[i], nf->edotyy[i], &nf->dsyy[i]);
+  }
+  /* vorticity and shear strain rate on the basic nodes (:157-236) */
+  for (int jy = 0; jy < NY; jy++)
+    for (int ix = 0; ix < NX; ix++) {
+      double vym, vxm;
+      if (ix == 0)
+        vym = bv->mechBCLeft.type[1] == 0 ? 2.0 * bv->mechBCLeft.value[1] - N2(nf->vy, jy, ix) : N2(nf->vy, jy, ix);
+      else
+        vym = N2(nf->vy, jy, ix - 1);
+      const double vyp = N2(nf->vy, jy, ix);
+      if (jy == 0)
+        vxm = bv->mechBCTop.type[0] == 0 ? 2.0 * bv->mechBCTop.value[0] - N2(nf->vx, jy, ix) : N2(nf->vx, jy, ix);
+      else
+        vxm = N2(nf->vx, jy - 1, ix);
+      const double vxp = N2(nf->vx, jy, ix);
+      const double dx = g->xc[ix + 1] - g->xc[ix];
+      const double dy = g->yc[jy + 1] - g->yc[jy];
+      N2(nf->wxy, jy, ix) = 0.5 * ((vxp - vxm) / dy - (vyp - vym) / dx);
+      N2(nf->edotxy, jy, ix) = 0.5 * ((vxp - vxm) / dy + (vyp - vym) / dx);
+    }
+  /* shear stress on the basic nodes (:257-271) */
+  for (size_t i = 0; i < (size_t)NX * NY; i++)
+    vep_node_update(nf->etaS[i], nf->muS[i], dt, &nf->soxy[i], nf->edotxy[i], &nf->dsxy[i]);
+  /* shear heating, second invariants (:355-394) */
+  for (size_t i = 0; i < (size_t)NX * NY; i++) {
+    nodalHeating[i] = 0.0;
+    nf->eii[i] = 0.0;
+    nf->sii[i] = 0.0;
+  }
+  for (int jy = 1; jy < NY - 1; jy++)
+    for (int ix = 1; ix < NX - 1; ix++) {
+      const double *sxx = nf->soxx, *syy = nf->soyy, *eN = nf->etaN, *exx = nf->edotxx, *eyy = nf->edotyy;
+      double hs = (N2(sxx, jy, ix) * N2(sxx, jy, ix) / N2(eN, jy, ix) +
+                   N2(sxx, jy + 1, ix) * N2(sxx, jy + 1, ix) / N2(eN, jy + 1, ix) +
+                   N2(sxx, jy, ix + 1) * N2(sxx, jy, ix + 1) / N2(eN, jy, ix + 1) +
+                   N2(sxx, jy + 1, ix + 1) * N2(sxx, jy + 1, ix + 1) / N2(eN, jy + 1, ix + 1)) /
+                  8.0;
+      hs = hs + (N2(syy, jy, ix) * N2(syy, jy, ix) / N2(eN, jy, ix) +
+                 N2(syy, jy + 1, ix) * N2(syy, jy + 1, ix) / N2(eN, jy + 1, ix) +
+                 N2(syy, jy, ix + 1) * N2(syy, jy, ix + 1) / N2(eN, jy, ix + 1) +
+                 N2(syy, jy + 1, ix + 1) * N2(syy, jy + 1, ix + 1) / N2(eN, jy + 1, ix + 1)) /
+                    8.0;
+      hs = hs + N2(nf->soxy, jy, ix) * N2(nf->soxy, jy, ix) / N2(nf->etaS, jy, ix);
+      N2(nodalHeating, jy, ix) = N2(nodalHeating, jy, ix) + hs;
+      N2(nf->eii, jy, ix) =
+          sqrt((N2(exx, jy, ix) * N2(exx, jy, ix) + N2(exx, jy + 1, ix) * N2(exx, jy + 1, ix) +
+                N2(exx, jy, ix + 1) * N2(exx, jy, ix + 1) + N2(exx, jy + 1, ix + 1) * N2(exx, jy + 1, ix + 1)) /
+                   8.0 +
+               (N2(eyy, jy, ix) * N2(eyy, jy, ix) + N2(eyy, jy + 1, ix) * N2(eyy, jy + 1, ix) +
+                N2(eyy, jy, ix + 1) * N2(eyy, jy, ix + 1) + N2(eyy, jy + 1, ix + 1) * N2(eyy, jy + 1, ix + 1)) /
+                   8.0 +
+               N2(nf->edotxy, jy, ix) * N2(nf->edotxy, jy, ix));
+      N2(nf->sii, jy, ix) =
+          sqrt((N2(sxx, jy, ix) * N2(sxx, jy, ix) + N2(sxx, jy + 1, ix) * N2(sxx, jy + 1, ix) +
+                N2(sxx, jy, ix + 1) * N2(sxx, jy, ix + 1) + N2(sxx, jy + 1, ix + 1) * N2(sxx, jy + 1, ix + 1)) /
+                   8.0 +
+               (N2(syy, jy, ix) * N2(syy, jy, ix) + N2(syy, jy + 1, ix) * N2(syy, jy + 1, ix) +
+                N2(syy, jy, ix + 1) * N2(syy, jy, ix + 1) + N2(syy, jy + 1, ix + 1) * N2(syy, jy + 1, ix + 1)) /
+                   8.0 +
+               N2(nf->soxy, jy, ix) * N2(nf->soxy, jy, ix));
+    }
+  return ORC_OK;
+}
+
+/* retrieveSolutions.c:4-82 (without TEXTURE: foundnan stays 0).  S is the
+ * interleaved Stokes vector [jy][ix][vx, vy, p] (fdcode.h:242-245); its
+ * pressure entries are scaled in place by Kcont (VecStrideScale, :10), then the
+ * row-1 pressure mean weighted by dx/LX, summed in index order (:49-61), is
+ * removed from every node with jy > 0 and ix > 0 (:65-75). */
+int orc_retrieve_solutions(const GridData *g, NodalArrays *nf, double *S, double Kcont, int *foundnan) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  const size_t nn = (size_t)NX * NY;
+  for (size_t i = 0; i < nn; i++) S[3 * i + 2] = S[3 * i + 2] * Kcont;
+  for (size_t i = 0; i < nn; i++) {
+    nf->p[i] = S[3 * i + 2];
+    nf->vx[i] = S[3 * i + 0];
+    nf->vy[i] = S[3 * i + 1];
+  }
+  double p_mean = 0.0;
+  if (NY > 1)
+    for (int i = 1; i < NX; i++) {
+      const double dx = g->x[i] - g->x[i - 1];
+      p_mean += N2(nf->p, 1, i) * dx / g->LX;
+    }
+  for (int j = 1; j < NY; j++)
+    for (int i = 1; i < NX; i++) N2(nf->p, j, i) -= p_mean;
+  *foundnan = 0;
+  return ORC_OK;
+}
+
+/* limitTimestep.c:6-133 limitDisplacementTimestep.  *dtdisp is the running
+ * minimum over the nodes (vx, vy, density-change candidates; a zero velocity
+ * or density rate offers no candidate), then the 1% density-change limit from
+ * the extreme nodal rhodot (VecMax/VecMin, :112-121). */
+int orc_limit_displacement_timestep(const GridData *g, const NodalArrays *nf, double *dtdisp,
+                                    double dispFactor) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  double d = *dtdisp;
+  for (int ix = 0; ix < NX; ix++)
+    for (int jy = 0; jy < NY; jy++) {
+      double dx, dy, dt1;
+      if (ix == 0) {
+        dx = g->x[ix + 1] - g->x[ix];
+      } else if (ix == NX - 1) {
+        dx = g->x[ix] - g->x[ix - 1];
+      } else {
+        const double dx1 = g->x[ix + 1] - g->x[ix], dx2 = g->x[ix] - g->x[ix - 1];
+        dx = dx1 <= dx2 ? dx1 : dx2;
+      }
+      dt1 = N2(nf->vx, jy, ix) != 0.0 ? dx / (fabs(N2(nf->vx, jy, ix))) * dispFactor : d;
+      if (d > dt1) d = dt1;
+      if (jy == 0) {
+        dy = g->y[jy + 1] - g->y[jy];
+      } else if (jy == NY - 1) {
+        dy = g->y[jy] - g->y[jy - 1];
+      } else {
+        const double dy1 = g->y[jy + 1] - g->y[jy], dy2 = g->y[jy] - g->y[jy - 1];
+        dy = dy1 <= dy2 ? dy1 : dy2;
+      }
+      dt1 = N2(nf->vy, jy, ix) != 0.0 ? dy / (fabs(N2(nf->vy, jy, ix))) * dispFactor : d;
+      if (d > dt1) d = dt1;
+      if (N2(nf->rhodot, jy, ix) != 0.0) dt1 = fabs(0.01 * N2(nf->rho, jy, ix) / N2(nf->rhodot, jy, ix));
+      if (d > dt1) d = dt1;
+    }
+  /* VecMax / VecMin: the first entry, then strictly larger / smaller ones */
+  const size_t nn = (size_t)NX * NY;
+  double rmax = nf->rhodot[0], rmin = nf->rhodot[0];
+  for (size_t i = 1; i < nn; i++) {
+    if (nf->rhodot[i] > rmax) rmax = nf->rhodot[i];
+    if (nf->rhodot[i] < rmin) rmin = nf->rhodot[i];
+  }
+  rmax = fabs(rmax);
+  rmin = fabs(rmin);
+  if (rmin > rmax) rmax = rmin;
+  if (rmax * d > 0.01) d = 0.01 / rmax;
+  *dtdisp = d;
+  return ORC_OK;
+}
+#undef N2

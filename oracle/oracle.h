@@ -32,7 +32,8 @@ enum {
   ORC_ERR_BAD_METHOD = 4,
   ORC_ERR_NAN_RESIDUAL = 5,
   ORC_ERR_VISCOSITY_LAW = 6,
-  ORC_ERR_NOMEM = 7
+  ORC_ERR_NOMEM = 7,
+  ORC_ERR_BC = 8
 };
 
 /* markers.c:85-106 findMarkerCells (+ findMarkerCell 41-83) */
@@ -99,6 +100,17 @@ void orc_update_marker_density(MarkerSet *ms, const Materials *mat);
 double orc_max_marker_T(const MarkerSet *ms);
 /* main_subduction.c:437-448 maximum |sxx| over active markers */
 double orc_max_abs_sxx(const MarkerSet *ms);
+
+/* nodalStressStrain.c:6-485 nodalStressStrain (strain rates, VEP nodal stress
+ * update, shear heating, second invariants).  ORC_ERR_BC for a left/top side
+ * whose ghost velocity the reference leaves undefined (type not 0 or 1). */
+int orc_nodal_stress_strain(const GridData *g, NodalArrays *nf, const BoundaryValues *bv, double dt,
+                            double *nodalHeating);
+/* retrieveSolutions.c:4-82 retrieveSolutions (S: interleaved [vx, vy, p], p scaled in place) */
+int orc_retrieve_solutions(const GridData *g, NodalArrays *nf, double *S, double Kcont, int *foundnan);
+/* limitTimestep.c:6-133 limitDisplacementTimestep (dispFactor = options.displacementStepLimit) */
+int orc_limit_displacement_timestep(const GridData *g, const NodalArrays *nf, double *dtdisp,
+                                    double dispFactor);
 
 #ifdef __cplusplus
 }
