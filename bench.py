@@ -403,6 +403,7 @@ def bench_vep(a):
                  "wall_ms_per_step": wall / a.steps * 1e3,
                  "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
     }
+    out["nodal_step"] = bench_nodal_step(a)
     if not a.no_cpu:
         cb = cpu_baseline_vep(w, a.cpu_budget)
         out["cpu_baseline"] = cb
@@ -411,6 +412,59 @@ def bench_vep(a):
         out["cpu_baseline"] = None
     print(json.dumps(out))
     return 0
+
+
+# algorithmic bytes per node of the nodal step: retrieveSolutions (read S 24,
+# write S.p 8, vx, vy, p 24, shift p 16), nodalStressStrain (read vx, vy, etaN,
+# muN, etaS, muS, s(3) 72; write e(3), wxy, s(3), ds(3) 80; then read s(3),
+# etaN, etaS, e(3) 64, write heating, eii, sii 24), limitDisplacementTimestep
+# (read vx, vy, rho, rhodot 32)
+NODAL_STEP_BYTES_PER_NODE = 72 + 152 + 88 + 32
+
+
+def bench_nodal_step(a, reps=20):
+    """retrieveSolutions + nodalStressStrain + limitDisplacementTimestep on the
+    801 x 401 tanh-stretched grid, in the reference's call order; the time step
+    is read back every repetition as the reference does.  Kcont = 1 so that the
+    repeated in-place scaling of the solution vector leaves it unchanged."""
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import MarkerPath
+    from fvmic2d_amd.lib import DeviceArray
+    w = workloads.nodal_box(801, 401)
+    mp = MarkerPath(w)
+    rng = np.random.default_rng(0)
+    S = DeviceArray(mp.ctx, np.stack([w.nodal["vx"], w.nodal["vy"], rng.normal(size=(w.NY, w.NX))], -1))
+    heat = DeviceArray(mp.ctx, np.zeros((w.NY, w.NX)))
+
+    def step():
+        mp.retrieve_solutions(S, 1.0)
+        mp.nodal_stress_strain(w.dt, heat)
+        return mp.limit_displacement_timestep(w.dt)
+    step()
+    mp.ctx.sync()
+    mp.ctx.timer_start()
+    for _ in range(reps):
+        step()
+    ms = mp.ctx.timer_stop() / reps
+    nodes = w.NX * w.NY
+    res = {"grid": "801x401 tanh-stretched", "nodes": nodes, "ms_per_call_sequence": ms,
+           "algorithmic_bytes": nodes * NODAL_STEP_BYTES_PER_NODE,
+           "achieved_GBs": nodes * NODAL_STEP_BYTES_PER_NODE / (ms / 1e3) / 1e9}
+    if not a.no_cpu:
+        import oracle
+        st = oracle.State(w)
+        Sh = np.stack([w.nodal["vx"], w.nodal["vy"], rng.normal(size=(w.NY, w.NX))], -1).ravel().copy()
+        hh = np.zeros((w.NY, w.NX))
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 2.0:
+            st.retrieve_solutions(Sh, 1.0)
+            st.nodal_stress_strain(w.dt, hh)
+            st.limit_displacement_timestep(w.dt)
+            n += 1
+        res["cpu_oracle_ms"] = (time.perf_counter() - t0) / n * 1e3
+        res["speedup_vs_cpu"] = res["cpu_oracle_ms"] / ms
+    print("nodal step: %.3f ms per call sequence on the GPU" % ms, file=sys.stderr, flush=True)
+    return res
 
 
 def model_flops(net, H):

@@ -134,6 +134,19 @@ class MarkerPath:
                   C.byref(self.w.options))
         return d.value
 
+    # marker files ----------------------------------------------------------
+    def save_markers_binary(self, path, elapsed_time):
+        """saveMarkersBinary: returns the number of markers written."""
+        n = I32(0)
+        self.call("fvm_save_markers_binary", self.markers.ref, str(path).encode(), elapsed_time, C.byref(n))
+        return n.value
+
+    def restart_from_markers(self, path):
+        """restartFromMarkers: returns the elapsed time stored in the file."""
+        t = C.c_double()
+        self.call("fvm_restart_from_markers", self.markers.ref, str(path).encode(), C.byref(t))
+        return t.value
+
     def update_marker_strain_pressure(self, dt):
         self.call("fvm_update_marker_strain_pressure", self.grid.ref, self.nodal.ref,
                   self.markers.ref, C.byref(self.w.materials), C.byref(self.w.options), dt)

@@ -49,7 +49,8 @@ enum {
   FVM_ERR_ARG = 9,               /* invalid argument (NULL pointer, size) */
   FVM_ERR_TOO_MANY_FIELDS = 10,  /* more projected fields in one group than supported */
   FVM_ERR_NOT_CONVERGED = 11,    /* an iterative solve missed its tolerance */
-  FVM_ERR_BC = 12                /* a boundary type whose ghost value the reference leaves undefined */
+  FVM_ERR_BC = 12,               /* a boundary type whose ghost value the reference leaves undefined */
+  FVM_ERR_IO = 13                /* a marker file could not be opened, read or written, or has another layout */
 };
 
 /* ---- device marker store: one device column per Marker field --------------- */
@@ -226,6 +227,25 @@ int fvm_nodal_stress_strain(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *
  * synchronises.  Called at main_subduction.c:349. */
 int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
                                     double *displacementdt, const Options *opt);
+
+/* ---- marker files (checkpoint / restart) ------------------------------------ */
+/* io.c:299-402 saveMarkersBinary(markerset, iMonte, iTime, elapsedTime): writes
+ * `path` (the reference names it output/Markers.<iMonte>.<iTime>) in the
+ * reference's layout -- int32 count of active markers (cellX != -1), double
+ * elapsedTime, then per field the active markers in marker order: cpu, cellX,
+ * cellY (int32), X, Y, VX, VY (double), Mat (int32), T, Tdot, eta, mu, exx, exy,
+ * Eii, sxx, syy, sxy, p, rho, rhodot (double); the layout
+ * post_processing/getBinaryMarkers.m reads.  *nSaved (optional): markers
+ * written.  Synchronises; FVM_ERR_IO when the file cannot be written. */
+int fvm_save_markers_binary(fvm_ctx *ctx, const fvm_markers *mk, const char *path, double elapsedTime,
+                            int32_t *nSaved);
+/* restart.c:8-125 restartFromMarkers: reads such a file into mk (at most
+ * mk->maxMark markers, FVM_ERR_ARG beyond), sets mk->nMark and *elapsedTime.
+ * Every slot of the capacity is first reset as resetMarker does
+ * (markers.c:250-330: X = Y = -1, Mat = -1, the rest 0); the file's fields are
+ * then written to the first nMark slots.  FVM_ERR_IO for a missing, short or
+ * differently sized file.  Synchronises. */
+int fvm_restart_from_markers(fvm_ctx *ctx, fvm_markers *mk, const char *path, double *elapsedTime);
 
 /* ---- whole marker step, in the reference's call order (main_subduction.c) -- */
 /* Thermal-convection step: findMarkerCells (:243), density loop (:255-264),

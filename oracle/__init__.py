@@ -40,6 +40,8 @@ _SIGS = {
     "orc_nodal_stress_strain": [P, P, P, D, P],
     "orc_retrieve_solutions": [P, P, P, D, C.POINTER(I)],
     "orc_limit_displacement_timestep": [P, P, C.POINTER(D), D],
+    "orc_save_markers_binary": [P, C.c_char_p, D],
+    "orc_restart_from_markers": [P, C.c_char_p, C.POINTER(D)],
 }
 _RES = {"orc_max_marker_T": D, "orc_max_abs_sxx": D, "orc_limit_viscosity": None,
         "orc_update_marker_density": None}
@@ -168,6 +170,18 @@ class State:
         self._c("orc_limit_displacement_timestep", C.byref(self.grid), C.byref(self.nf), C.byref(d),
                 self.w.options.displacementStepLimit)
         return d.value
+
+    def save_markers_binary(self, path, elapsed_time):
+        self._c("orc_save_markers_binary", C.byref(self.ms), str(path).encode(), elapsed_time)
+
+    def restart_from_markers(self, path, capacity):
+        """Replaces the markers by the file's (capacity slots); returns the elapsed time."""
+        self.markers = np.zeros(capacity, dtype=abi.MARKER_DTYPE)
+        self.ms = abi.MarkerSet(0, capacity, 0, self.markers.ctypes.data)
+        t = D()
+        self._c("orc_restart_from_markers", C.byref(self.ms), str(path).encode(), C.byref(t))
+        self.markers = self.markers[:capacity]
+        return t.value
 
     def limit_viscosity(self):
         self.lib.orc_limit_viscosity(C.byref(self.ms), C.byref(self.w.options))

@@ -1105,3 +1105,68 @@ int orc_limit_displacement_timestep(const GridData *g, const NodalArrays *nf, do
   return ORC_OK;
 }
 #undef N2
+
+/* ---- marker files (io.c:299-402, restart.c:8-125) ------------------------- */
+/* One rank.  The file: int32 count of active markers (cellX != -1), double
+ * elapsed time, then one column per field over the active markers in marker
+ * order; the field order and types are those of saveMarkersBinary and of
+ * post_processing/getBinaryMarkers.m. */
+typedef struct {
+  size_t off;
+  int is_int;
+} FileField;
+static const FileField kFileFields[21] = {
+    {offsetof(Marker, cpu), 1},   {offsetof(Marker, cellX), 1}, {offsetof(Marker, cellY), 1},
+    {offsetof(Marker, X), 0},     {offsetof(Marker, Y), 0},     {offsetof(Marker, VX), 0},
+    {offsetof(Marker, VY), 0},    {offsetof(Marker, Mat), 1},   {offsetof(Marker, T), 0},
+    {offsetof(Marker, Tdot), 0},  {offsetof(Marker, eta), 0},   {offsetof(Marker, mu), 0},
+    {offsetof(Marker, e.T11), 0}, {offsetof(Marker, e.T12), 0}, {offsetof(Marker, Eii), 0},
+    {offsetof(Marker, s.T11), 0}, {offsetof(Marker, s.T22), 0}, {offsetof(Marker, s.T12), 0},
+    {offsetof(Marker, p), 0},     {offsetof(Marker, rho), 0},   {offsetof(Marker, rhodot), 0}};
+
+int orc_save_markers_binary(const MarkerSet *ms, const char *path, double elapsedTime) {
+  FILE *f = fopen(path, "wb");
+  if (!f) return ORC_ERR_IO;
+  int32_t n = 0;
+  for (int m = 0; m < ms->nMark; m++) n += ms->markers[m].cellX != -1;
+  int ok = fwrite(&n, sizeof(int32_t), 1, f) == 1 && fwrite(&elapsedTime, sizeof(double), 1, f) == 1;
+  for (int k = 0; k < 21 && ok; k++)
+    for (int m = 0; m < ms->nMark && ok; m++) {
+      if (ms->markers[m].cellX == -1) continue;
+      const char *src = (const char *)&ms->markers[m] + kFileFields[k].off;
+      ok = fwrite(src, kFileFields[k].is_int ? 4 : 8, 1, f) == 1;
+    }
+  return (fclose(f) == 0 && ok) ? ORC_OK : ORC_ERR_IO;
+}
+
+/* resetMarker (markers.c:250-330) on every slot of the capacity, then the file's
+ * fields on the first n; strainRateResidual, which resetMarker leaves as it
+ * was, is zeroed. */
+int orc_restart_from_markers(MarkerSet *ms, const char *path, double *elapsedTime) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return ORC_ERR_IO;
+  int32_t n;
+  if (fread(&n, sizeof(int32_t), 1, f) != 1 || fread(elapsedTime, sizeof(double), 1, f) != 1 || n < 0 ||
+      n > ms->maxMark) {
+    fclose(f);
+    return ORC_ERR_IO;
+  }
+  for (int m = 0; m < ms->maxMark; m++) {
+    Marker *mk = &ms->markers[m];
+    memset(mk, 0, sizeof(*mk));
+    mk->X = -1;
+    mk->Y = -1;
+    mk->Mat = -1;
+  }
+  int ok = 1;
+  for (int k = 0; k < 21 && ok; k++)
+    for (int m = 0; m < n && ok; m++) {
+      char *dst = (char *)&ms->markers[m] + kFileFields[k].off;
+      ok = fread(dst, kFileFields[k].is_int ? 4 : 8, 1, f) == 1;
+    }
+  if (ok && fgetc(f) != EOF) ok = 0; /* longer than the layout says */
+  fclose(f);
+  ms->nMark = n;
+  ms->nOut = 0;
+  return ok ? ORC_OK : ORC_ERR_IO;
+}

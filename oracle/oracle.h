@@ -33,7 +33,8 @@ enum {
   ORC_ERR_NAN_RESIDUAL = 5,
   ORC_ERR_VISCOSITY_LAW = 6,
   ORC_ERR_NOMEM = 7,
-  ORC_ERR_BC = 8
+  ORC_ERR_BC = 8,
+  ORC_ERR_IO = 9
 };
 
 /* markers.c:85-106 findMarkerCells (+ findMarkerCell 41-83) */
@@ -111,6 +112,11 @@ int orc_retrieve_solutions(const GridData *g, NodalArrays *nf, double *S, double
 /* limitTimestep.c:6-133 limitDisplacementTimestep (dispFactor = options.displacementStepLimit) */
 int orc_limit_displacement_timestep(const GridData *g, const NodalArrays *nf, double *dtdisp,
                                     double dispFactor);
+
+/* io.c:299-402 saveMarkersBinary / restart.c:8-125 restartFromMarkers (one rank;
+ * the path is the caller's: the reference names it output/Markers.<iMonte>.<iTime>) */
+int orc_save_markers_binary(const MarkerSet *ms, const char *path, double elapsedTime);
+int orc_restart_from_markers(MarkerSet *ms, const char *path, double *elapsedTime);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,77 @@
+"""Marker files written and read on the GPU (fvm_save_markers_binary,
+fvm_restart_from_markers) against the oracle: byte-identical files, bitwise
+equal restarted markers, a save -> restart -> save round trip at 1.44M markers,
+and the refusals (wrong layout, capacity)."""
+import numpy as np
+import pytest
+
+import oracle
+from fvmic2d_amd import workloads
+from fvmic2d_amd.host import MarkerPath
+from fvmic2d_amd.lib import FvmError
+from helpers import assert_bitwise
+from test_oracle_markerfile import _workload
+
+pytestmark = pytest.mark.gpu
+
+
+def test_file_is_byte_identical_to_the_oracle(tmp_path):
+    print("  case: convection_small with random fields, every 7th marker inactive", flush=True)
+    w = _workload()
+    st, mp = oracle.State(w), MarkerPath(w)
+    st.save_markers_binary(tmp_path / "ref", 42.0)
+    n = mp.save_markers_binary(tmp_path / "gpu", 42.0)
+    ref = open(tmp_path / "ref", "rb").read()
+    assert n == int(np.frombuffer(ref, "<i4", 1)[0])
+    assert open(tmp_path / "gpu", "rb").read() == ref
+
+
+def test_restart_matches_the_oracle(tmp_path):
+    w = _workload()
+    st = oracle.State(w)
+    st.save_markers_binary(tmp_path / "f", 7.25)
+    cap = w.n_markers + 5
+    t_ref = st.restart_from_markers(tmp_path / "f", cap)
+    mp = MarkerPath(w, capacity=cap)
+    assert mp.restart_from_markers(tmp_path / "f") == t_ref
+    got = mp.markers.download()
+    ref = st.markers[:st.ms.nMark]
+    assert len(got) == len(ref)
+    for f in ref.dtype.names:
+        assert_bitwise(got[f], ref[f], f)
+
+
+def test_round_trip_at_full_size(tmp_path):
+    print("  case: convection_tdep 201x201, 1.44M markers", flush=True)
+    w = workloads.convection_tdep()
+    mp = MarkerPath(w)
+    mp.step_thermal()
+    before = mp.markers.download()
+    n = mp.save_markers_binary(tmp_path / "a", 1.5)
+    assert n == int((before["cellX"] != -1).sum())
+    mp2 = MarkerPath(w, ctx=mp.ctx)
+    assert mp2.restart_from_markers(tmp_path / "a") == 1.5
+    after = mp2.markers.download()
+    act = before[before["cellX"] != -1]
+    for f in ("X", "Y", "VX", "VY", "T", "Tdot", "eta", "mu", "exx", "exy", "Eii", "sxx", "syy", "sxy", "p",
+              "rho", "rhodot", "Mat", "cellX", "cellY", "cpu"):
+        assert_bitwise(after[f], act[f], f)
+    mp2.save_markers_binary(tmp_path / "b", 1.5)
+    assert open(tmp_path / "a", "rb").read() == open(tmp_path / "b", "rb").read()
+
+
+def test_refusals(tmp_path):
+    w = _workload()
+    mp = MarkerPath(w)
+    mp.save_markers_binary(tmp_path / "f", 0.0)
+    raw = open(tmp_path / "f", "rb").read()
+    open(tmp_path / "short", "wb").write(raw[:-4])
+    with pytest.raises(FvmError):
+        mp.restart_from_markers(tmp_path / "short")
+    with pytest.raises(FvmError):
+        mp.restart_from_markers(tmp_path / "missing")
+    w_small = _workload()
+    w_small.markers = w_small.markers[:2]
+    small = MarkerPath(w_small, capacity=3)
+    with pytest.raises(FvmError):
+        small.restart_from_markers(tmp_path / "f")
