@@ -19,7 +19,7 @@
 #include "fvm_abi.h"
 #include "fvm_compat.h"
 
-#define NPOOL 16
+#define NPOOL 24
 
 static fvm_ctx *g_ctx = NULL;
 static fvm_markers g_mk;
@@ -411,6 +411,145 @@ int32_t checkConvergence(double atol, double rtol, int32_t iter) {
   return 0;
 }
 
+/* ---- retrieveSolutions.c:4-82 ---------------------------------------------------- */
+static double *g_S = NULL; /* device copy of the interleaved solution vector */
+static size_t g_S_cap = 0;
+int retrieveSolutions(GridData *grid, NodalFields *nf, Vec S, double Kcont, int32_t *foundnan) {
+  TRY(begin(grid, NULL));
+  const size_t nn = (size_t)grid->NX * grid->NY;
+  if (3 * nn > g_S_cap) {
+    if (g_S) fvm_free(g_S);
+    g_S = NULL;
+    g_S_cap = 0;
+    TRY(fvm_malloc((void **)&g_S, 3 * nn * sizeof(double)));
+    g_S_cap = 3 * nn;
+  }
+  TRY(fvm_memcpy_htod(g_ctx, g_S, vec_data(S), 3 * nn * sizeof(double)));
+  NodalArrays d;
+  memset(&d, 0, sizeof(d));
+  TRY(pool(grid, 0, &d.vx));
+  TRY(pool(grid, 1, &d.vy));
+  TRY(pool(grid, 2, &d.p));
+  TRY(fvm_retrieve_solutions(g_ctx, &g_grid, &d, g_S, Kcont, foundnan));
+  TRY(fvm_memcpy_dtoh(g_ctx, vec_data(S), g_S, 3 * nn * sizeof(double))); /* pressure scaled in place */
+  TRY(get(grid, d.vx, nf->vx));
+  TRY(get(grid, d.vy, nf->vy));
+  TRY(get(grid, d.p, nf->p));
+  return 0;
+}
+
+/* ---- nodalStressStrain.c:6-485 ------------------------------------------------------ */
+int nodalStressStrain(GridData *grid, NodalFields *nf, Options *options, BoundaryValues *bv, double dt,
+                      Vec nodalHeating, double gy) {
+  (void)options;
+  (void)gy;
+  TRY(begin(grid, NULL));
+  NodalArrays d;
+  memset(&d, 0, sizeof(d));
+  TRY(put(grid, 0, nf->vx, (double **)&d.vx));
+  TRY(put(grid, 1, nf->vy, (double **)&d.vy));
+  TRY(put(grid, 2, nf->etaN, &d.etaN));
+  TRY(put(grid, 3, nf->muN, &d.muN));
+  TRY(put(grid, 4, nf->etaS, &d.etaS));
+  TRY(put(grid, 5, nf->muS, &d.muS));
+  TRY(put(grid, 6, nf->soxx, &d.soxx));
+  TRY(put(grid, 7, nf->soyy, &d.soyy));
+  TRY(put(grid, 8, nf->soxy, &d.soxy));
+  TRY(pool(grid, 9, &d.edotxx));
+  TRY(pool(grid, 10, &d.edotyy));
+  TRY(pool(grid, 11, &d.edotxy));
+  TRY(pool(grid, 12, &d.wxy));
+  TRY(pool(grid, 13, &d.dsxx));
+  TRY(pool(grid, 14, &d.dsyy));
+  TRY(pool(grid, 15, &d.dsxy));
+  TRY(pool(grid, 16, &d.eii));
+  TRY(pool(grid, 17, &d.sii));
+  double *heat;
+  TRY(pool(grid, 18, &heat));
+  TRY(fvm_nodal_stress_strain(g_ctx, &g_grid, &d, bv, dt, heat));
+  TRY(get(grid, d.edotxx, nf->edotxx));
+  TRY(get(grid, d.edotyy, nf->edotyy));
+  TRY(get(grid, d.edotxy, nf->edotxy));
+  TRY(get(grid, d.wxy, nf->wxy));
+  TRY(get(grid, d.soxx, nf->soxx));
+  TRY(get(grid, d.soyy, nf->soyy));
+  TRY(get(grid, d.soxy, nf->soxy));
+  TRY(get(grid, d.dsxx, nf->dsxx));
+  TRY(get(grid, d.dsyy, nf->dsyy));
+  TRY(get(grid, d.dsxy, nf->dsxy));
+  TRY(get(grid, d.eii, nf->eii));
+  TRY(get(grid, d.sii, nf->sii));
+  TRY(get(grid, heat, nodalHeating));
+  return 0;
+}
+
+/* ---- limitTimestep.c:6-133 ---------------------------------------------------------- */
+int limitDisplacementTimestep(GridData *grid, NodalFields *nf, double *displacementdt, Options *options) {
+  TRY(begin(grid, NULL));
+  NodalArrays d;
+  memset(&d, 0, sizeof(d));
+  TRY(put(grid, 0, nf->vx, &d.vx));
+  TRY(put(grid, 1, nf->vy, &d.vy));
+  TRY(put(grid, 2, nf->rho, &d.rho));
+  TRY(put(grid, 3, nf->rhodot, &d.rhodot));
+  TRY(fvm_limit_displacement_timestep(g_ctx, &g_grid, &d, displacementdt, options));
+  return 0;
+}
+
+/* ---- io.c:299-402 / restart.c:8-125 (one rank; files output/Markers.<iMonte>.<iTime>) -- */
+int saveMarkersBinary(MarkerSet *markerset, int32_t iMonte, int32_t iTime, double elapsedTime) {
+  char fn[80];
+  snprintf(fn, sizeof(fn), "output/Markers.%d.%d", iMonte, iTime);
+  TRY(ensure_ctx());
+  TRY(upload_markers(markerset));
+  TRY(fvm_save_markers_binary(g_ctx, &g_mk, fn, elapsedTime, NULL));
+  return 0;
+}
+
+/* Allocates markerset->markers (release it with free()) with room for
+ * nMark * options->maxMarkFactor markers, as restart.c:45 does. */
+int restartFromMarkers(MarkerSet *markerset, GridData *grid, Materials *materials, Options *options, int32_t iMonte,
+                       int32_t iTime, double *elapsedTime) {
+  (void)grid;
+  (void)materials;
+  char fn[80];
+  snprintf(fn, sizeof(fn), "output/Markers.%d.%d", iMonte, iTime);
+  TRY(ensure_ctx());
+  FILE *f = fopen(fn, "rb");
+  int32_t n = -1;
+  if (f) {
+    if (fread(&n, sizeof(int32_t), 1, f) != 1) n = -1;
+    fclose(f);
+  }
+  if (n < 0) {
+    fprintf(stderr, "fvm_compat: cannot read %s\n", fn);
+    g_last_status = FVM_ERR_IO;
+    return FVM_ERR_IO;
+  }
+  int32_t cap = (int32_t)((double)n * options->maxMarkFactor);
+  if (cap < n) cap = n;
+  if (!g_mk_alloc || g_mk.maxMark < cap) {
+    if (g_mk_alloc) fvm_markers_free(&g_mk);
+    g_mk_alloc = 0;
+    TRY(fvm_markers_alloc(g_ctx, cap > 0 ? cap : 1, &g_mk));
+    g_mk_alloc = 1;
+  }
+  TRY(fvm_restart_from_markers(g_ctx, &g_mk, fn, elapsedTime));
+  Marker *m = (Marker *)calloc(cap > 0 ? (size_t)cap : 1, sizeof(Marker));
+  if (!m) return FVM_ERR_NOMEM;
+  TRY(fvm_markers_download_aos(g_ctx, &g_mk, m));
+  for (int32_t i = n; i < cap; i++) { /* resetMarker (markers.c:250-330) */
+    m[i].X = -1.0;
+    m[i].Y = -1.0;
+    m[i].Mat = -1;
+  }
+  markerset->markers = m;
+  markerset->nMark = n;
+  markerset->maxMark = cap;
+  markerset->nOut = 0;
+  return 0;
+}
+
 int fvm_compat_last_status(void) { return g_last_status; }
 
 void fvm_compat_finalize(void) {
@@ -427,6 +566,9 @@ void fvm_compat_finalize(void) {
   if (g_scalar) fvm_free(g_scalar);
   g_scalar = NULL;
   g_scalar_cap = 0;
+  if (g_S) fvm_free(g_S);
+  g_S = NULL;
+  g_S_cap = 0;
   if (g_ctx) fvm_ctx_destroy(g_ctx);
   g_ctx = NULL;
 }

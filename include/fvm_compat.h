@@ -93,6 +93,21 @@ int updateGlobalStrainRateResidual(GridData *grid, MarkerSet *markerset, int32_t
 double getGlobalStrainRateResidual(int32_t iter);
 int32_t checkConvergence(double atol, double rtol, int32_t iter);
 
+/* retrieveSolutions.h: S is the interleaved Stokes solution (a Vec of NX*NY*3) */
+int retrieveSolutions(GridData *grid, NodalFields *nodalFields, Vec S, double Kcont, int32_t *foundnan);
+/* nodalStressStrain.h (nodalStressStrain.c:6-485) */
+int nodalStressStrain(GridData *grid, NodalFields *nodalFields, Options *options, BoundaryValues *bv,
+                      double dt, Vec nodalHeating, double gy);
+/* limitTimestep.h (limitTimestep.c:6-133) */
+int limitDisplacementTimestep(GridData *grid, NodalFields *nodalFields, double *displacementdt,
+                              Options *options);
+/* io.h saveMarkersBinary (io.c:299-402): writes output/Markers.<iMonte>.<iTime> */
+int saveMarkersBinary(MarkerSet *markerset, int32_t iMonte, int32_t iTime, double elapsedTime);
+/* restart.h restartFromMarkers (restart.c:8-125): allocates markerset->markers
+ * (nMark * options->maxMarkFactor slots; release with free()) */
+int restartFromMarkers(MarkerSet *markerset, GridData *grid, Materials *materials, Options *options,
+                       int32_t iMonte, int32_t iTime, double *elapsedTime);
+
 /* Release the GPU state the wrappers keep (context, device copies). */
 void fvm_compat_finalize(void);
 /* Status of the last wrapper call that has no status return (findMarkerCells). */

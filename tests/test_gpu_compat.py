@@ -131,3 +131,67 @@ def test_compat_vep_functions(L):
     for n in ("dsxx", "dsyy", "dsxy"):
         assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
     L.fvm_compat_finalize()
+
+
+def test_compat_nodal_step(L):
+    """retrieveSolutions, nodalStressStrain, limitDisplacementTimestep through the
+    reference's prototypes on host Vecs."""
+    w = workloads.nodal_box(61, 31)
+    hp = HostProblem(w)
+    st = oracle.State(w)
+    nn = w.NX * w.NY
+    rng = np.random.default_rng(5)
+    S = rng.normal(size=3 * nn)
+    S_ref = S.copy()
+    Sv = HostVec(3 * nn, S.ctypes.data)
+    fn = C.c_int32(7)
+    _rc(L.retrieveSolutions(C.byref(hp.grid), C.byref(hp.nf), C.byref(Sv), C.c_double(3.5), C.byref(fn)))
+    st.retrieve_solutions(S_ref, 3.5)
+    assert fn.value == 0 and (S == S_ref).all()
+    for n in ("vx", "vy", "p"):
+        assert (hp.arr[n] == st.nodal[n]).all(), n
+    heat = np.zeros((w.NY, w.NX))
+    hv = HostVec(nn, heat.ctypes.data)
+    _rc(L.nodalStressStrain(C.byref(hp.grid), C.byref(hp.nf), C.byref(w.options), C.byref(w.bv),
+                            C.c_double(w.dt), C.byref(hv), C.c_double(10.0)))
+    h_ref = np.zeros((w.NY, w.NX))
+    st.nodal_stress_strain(w.dt, h_ref)
+    for n in ("edotxx", "edotyy", "edotxy", "wxy", "soxx", "soyy", "soxy", "dsxx", "dsyy", "dsxy", "eii", "sii"):
+        assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
+    assert_close(heat, h_ref, "nodalHeating", scale=field_scale(h_ref))
+    d = C.c_double(w.dt)
+    _rc(L.limitDisplacementTimestep(C.byref(hp.grid), C.byref(hp.nf), C.byref(d), C.byref(w.options)))
+    assert d.value == st.limit_displacement_timestep(w.dt)
+    L.fvm_compat_finalize()
+
+
+def test_compat_marker_files(L, tmp_path, monkeypatch):
+    """saveMarkersBinary / restartFromMarkers write and read output/Markers.<iMonte>.<iTime>."""
+    from test_oracle_markerfile import _workload
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "output").mkdir()
+    w = _workload()
+    hp = HostProblem(w)
+    _rc(L.saveMarkersBinary(C.byref(hp.ms), 2, 17, C.c_double(9.5)))
+    st = oracle.State(w)
+    st.save_markers_binary(tmp_path / "ref", 9.5)
+    assert open(tmp_path / "output" / "Markers.2.17", "rb").read() == open(tmp_path / "ref", "rb").read()
+    ms = abi.MarkerSet()
+    t = C.c_double()
+    opt = abi.default_options()
+    opt.maxMarkFactor = 1.5
+    _rc(L.restartFromMarkers(C.byref(ms), C.byref(hp.grid), C.byref(w.materials), C.byref(opt), 2, 17,
+                             C.byref(t)))
+    assert t.value == 9.5
+    n_act = int((w.markers["cellX"] != -1).sum())
+    assert ms.nMark == n_act and ms.maxMark == int(n_act * 1.5)
+    got = np.ctypeslib.as_array(C.cast(ms.markers, C.POINTER(C.c_uint8)), shape=(ms.maxMark * 216,))
+    got = got.view(abi.MARKER_DTYPE)
+    t_ref = st.restart_from_markers(tmp_path / "ref", ms.maxMark)
+    assert t_ref == 9.5
+    for f in abi.MARKER_DTYPE.names:
+        assert_bitwise(got[f], st.markers[f], f)
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free(ms.markers)
+    L.fvm_compat_finalize()
