@@ -236,14 +236,31 @@ __global__ void __launch_bounds__(256) k_dt_partial(DGrid g, const double *__res
 // limitTimestep.c:112-121 on the combined partials.  VecMax/VecMin start from
 // the first entry and take strictly larger/smaller ones: a NaN first entry
 // stays, later NaNs are skipped.
-__global__ void k_dt_final(int nb, const DtPart *__restrict__ part, const double *__restrict__ rhodot, double d0,
-                           double *__restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  DtPart r = part[0];
-  for (int k = 1; k < nb; k++) {
+__global__ void __launch_bounds__(256) k_dt_final(int nb, const DtPart *__restrict__ part,
+                                                  const double *__restrict__ rhodot, double d0,
+                                                  double *__restrict__ out) {
+  // min and max are exact in any order: the block combines the partials, then
+  // one thread applies the reference's final steps
+  __shared__ DtPart s_p[4];
+  DtPart r{INFINITY, -INFINITY, INFINITY};
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
     take_min(r.cmin, part[k].cmin);
     if (part[k].rmax > r.rmax) r.rmax = part[k].rmax;
     if (part[k].rmin < r.rmin) r.rmin = part[k].rmin;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    take_min(r.cmin, __shfl_xor(r.cmin, off));
+    const double a = __shfl_xor(r.rmax, off), b = __shfl_xor(r.rmin, off);
+    if (a > r.rmax) r.rmax = a;
+    if (b < r.rmin) r.rmin = b;
+  }
+  if ((threadIdx.x & 63) == 0) s_p[threadIdx.x >> 6] = r;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 1; k < (int)(blockDim.x >> 6); k++) {
+    take_min(r.cmin, s_p[k].cmin);
+    if (s_p[k].rmax > r.rmax) r.rmax = s_p[k].rmax;
+    if (s_p[k].rmin < r.rmin) r.rmin = s_p[k].rmin;
   }
   double d = d0;
   take_min(d, r.cmin);
@@ -323,7 +340,7 @@ extern "C" int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, 
     k_dt_partial<<<nb, 256, 0, ctx->stream>>>(dgrid(g), nf->vx, nf->vy, nf->rho, nf->rhodot,
                                               opt->displacementStepLimit, part);
     FVM_LAUNCH_CHECK(ctx);
-    k_dt_final<<<1, 64, 0, ctx->stream>>>(nb, part, nf->rhodot, *displacementdt, dout);
+    k_dt_final<<<1, 256, 0, ctx->stream>>>(nb, part, nf->rhodot, *displacementdt, dout);
     FVM_LAUNCH_CHECK(ctx);
   }
   double *h = (double *)ctx->pinned.ptr + 24;
