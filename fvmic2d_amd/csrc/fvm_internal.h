@@ -341,6 +341,14 @@ int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays 
                                const BoundaryValues *bv, double dt, const double *deltaT, const double *dTsub,
                                double dt_thermal);
 
+// Jacobi BiCGStab (pressure_guess.hip) on a 5-point system over the NX*NY
+// nodes in node order: diagonal d, east (i+1), west (i-1), south (i+NX) and
+// north (i-NX) coefficients, 0 where a row has no such neighbour; x0 = 0.
+// Writes x, synchronises; FVM_ERR_NOT_CONVERGED when rtol is not reached.
+int fvm_bicgstab_5pt(fvm_ctx *ctx, int NX, int NY, const double *d, const double *e, const double *w,
+                     const double *sth, const double *nth, const double *b, double *x, double rtol, int32_t maxit,
+                     int32_t *iters, double *relres);
+
 // sort building blocks reused by the batched ensemble (sort.hip)
 // One launch: count[0..ncount) = 0, counters[0..1] = 0, state[0..nstate) = 0.
 int fvm_launch_zero_sort(fvm_ctx *ctx, int32_t *count, int64_t ncount, int32_t *counters, unsigned long long *state,

@@ -179,6 +179,35 @@ __global__ void __launch_bounds__(BCG_BS) k_bcg_setup(PGrid g, const double *__r
   }
 }
 
+// the same start for a system given as tabulated coefficients: b from the
+// caller, Jacobi scale 1/diagonal
+__global__ void __launch_bounds__(BCG_BS) k_bcg_setup_sys(int64_t n, const double *__restrict__ b, Coef c,
+                                                          BcgVecs V) {
+  const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
+  double bb = 0.0;
+  if (i < n) {
+    const double bi = b[i];
+    V.b[i] = bi;
+    V.dinv[i] = 1.0 / c.d[i];
+    V.x[i] = 0.0;
+    V.r[i] = bi;
+    V.rh[i] = bi;
+    V.p[i] = 0.0;
+    V.v[i] = 0.0;
+    bb = bi * bi;
+  }
+  const double tot = block_total(bb);
+  if (threadIdx.x == 0) {
+    V.prho[blockIdx.x] = tot;
+    V.prr[blockIdx.x] = tot;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    V.sc[S_RHO_OLD] = 1.0;
+    V.sc[S_ALPHA] = 1.0;
+    V.sc[S_OMEGA] = 1.0;
+  }
+}
+
 // rho = (rhat, r); beta; p = r + beta (p - omega v); y = Dinv p
 __global__ void __launch_bounds__(BCG_BS) k_bcg_p(int64_t n, int nblk, int first, BcgVecs V) {
   const double rho = partial_total(V.prho, nblk);
@@ -295,34 +324,28 @@ extern "C" int fvm_pressure_guess_apply(fvm_ctx *ctx, const fvm_grid *g, const d
   return FVM_OK;
 }
 
-extern "C" int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const double *rho, double gy, double gx,
-                                          double rtol, int32_t maxit, double *S, double *p_out, int32_t *iters,
-                                          double *relres) {
-  if (!ctx || !rho || (!S && !p_out) || !(rtol > 0.0) || maxit < 1) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
-  if (g->NX < 5 || g->NY < 5) return fvm_set_err(ctx, FVM_ERR_ARG, "pressure guess needs NX, NY >= 5 (pinned cell (3,3))");
-  const int64_t n = (int64_t)g->NX * g->NY;
+// workspace: 11 vectors, 5 coefficient arrays, 5 partial arrays, scalars
+static int bcg_vecs(fvm_ctx *ctx, int64_t n, BcgVecs *V, int *nblk_out) {
   const int64_t np = (n + 63) / 64 * 64;
   const int nblk = (int)((n + BCG_BS - 1) / BCG_BS);
   const int64_t npart = ((int64_t)nblk + 63) / 64 * 64;
-  // 11 vectors, 5 coefficient arrays, 5 partial arrays, scalars
   FVM_TRY(fvm_ws(ctx, ctx->scratch, (16 * np + 5 * npart + 64) * sizeof(double)));
   double *w = (double *)ctx->scratch.ptr;
-  BcgVecs V;
-  double **vecs[] = {&V.b, &V.dinv, &V.x, &V.r, &V.rh, &V.p, &V.v, &V.s, &V.t, &V.y, &V.z,
-                     &V.cd, &V.cr, &V.cl, &V.cdn, &V.cup};
+  double **vecs[] = {&V->b, &V->dinv, &V->x, &V->r, &V->rh, &V->p, &V->v, &V->s, &V->t, &V->y, &V->z,
+                     &V->cd, &V->cr, &V->cl, &V->cdn, &V->cup};
   for (int k = 0; k < 16; k++) *vecs[k] = w + (int64_t)k * np;
   double *pw = w + 16 * np;
-  V.prv = pw, V.pts = pw + npart, V.ptt = pw + 2 * npart, V.prho = pw + 3 * npart, V.prr = pw + 4 * npart;
-  V.sc = pw + 5 * npart;
-  const Coef c = {V.cd, V.cr, V.cl, V.cdn, V.cup};
-  const PGrid pg = pgrid(g);
+  V->prv = pw, V->pts = pw + npart, V->ptt = pw + 2 * npart, V->prho = pw + 3 * npart, V->prr = pw + 4 * npart;
+  V->sc = pw + 5 * npart;
+  *nblk_out = nblk;
+  return FVM_OK;
+}
+
+// BiCGStab iterations after a setup kernel: the host reads ||r|| from the (r, r)
+// partials (summed in block order) every eight iterations
+static int bcg_loop(fvm_ctx *ctx, const PGrid &pg, const Coef &c, const BcgVecs &V, int64_t n, int nblk,
+                    double rtol, int32_t maxit, int32_t *it_out, double *rel_out) {
   const unsigned nb = (unsigned)nblk;
-  FVM_PHASE(ctx, FVM_PHASE_OTHER);
-  k_pg_rhs<<<nb, BCG_BS, 0, ctx->stream>>>(pg, rho, gy, gx, V.b, V.dinv);
-  k_bcg_setup<<<nb, BCG_BS, 0, ctx->stream>>>(pg, rho, gy, gx, V);
-  FVM_LAUNCH_CHECK(ctx);
-  // ||r|| from the (r, r) partials, summed on the host in block order
   std::vector<double> hp((size_t)nblk);
   auto rnorm = [&](double *out) -> int {
     FVM_HIP(ctx, hipMemcpyAsync(hp.data(), V.prr, sizeof(double) * nblk, hipMemcpyDeviceToHost, ctx->stream));
@@ -358,11 +381,60 @@ extern "C" int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const
       it++;
     }
   }
+  *it_out = it;
+  *rel_out = rel;
+  return FVM_OK;
+}
+
+extern "C" int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const double *rho, double gy, double gx,
+                                          double rtol, int32_t maxit, double *S, double *p_out, int32_t *iters,
+                                          double *relres) {
+  if (!ctx || !rho || (!S && !p_out) || !(rtol > 0.0) || maxit < 1) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  if (g->NX < 5 || g->NY < 5) return fvm_set_err(ctx, FVM_ERR_ARG, "pressure guess needs NX, NY >= 5 (pinned cell (3,3))");
+  const int64_t n = (int64_t)g->NX * g->NY;
+  BcgVecs V;
+  int nblk = 0;
+  FVM_TRY(bcg_vecs(ctx, n, &V, &nblk));
+  const Coef c = {V.cd, V.cr, V.cl, V.cdn, V.cup};
+  const PGrid pg = pgrid(g);
+  const unsigned nb = (unsigned)nblk;
+  FVM_PHASE(ctx, FVM_PHASE_OTHER);
+  k_pg_rhs<<<nb, BCG_BS, 0, ctx->stream>>>(pg, rho, gy, gx, V.b, V.dinv);
+  k_bcg_setup<<<nb, BCG_BS, 0, ctx->stream>>>(pg, rho, gy, gx, V);
+  FVM_LAUNCH_CHECK(ctx);
+  int32_t it = 0;
+  double rel = 0.0;
+  FVM_TRY(bcg_loop(ctx, pg, c, V, n, nblk, rtol, maxit, &it, &rel));
   k_store_p<<<nb, BCG_BS, 0, ctx->stream>>>(n, V.x, S, p_out);
   FVM_LAUNCH_CHECK(ctx);
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (iters) *iters = it;
   if (relres) *relres = rel;
   if (!(rel <= rtol)) return fvm_set_err(ctx, FVM_ERR_NOT_CONVERGED, "pressure guess: relative residual %.3e after %d iterations", rel, it);
+  return FVM_OK;
+}
+
+// Jacobi BiCGStab on a 5-point system over the NX*NY nodes (fvm_internal.h)
+int fvm_bicgstab_5pt(fvm_ctx *ctx, int NX, int NY, const double *d, const double *e, const double *w,
+                     const double *sth, const double *nth, const double *b, double *x, double rtol, int32_t maxit,
+                     int32_t *iters, double *relres) {
+  const int64_t n = (int64_t)NX * NY;
+  BcgVecs V;
+  int nblk = 0;
+  FVM_TRY(bcg_vecs(ctx, n, &V, &nblk));
+  const Coef c = {d, e, w, sth, nth};
+  const PGrid pg = PGrid{NX, NY, nullptr, nullptr, nullptr, nullptr};
+  FVM_PHASE(ctx, FVM_PHASE_OTHER);
+  k_bcg_setup_sys<<<(unsigned)nblk, BCG_BS, 0, ctx->stream>>>(n, b, c, V);
+  FVM_LAUNCH_CHECK(ctx);
+  int32_t it = 0;
+  double rel = 0.0;
+  FVM_TRY(bcg_loop(ctx, pg, c, V, n, nblk, rtol, maxit, &it, &rel));
+  FVM_HIP(ctx, hipMemcpyAsync(x, V.x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (iters) *iters = it;
+  if (relres) *relres = rel;
+  if (!(rel <= rtol)) return fvm_set_err(ctx, FVM_ERR_NOT_CONVERGED, "5-point solve: relative residual %.3e after %d iterations", rel, it);
   return FVM_OK;
 }

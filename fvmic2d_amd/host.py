@@ -134,6 +134,20 @@ class MarkerPath:
                   C.byref(self.w.options))
         return d.value
 
+    # energy equation -------------------------------------------------------
+    def form_thermal_system(self, dt, do_steady, coef, rhs):
+        """formThermalSystem: coef a device array of 5*NX*NY (diagonal, east, west,
+        south, north), rhs a device array of NX*NY (heating in, system RHS out)."""
+        self.call("fvm_form_thermal_system", self.grid.ref, self.nodal.ref, C.byref(self.w.materials),
+                  C.byref(self.w.options), dt, I32(1 if do_steady else 0), C.c_void_p(coef.ptr),
+                  C.c_void_p(rhs.ptr))
+
+    def solve_thermal_system(self, coef, rhs, T, rtol=1e-12, maxit=20000):
+        its, rel = I32(0), C.c_double()
+        self.call("fvm_solve_thermal_system", self.grid.ref, C.c_void_p(coef.ptr), C.c_void_p(rhs.ptr),
+                  C.c_void_p(T.ptr), rtol, I32(maxit), C.byref(its), C.byref(rel))
+        return its.value, rel.value
+
     # marker files ----------------------------------------------------------
     def save_markers_binary(self, path, elapsed_time):
         """saveMarkersBinary: returns the number of markers written."""

@@ -95,6 +95,8 @@ _SIGS = {
     "fvm_nodal_stress_strain": [P, P, P, P, D, P],
     "fvm_limit_displacement_timestep": [P, P, P, C.POINTER(D), P],
     "fvm_save_markers_binary": [P, P, C.c_char_p, D, C.POINTER(I32)],
+    "fvm_form_thermal_system": [P, P, P, P, P, D, I32, P, P],
+    "fvm_solve_thermal_system": [P, P, P, P, P, D, I32, C.POINTER(I32), C.POINTER(D)],
     "fvm_restart_from_markers": [P, P, C.c_char_p, C.POINTER(D)],
     "fvm_step_thermal": [P, P, P, P, P, P, P, D, I32, P],
     "fvm_step_vep": [P, P, P, P, P, P, P, P, D, C.POINTER(I32), C.POINTER(D)],

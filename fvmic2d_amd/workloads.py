@@ -543,3 +543,41 @@ def nodal_box(NX=801, NY=401, LX=2.0, LY=1.0, seed=5, graded=True):
     return Workload(name="nodal_box", NX=NX, NY=NY, LX=LX, LY=LY, xp=xp, yp=yp, xcp=xcp, ycp=ycp, markers=mk,
                     nodal=nodal, materials=abi.Materials(), options=opt, bv=_bc_free_slip(), dt=3.15e10,
                     kind="plain", projections=[], seed=seed)
+
+
+def thermal_box(NX=801, NY=401, LX=660e3, LY=600e3, seed=6, graded=True, top_type=0):
+    """Nodal fields for the energy equation of the subduction set-up
+    (formThermalSystem): conductivity 2..4, density anomaly, heat capacity,
+    a lastT between 273 and 1573 K, vx of either sign near the right wall,
+    a subduction parameter set (30 degree slab of 50 Myr, 50 km plate with a
+    10 km root at 400 km), and heating terms for the right-hand side."""
+    rng = Rng(seed, 9)
+    x = spacing_tanh_band(NX, LX) if graded else spacing_uniform(NX, LX)
+    y = spacing_tanh_band(NY, LY) if graded else spacing_uniform(NY, LY)
+    xp, xcp = pad_grid(x, LX)
+    yp, ycp = pad_grid(y, LY)
+    X, Y = np.meshgrid(x / LX, y / LY)
+
+    def u(lo, hi):
+        return lo + (hi - lo) * rng.uniform(NY * NX).reshape(NY, NX)
+
+    nodal = {
+        "kThermal": u(2.0, 4.0), "rho": u(-50.0, 50.0), "Cp": u(1000.0, 1250.0),
+        "lastT": 273.0 + 1300.0 * np.clip(Y * 3.0, 0.0, 1.0) + u(-5.0, 5.0),
+        "vx": 1e-9 * np.sin(2 * np.pi * Y * 3.0) + 1e-10 * u(-1.0, 1.0),
+        "heating": u(0.0, 1e-6),
+    }
+    opt = abi.default_options()
+    opt.thermalBCTop.type[0] = top_type
+    opt.thermalBCTop.value[0] = 273.0
+    opt.slabAngle = math.pi / 6.0
+    opt.slabAge = 50.0
+    opt.overridingPlateThickness = 50e3
+    opt.rootThickness = 10e3
+    opt.rootCenter = 400e3
+    opt.rootWidth = 40e3
+    mat = abi.Materials()
+    mat.materialRho[0] = 3300.0
+    return Workload(name="thermal_box", NX=NX, NY=NY, LX=LX, LY=LY, xp=xp, yp=yp, xcp=xcp, ycp=ycp,
+                    markers=_new_markers(0), nodal=nodal, materials=mat, options=opt, bv=_bc_free_slip(),
+                    dt=3.15e11, kind="plain", projections=[], seed=seed)

@@ -228,6 +228,28 @@ int fvm_nodal_stress_strain(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *
 int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
                                     double *displacementdt, const Options *opt);
 
+/* ---- energy equation ------------------------------------------------------ */
+/* formThermalSystem.c:6-249 formThermalSystem(problem, grid, nodalFields, LHS,
+ * RHS, dt, options, doSteady) for the subduction set-up: the 5-point system on
+ * the basic nodes as five device arrays in coef (5*NX*NY doubles: diagonal, east
+ * (ix+1), west (ix-1), south (jy+1), north (jy-1); 0 where a row has no such
+ * entry), rows in node order.  rhs (device NX*NY, in/out) holds the heating
+ * terms on entry, as the reference's RHS does; interior rows gain
+ * (rho0 + rho) Cp / dt * lastT (doSteady == 0), boundary rows get their values:
+ * left wall the slab inflow temperature, right wall the plate geotherm or 1573
+ * where y <= plate depth or the mean vx of the node's two vx values is negative,
+ * else lastT (outflow), top thermalBCTop (type 0 value, type 1 zero gradient;
+ * other types FVM_ERR_BC), bottom lastT.  Reads nf->kThermal, rho, Cp, lastT, vx
+ * and materialRho[0]. */
+int fvm_form_thermal_system(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, const Materials *mat,
+                            const Options *opt, double dt, int32_t doSteady, double *coef, double *rhs);
+/* The "energy_" solve of that system (the reference uses a direct LU through
+ * its options file): BiCGStab from T = 0 on the system with every row divided by
+ * its diagonal, to rtol relative residual of that scaled system.
+ * Synchronises; FVM_ERR_NOT_CONVERGED when rtol is not met (T is written). */
+int fvm_solve_thermal_system(fvm_ctx *ctx, const fvm_grid *g, const double *coef, const double *rhs, double *T,
+                             double rtol, int32_t maxit, int32_t *iters, double *relres);
+
 /* ---- marker files (checkpoint / restart) ------------------------------------ */
 /* io.c:299-402 saveMarkersBinary(markerset, iMonte, iTime, elapsedTime): writes
  * `path` (the reference names it output/Markers.<iMonte>.<iTime>) in the
