@@ -404,6 +404,7 @@ def bench_vep(a):
                  "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
     }
     out["nodal_step"] = bench_nodal_step(a)
+    out["thermal_solve"] = bench_thermal_solve(a)
     if not a.no_cpu:
         cb = cpu_baseline_vep(w, a.cpu_budget)
         out["cpu_baseline"] = cb
@@ -412,6 +413,46 @@ def bench_vep(a):
         out["cpu_baseline"] = None
     print(json.dumps(out))
     return 0
+
+
+def bench_thermal_solve(a, reps=5):
+    """formThermalSystem + the energy solve (rtol 1e-12) on the 801 x 401
+    tanh-stretched grid with the subduction boundary set-up, against the host's
+    sparse LU (SuperLU) of the same assembled system, as the reference solves it
+    with a direct LU."""
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import MarkerPath
+    from fvmic2d_amd.lib import DeviceArray
+    w = workloads.thermal_box(801, 401)
+    mp = MarkerPath(w)
+    nn = w.NX * w.NY
+    coef = DeviceArray(mp.ctx, np.zeros(5 * nn))
+    rhs = DeviceArray(mp.ctx, w.nodal["heating"])
+    T = DeviceArray(mp.ctx, np.zeros(nn))
+
+    def once():
+        rhs.set(w.nodal["heating"])
+        mp.form_thermal_system(w.dt, False, coef, rhs)
+        return mp.solve_thermal_system(coef, rhs, T, rtol=1e-12, maxit=20000)
+    once()
+    mp.ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        its, rel = once()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    res = {"grid": "801x401 tanh-stretched", "nodes": nn, "gpu_ms": ms, "iterations": its, "relres": rel}
+    if not a.no_cpu:
+        import scipy.sparse.linalg as sla
+        from oracle import thermal as TH
+        A, b = TH.thermal_system(w.x, w.y, w.nodal, w.materials.materialRho[0], w.options, w.dt, False,
+                                 w.nodal["heating"])
+        Ac = A.tocsc()
+        t0 = time.perf_counter()
+        ref = sla.spsolve(Ac, b)
+        res["cpu_direct_ms"] = (time.perf_counter() - t0) * 1e3
+        res["max_rel_diff_vs_direct"] = float(np.abs(T.get().ravel() - ref).max() / np.abs(ref).max())
+    print("thermal solve: %.3f ms on the GPU (%d iterations)" % (ms, its), file=sys.stderr, flush=True)
+    return res
 
 
 # algorithmic bytes per node of the nodal step: retrieveSolutions (read S 24,
