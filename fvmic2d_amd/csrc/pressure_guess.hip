@@ -212,7 +212,10 @@ __global__ void __launch_bounds__(BCG_BS) k_bcg_setup_sys(int64_t n, const doubl
 __global__ void __launch_bounds__(BCG_BS) k_bcg_p(int64_t n, int nblk, int first, BcgVecs V) {
   const double rho = partial_total(V.prho, nblk);
   const double omega = V.sc[S_OMEGA];
-  const double beta = first ? 0.0 : (rho / V.sc[S_RHO_OLD]) * (V.sc[S_ALPHA] / omega);
+  // after an exact solve (r = 0) rho_old and omega are 0: restart with
+  // beta = 0 instead of forming 0/0, so x stays the solution
+  const double rho_old = V.sc[S_RHO_OLD];
+  const double beta = (first || rho_old == 0.0 || omega == 0.0) ? 0.0 : (rho / rho_old) * (V.sc[S_ALPHA] / omega);
   const int64_t i = (int64_t)blockIdx.x * BCG_BS + threadIdx.x;
   if (i < n) {
     const double pi = V.r[i] + beta * (V.p[i] - omega * V.v[i]);

@@ -535,8 +535,10 @@ def nodal_box(NX=801, NY=401, LX=2.0, LY=1.0, seed=5, graded=True):
     }
     nodal["muN"][::7, ::5] = 0.0
     nodal["muS"][::5, ::7] = 0.0
-    nodal["vx"][3, 4] = 0.0
-    nodal["rhodot"][2, 2] = 0.0
+    if NY > 3 and NX > 4:
+        nodal["vx"][3, 4] = 0.0
+    if NY > 2 and NX > 2:
+        nodal["rhodot"][2, 2] = 0.0
     mk = _new_markers(0)
     opt = abi.default_options()
     opt.displacementStepLimit = 0.1

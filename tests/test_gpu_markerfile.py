@@ -75,3 +75,13 @@ def test_refusals(tmp_path):
     small = MarkerPath(w_small, capacity=3)
     with pytest.raises(FvmError):
         small.restart_from_markers(tmp_path / "f")
+
+
+def test_empty_and_all_inactive_sets(tmp_path):
+    w = _workload()
+    w.markers["cellX"][:] = -1
+    mp = MarkerPath(w, capacity=w.n_markers + 3)
+    assert mp.save_markers_binary(tmp_path / "none", 4.0) == 0
+    assert open(tmp_path / "none", "rb").read() == np.int32(0).tobytes() + np.float64(4.0).tobytes()
+    assert mp.restart_from_markers(tmp_path / "none") == 4.0
+    assert mp.markers.n == 0

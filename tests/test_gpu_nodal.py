@@ -101,3 +101,23 @@ def test_nodal_step_in_sequence_at_full_size():
     assert mp.limit_displacement_timestep(w.dt) == d_ref
     for n in OUT + ["p"]:
         assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 2), (2, 5), (3, 3)])
+def test_nodal_step_minimal_grids(shape):
+    """Grids with no interior node (heating, eii, sii all zero) or a single one."""
+    w = _case(shape, False, 1)
+    st, mp = oracle.State(w), MarkerPath(w)
+    h_ref = np.zeros((w.NY, w.NX))
+    st.nodal_stress_strain(w.dt, h_ref)
+    h = DeviceArray(mp.ctx, np.full((w.NY, w.NX), 3.0))
+    mp.nodal_stress_strain(w.dt, h)
+    for n in OUT:
+        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
+    assert_close(h.get().reshape(w.NY, w.NX), h_ref, "nodalHeating", scale=field_scale(h_ref))
+    S = np.random.default_rng(1).normal(size=3 * w.NX * w.NY)
+    S_ref = S.copy()
+    st.retrieve_solutions(S_ref, 2.0)
+    mp.retrieve_solutions(DeviceArray(mp.ctx, S), 2.0)
+    assert_bitwise(mp.get_nodal("p"), st.nodal["p"], "p")
+    assert mp.limit_displacement_timestep(1e30) == st.limit_displacement_timestep(1e30)

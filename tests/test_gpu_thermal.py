@@ -61,3 +61,18 @@ def test_top_type_without_row_is_an_error():
     nn = w.NX * w.NY
     with pytest.raises(FvmError):
         mp.form_thermal_system(w.dt, False, DeviceArray(mp.ctx, np.zeros(5 * nn)), DeviceArray(mp.ctx, np.zeros(nn)))
+
+
+@pytest.mark.parametrize("shape,top", [((3, 3), 0), ((3, 4), 1), ((5, 3), 0)])
+def test_thermal_minimal_grids(shape, top):
+    w, mp, coef, rhs, A, b = _run(shape, top, False)
+    bands = TH.five_bands(A, w.NX, w.NY)
+    got = coef.get().reshape(5, -1)
+    for r in range(5):
+        scale = np.abs(bands[r]).max() or 1.0
+        assert np.abs(got[r] - bands[r]).max() <= 1e-14 * scale
+    assert np.abs(rhs.get().ravel() - b).max() <= 1e-14 * np.abs(b).max()
+    T = DeviceArray(mp.ctx, np.zeros(w.NX * w.NY))
+    mp.solve_thermal_system(coef, rhs, T, rtol=1e-12, maxit=1000)
+    ref = sla.spsolve(A.tocsc(), b)
+    assert np.abs(T.get().ravel() - ref).max() <= 1e-8 * np.abs(ref).max()
