@@ -506,6 +506,35 @@ int saveMarkersBinary(MarkerSet *markerset, int32_t iMonte, int32_t iTime, doubl
   return 0;
 }
 
+/* io.c:150-296 with format OUTPUT_NODAL_FIELDS_BINARY (the only one written
+ * here; the ASCII/VTK viewers are PETSc's own): output/loadNodalFields_<iMonte>_<iTime>.petscbin */
+int saveNodalFields(NodalFields *nf, GridData *grid, int32_t iMonte, int32_t iTime, double dt, double elapsedTime,
+                    int32_t format) {
+  (void)dt;
+  (void)elapsedTime;
+  if (format != FVM_OUTPUT_NODAL_FIELDS_BINARY) {
+    fprintf(stderr, "fvm_compat: saveNodalFields writes the binary format only\n");
+    g_last_status = FVM_ERR_ARG;
+    return FVM_ERR_ARG;
+  }
+  char fn[80];
+  snprintf(fn, sizeof(fn), "output/loadNodalFields_%d_%d.petscbin", iMonte, iTime);
+  TRY(begin(grid, NULL));
+  NodalArrays d;
+  memset(&d, 0, sizeof(d));
+  Vec src[20] = {nf->vx,   nf->vy,   nf->p,    nf->rho,    nf->rhodot, nf->etaN,   nf->etaS,
+                 nf->kThermal, nf->lastT, nf->Cp, nf->muN, nf->muS,    nf->soxx,   nf->soyy,
+                 nf->soxy, nf->edotxx, nf->edotyy, nf->edotxy, nf->wxy, nf->ha};
+  double **dst[19] = {&d.vx,   &d.vy,   &d.p,    &d.rho,    &d.rhodot, &d.etaN,   &d.etaS,
+                      &d.kThermal, &d.lastT, &d.Cp, &d.muN, &d.muS,    &d.soxx,   &d.soyy,
+                      &d.soxy, &d.edotxx, &d.edotyy, &d.edotxy, &d.wxy};
+  for (int k = 0; k < 19; k++) TRY(put(grid, k, src[k], dst[k]));
+  double *ha = NULL;
+  if (src[19]) TRY(put(grid, 19, src[19], &ha));
+  TRY(fvm_save_nodal_fields(g_ctx, &g_grid, &d, ha, fn));
+  return 0;
+}
+
 /* Allocates markerset->markers (release it with free()) with room for
  * nMark * options->maxMarkFactor markers, as restart.c:45 does. */
 int restartFromMarkers(MarkerSet *markerset, GridData *grid, Materials *materials, Options *options, int32_t iMonte,

@@ -97,7 +97,77 @@ __global__ void __launch_bounds__(IO_BLOCK) k_unpack(int32_t n, int32_t cap, con
   }
 }
 
+// ---- nodal fields in PETSc's binary Vec format (io.c:150-296, -output_binary) ----
+// 20 VecView records back to back, in saveNodalFields' order; each record is the
+// big-endian int32 class id 1211214, the big-endian int32 length and the values
+// as big-endian doubles in the DMDA's natural order ([jy][ix], ix fastest).
+#define NNODAL 20
+#define PETSC_VEC_CLASSID 1211214
+struct NodalCols {
+  const double *src[NNODAL];  // NULL writes zeros
+};
+
+__global__ void __launch_bounds__(IO_BLOCK) k_nodal_image(int64_t nn, NodalCols nc, uint32_t *__restrict__ img) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const int64_t rec = 2 + 2 * nn;  // words per record
+#pragma unroll 4
+  for (int k = 0; k < NNODAL; k++) {
+    uint32_t *r = img + k * rec;
+    if (i == 0) {
+      r[0] = __builtin_bswap32((uint32_t)PETSC_VEC_CLASSID);
+      r[1] = __builtin_bswap32((uint32_t)nn);
+    }
+    const unsigned long long u =
+        nc.src[k] ? (unsigned long long)__double_as_longlong(nc.src[k][i]) : 0ull;
+    r[2 + 2 * i] = __builtin_bswap32((uint32_t)(u >> 32));
+    r[3 + 2 * i] = __builtin_bswap32((uint32_t)u);
+  }
+}
+
 }  // namespace
+
+extern "C" int fvm_save_nodal_fields(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, const double *ha,
+                                     const char *path) {
+  if (!ctx || !nf || !path) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  const int64_t nn = (int64_t)g->NX * g->NY;
+  if (nn > 0x7fffffff) return fvm_set_err(ctx, FVM_ERR_ARG, "PETSc binary Vec length limited to int32");
+  // saveNodalFields' order (io.c:212-235)
+  const double *cols[NNODAL] = {nf->vx,   nf->vy,     nf->p,      nf->rho,    nf->rhodot, nf->etaN,   nf->etaS,
+                                nf->kThermal, nf->lastT, nf->Cp,   nf->muN,    nf->muS,    nf->soxx,   nf->soyy,
+                                nf->soxy, nf->edotxx, nf->edotyy, nf->edotxy, nf->wxy,    ha};
+  NodalCols nc;
+  for (int k = 0; k < NNODAL; k++) {
+    if (!cols[k] && k < NNODAL - 1)
+      return fvm_set_err(ctx, FVM_ERR_ARG, "saveNodalFields: nodal field %d missing", k);
+    nc.src[k] = cols[k];
+  }
+  const size_t bytes = (size_t)NNODAL * (8 + 8 * (size_t)nn);
+  FVM_TRY(fvm_ws(ctx, ctx->staging, bytes));
+  {
+    FVM_PHASE(ctx, FVM_PHASE_OTHER);
+    k_nodal_image<<<fvm_blocks(nn, IO_BLOCK), IO_BLOCK, 0, ctx->stream>>>(nn, nc, (uint32_t *)ctx->staging.ptr);
+    FVM_LAUNCH_CHECK(ctx);
+  }
+  char *host = nullptr;
+  FVM_HIP(ctx, hipHostMalloc((void **)&host, bytes, hipHostMallocDefault));
+  int rc = FVM_OK;
+  hipError_t e = hipMemcpyAsync(host, ctx->staging.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) rc = fvm_set_err(ctx, FVM_ERR_HIP, "nodal file copy: %s", hipGetErrorString(e));
+  if (rc == FVM_OK) {
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+      rc = fvm_set_err(ctx, FVM_ERR_IO, "cannot open %s for writing", path);
+    } else {
+      const bool ok = fwrite(host, 1, bytes, f) == bytes;
+      if (fclose(f) != 0 || !ok) rc = fvm_set_err(ctx, FVM_ERR_IO, "short write to %s", path);
+    }
+  }
+  (void)hipHostFree(host);
+  return rc;
+}
 
 extern "C" int fvm_save_markers_binary(fvm_ctx *ctx, const fvm_markers *mk, const char *path, double elapsedTime,
                                        int32_t *nSaved) {

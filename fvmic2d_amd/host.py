@@ -155,6 +155,11 @@ class MarkerPath:
         self.call("fvm_save_markers_binary", self.markers.ref, str(path).encode(), elapsed_time, C.byref(n))
         return n.value
 
+    def save_nodal_fields(self, path, ha=None):
+        """saveNodalFields in PETSc's binary Vec format; ha a device array or None."""
+        self.call("fvm_save_nodal_fields", self.grid.ref, self.nodal.ref,
+                  C.c_void_p(ha.ptr) if ha is not None else None, str(path).encode())
+
     def restart_from_markers(self, path):
         """restartFromMarkers: returns the elapsed time stored in the file."""
         t = C.c_double()

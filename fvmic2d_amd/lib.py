@@ -98,6 +98,7 @@ _SIGS = {
     "fvm_form_thermal_system": [P, P, P, P, P, D, I32, P, P],
     "fvm_solve_thermal_system": [P, P, P, P, P, D, I32, C.POINTER(I32), C.POINTER(D)],
     "fvm_restart_from_markers": [P, P, C.c_char_p, C.POINTER(D)],
+    "fvm_save_nodal_fields": [P, P, P, P, C.c_char_p],
     "fvm_step_thermal": [P, P, P, P, P, P, P, D, I32, P],
     "fvm_step_vep": [P, P, P, P, P, P, P, P, D, C.POINTER(I32), C.POINTER(D)],
     "fvm_profile_enable": [P, I32],

@@ -269,6 +269,17 @@ int fvm_save_markers_binary(fvm_ctx *ctx, const fvm_markers *mk, const char *pat
  * differently sized file.  Synchronises. */
 int fvm_restart_from_markers(fvm_ctx *ctx, fvm_markers *mk, const char *path, double *elapsedTime);
 
+/* io.c:150-296 saveNodalFields(..., OUTPUT_NODAL_FIELDS_BINARY): writes `path`
+ * (the reference names it output/loadNodalFields_<iMonte>_<iTime>.petscbin) as 20
+ * PETSc binary Vec records in the reference's order -- vx, vy, p, rho, rhodot,
+ * etaN, etaS, kThermal, lastT, Cp, muN, muS, soxx, soyy, soxy, edotxx, edotyy,
+ * edotxy, wxy, ha -- each the big-endian int32 class id 1211214, the big-endian
+ * int32 length NX*NY and the values as big-endian doubles in natural order
+ * ([jy][ix]).  ha (adiabatic heating, device NX*NY) may be NULL: zeros are
+ * written.  PETSc's companion .info file is not written.  Synchronises. */
+int fvm_save_nodal_fields(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, const double *ha,
+                          const char *path);
+
 /* ---- whole marker step, in the reference's call order (main_subduction.c) -- */
 /* Thermal-convection step: findMarkerCells (:243), density loop (:255-264),
  * the projection list (or projectMarkersNodesAll2 when list == NULL, :268),

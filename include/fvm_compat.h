@@ -108,6 +108,12 @@ int saveMarkersBinary(MarkerSet *markerset, int32_t iMonte, int32_t iTime, doubl
 int restartFromMarkers(MarkerSet *markerset, GridData *grid, Materials *materials, Options *options,
                        int32_t iMonte, int32_t iTime, double *elapsedTime);
 
+/* io.h saveNodalFields (io.c:150-296): the binary format (fdcode.h's
+ * OUTPUT_NODAL_FIELDS_BINARY) only; writes output/loadNodalFields_<iMonte>_<iTime>.petscbin */
+#define FVM_OUTPUT_NODAL_FIELDS_BINARY 0 /* fdcode.h:249 */
+int saveNodalFields(NodalFields *nodalFields, GridData *grid, int32_t iMonte, int32_t iTime, double dt,
+                    double elapsedTime, int32_t format);
+
 /* Release the GPU state the wrappers keep (context, device copies). */
 void fvm_compat_finalize(void);
 /* Status of the last wrapper call that has no status return (findMarkerCells). */

@@ -195,3 +195,20 @@ def test_compat_marker_files(L, tmp_path, monkeypatch):
     libc.free.argtypes = [C.c_void_p]
     libc.free(ms.markers)
     L.fvm_compat_finalize()
+
+
+def test_compat_save_nodal_fields(L, tmp_path, monkeypatch):
+    from test_gpu_markerfile import NODAL_FILE_ORDER, read_petsc_vecs
+    monkeypatch.chdir(tmp_path)
+    (tmp_path / "output").mkdir()
+    w = workloads.nodal_box(19, 11)
+    rng = np.random.default_rng(4)
+    for n in NODAL_FILE_ORDER:
+        w.nodal[n] = rng.normal(size=(w.NY, w.NX))
+    hp = HostProblem(w)
+    _rc(L.saveNodalFields(C.byref(hp.nf), C.byref(hp.grid), 1, 8, C.c_double(1.0), C.c_double(2.0), 0))
+    vecs = read_petsc_vecs(tmp_path / "output" / "loadNodalFields_1_8.petscbin")
+    for name, v in zip(NODAL_FILE_ORDER, vecs):
+        assert_bitwise(v, hp.arr[name].ravel(), name)
+    assert L.saveNodalFields(C.byref(hp.nf), C.byref(hp.grid), 1, 8, C.c_double(1.0), C.c_double(2.0), 1) != 0
+    L.fvm_compat_finalize()

@@ -85,3 +85,39 @@ def test_empty_and_all_inactive_sets(tmp_path):
     assert open(tmp_path / "none", "rb").read() == np.int32(0).tobytes() + np.float64(4.0).tobytes()
     assert mp.restart_from_markers(tmp_path / "none") == 4.0
     assert mp.markers.n == 0
+
+
+NODAL_FILE_ORDER = ["vx", "vy", "p", "rho", "rhodot", "etaN", "etaS", "kThermal", "lastT", "Cp", "muN", "muS",
+                    "soxx", "soyy", "soxy", "edotxx", "edotyy", "edotxy", "wxy", "ha"]
+
+
+def read_petsc_vecs(path):
+    """PETSc's binary Vec records as PetscBinaryRead reads them: big-endian int32
+    class id (1211214 for a Vec), big-endian int32 length, big-endian doubles."""
+    raw = open(path, "rb").read()
+    off, out = 0, []
+    while off < len(raw):
+        cid, n = np.frombuffer(raw, ">i4", 2, off)
+        assert cid == 1211214
+        out.append(np.frombuffer(raw, ">f8", int(n), off + 8).astype(np.float64))
+        off += 8 + 8 * int(n)
+    assert off == len(raw)
+    return out
+
+
+@pytest.mark.parametrize("with_ha", [True, False])
+def test_nodal_fields_petsc_binary(tmp_path, with_ha):
+    w = workloads.nodal_box(37, 23)
+    rng = np.random.default_rng(2)
+    for n in NODAL_FILE_ORDER[:-1]:
+        if n not in w.nodal:
+            w.nodal[n] = rng.normal(size=(w.NY, w.NX))
+    mp = MarkerPath(w)
+    ha = rng.normal(size=(w.NY, w.NX))
+    from fvmic2d_amd.lib import DeviceArray
+    mp.save_nodal_fields(tmp_path / "n.petscbin", DeviceArray(mp.ctx, ha) if with_ha else None)
+    vecs = read_petsc_vecs(tmp_path / "n.petscbin")
+    assert len(vecs) == 20
+    for name, v in zip(NODAL_FILE_ORDER, vecs):
+        want = (ha if with_ha else np.zeros_like(ha)) if name == "ha" else mp.get_nodal(name)
+        assert_bitwise(v, np.asarray(want).ravel(), name)
