@@ -50,7 +50,7 @@ def algorithmic_bytes(w):
     }
 
 
-STEP_BYTES_PER_MARKER = 268  # P1 60 + P2 32 + P3 176
+STEP_BYTES_PER_MARKER = 268  # locate + projections 60, subgrid T pass 1 32, pass 2 + RK4 176
 STEP_BYTES_PER_NODE = 224
 
 
@@ -199,7 +199,7 @@ def traffic_per_launch(kernel):
 
 def bench_learned_guess(a):
     """The learned Stokes initial guess: plain-method data, U-Net training on the
-    GPU kernels, and the judge's FGMRES iterations to 1e-8 from the plain,
+    GPU kernels, and the host Stokes solve's FGMRES iterations to 1e-8 from the plain,
     previous-solution and learned guesses on held-out states."""
     from fvmic2d_amd import learned_guess as lg
     N = a.lg_grid
@@ -241,7 +241,7 @@ def bench_learned_guess(a):
         "config": {"workload": "learned_guess_box%d" % N, "grid": "%dx%d" % (N, N),
                    "viscosity_contrast": lg.CONTRAST, "train_states": a.lg_train, "test_states": a.lg_test,
                    "unet": "4 levels, 32 base channels, 3x3 convs", "epochs": a.lg_epochs,
-                   "judge": "FGMRES(100) + u/p field split (LU velocity block, diagonal Schur), rtol 1e-8"},
+                   "stokes_solve": "FGMRES(100) + u/p field split (LU velocity block, diagonal Schur), rtol 1e-8"},
         "learned_guess": summ,
         "train_loss_first_last": [hist[0], hist[-1]],
         "seconds": {"host_solves": t_gen, "gpu_pressure_guesses": t_pg, "training": model.train_seconds,
@@ -536,7 +536,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--learned-guess", action="store_true",
-                    help="train and judge the learned Stokes initial guess (own JSON line)")
+                    help="train and evaluate the learned Stokes initial guess (own JSON line)")
     ap.add_argument("--lg-grid", type=int, default=129)
     ap.add_argument("--lg-train", type=int, default=512)
     ap.add_argument("--lg-test", type=int, default=24)
@@ -548,7 +548,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
-              "and this build has no multi-GPU path.", file=sys.stderr)
+              "and this library has no multi-GPU path.", file=sys.stderr)
         return 2
     if a.copy_calibration:
         return copy_calibration()

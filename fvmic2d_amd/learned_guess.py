@@ -5,15 +5,15 @@
    sit slightly displaced, as between two time steps.
 2. The plain method's own solves give the training targets: the previous and
    current Stokes solutions (host sparse direct solve of the same system the
-   judge iterates on) and the plain pressure guess of both states
+   host Stokes solve iterates on) and the plain pressure guess of both states
    (initialPressureGuess on the GPU).
 3. The U-Net (fvmic2d_amd.unet, hand-written MFMA kernels) maps
    [log10 eta / 4, (rho - rho0) / drho, vx_prev, vy_prev, q_prev] to the
    dynamic pressure q = p - p_plain on the node grid (the pressure minus the
    plain pressure guess); inputs and output are scaled by dataset constants.
    The guess is p = p_plain + q with the velocities that balance it: one
-   solve with the velocity block the judge's preconditioner factors anyway.
-4. The judge: FGMRES + field split to a 1e-8 relative residual (host), from
+   solve with the velocity block the Stokes solve's preconditioner factors anyway.
+4. The host Stokes solve: FGMRES + field split to a 1e-8 relative residual (host), from
    the plain guess as the reference uses it (zero velocity, pressure from
    initialPressureGuess), from the plain pressure with its balancing
    velocities, from the previous solution, and from the learned guess -- which
@@ -90,7 +90,7 @@ def _worker(args):
 
 def pressure_plain(ctx, grid, rho_dev, p_dev, Kc, N):
     """initialPressureGuess on the GPU, scaled to the Stokes unknown (p / Kcont)
-    and re-pinned to 0 in cell (1, 1) like the judge's system."""
+    and re-pinned to 0 in cell (1, 1) like the Stokes solve's system."""
     from .host import initial_pressure_guess
     initial_pressure_guess(ctx, grid, rho_dev, G, rtol=1e-10, maxit=50000, p_out=p_dev)
     p = p_dev.get().reshape(N, N) / Kc

@@ -1,11 +1,11 @@
-"""Host-side Stokes solve: the judge of the learned initial guess.
+"""Host-side Stokes solve: the measure of the learned initial guess.
 
 The reference solves its visco-elasto-plastic Stokes system on the host with a
 PETSc KSP (options prefix "stokes_", kspLinearSolve.c:13-97).  This module is
 that host step for a variable-viscosity free-slip box: the system is assembled
 from vep_system.c's stencils -- viscoelastic factor Z (:58-92), Kbond/Kcont
 scaling (:115-123), continuity and pressure ghost rows (:240-275), interior
-x-Stokes (:325-411) and y-Stokes (:445-533) stencils -- with this build's own
+x-Stokes (:325-411) and y-Stokes (:445-533) stencils -- with this code's own
 box boundaries (vx = 0 on the side walls, vy = 0 on top and bottom, mirrored
 tangential velocity on the ghost rows, pressure pinned in one cell) in place
 of the subduction-specific kinematic constraints.  Unknowns are ordered
@@ -18,7 +18,7 @@ velocity block solved exactly (sparse LU) and the Schur complement
 approximated by the diagonal -Kcont^2 / etaN -- the analogue of
 -stokes_ksp_type fgmres -stokes_pc_type fieldsplit with a nonzero initial
 guess.  Iteration counts to rtol = 1e-8 are the figure the learned guess is
-judged by.
+measured by.
 """
 import numpy as np
 import scipy.sparse as sp

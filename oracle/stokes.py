@@ -12,7 +12,7 @@ baseline legs, never by the product path.
 - stokes_system: the generic free-slip box built from vep_system.c's stencils:
   the viscoelastic viscosity factor Z (:58-92), Kbond/Kcont (:115-123), the
   continuity rows and pressure ghost rows (:240-275), the interior x-Stokes
-  (:325-411) and y-Stokes (:445-533) stencils.  Boundaries are this build's own
+  (:325-411) and y-Stokes (:445-533) stencils.  Boundaries are this code's own
   free-slip box (the reference's subduction-specific kinematic constraints,
   open right/bottom boundaries and plate-depth pressure pin are not part of
   it): vx = 0 on the left/right walls, vy = 0 on the top/bottom walls, mirrored

@@ -1,5 +1,5 @@
 """The learned Stokes guess end to end at a small size: data from the plain
-method's solves, U-Net training by the GPU kernels, and the judge (FGMRES +
+method's solves, U-Net training by the GPU kernels, and the host Stokes solve (FGMRES +
 field split) started from the plain, previous and learned guesses."""
 import numpy as np
 import pytest

@@ -1,4 +1,4 @@
-"""The host Stokes judge (fvmic2d_amd/stokes.py) against the CPU restatement
+"""The host Stokes solve (fvmic2d_amd/stokes.py) against the CPU restatement
 (oracle/stokes.py): identical assembled systems, and FGMRES + field split
 converging to the direct solution."""
 import numpy as np
