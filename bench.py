@@ -583,6 +583,25 @@ def main():
     active = n0 - nout
     value = active * a.steps / (ms / 1e3)
 
+    # end-to-end: every step also takes the host solves' outputs in (vx, vy, wxy,
+    # thisT host-to-device) and hands the 17 projected nodal fields back
+    # (device-to-host), as a driver with host-side PETSc solves would
+    host_in = {n: mp.get_nodal(n) for n in ("vx", "vy", "wxy", "thisT")}
+    outs = [p[4] for p in w.projections]
+    e2e_steps = min(a.steps, 20)
+    mp.ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(e2e_steps):
+        for n, v in host_in.items():
+            mp.set_nodal(n, v)
+        mp.step_thermal()
+        for n in outs:
+            mp.get_nodal(n)
+    e2e_ms = (time.perf_counter() - t0) / e2e_steps * 1e3
+    nodes = w.NX * w.NY
+    end_to_end = {"ms_per_step": e2e_ms, "markers_per_s": n0 * 1e3 / e2e_ms, "steps": e2e_steps,
+                  "bytes_h2d_per_step": 4 * nodes * 8, "bytes_d2h_per_step": len(outs) * nodes * 8}
+
     # per-kernel timing pass: every launch bracketed by HIP events on the same stream
     phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_thermal, 5)
     ab = algorithmic_bytes(w)
@@ -606,6 +625,7 @@ def main():
                  "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
                  "wall_ms_per_step": wall / a.steps * 1e3,
                  "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
+        "end_to_end": end_to_end,
     }
     if not a.no_cpu:
         cb = cpu_baseline(a.cpu_budget)
