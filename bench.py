@@ -73,6 +73,38 @@ def cpu_baseline(budget_s=12.0):
             "cpu": platform.processor() or platform.machine()}
 
 
+def _oracle_steps(budget_s):
+    """One worker of the all-core baseline: oracle steps of convection_tdep for
+    budget_s seconds; returns (markers stepped, seconds)."""
+    import oracle
+    from fvmic2d_amd import workloads
+    w = workloads.convection_tdep()
+    st = oracle.State(w, extra=[p[4] for p in w.projections])
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        st.step_thermal()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            return w.n_markers * steps, el
+
+
+def cpu_baseline_all_cores(budget_s=12.0, max_workers=16):
+    """The oracle's step on every host core at once: one independent copy of the
+    workload per process (the reference's ranks each step their own share; this
+    leaves out their communication, so it bounds the reference's multi-rank rate
+    from above).  Forks before this process touches the GPU."""
+    import multiprocessing
+    n = max(1, min(max_workers, os.cpu_count() or 1))
+    with multiprocessing.get_context("fork").Pool(n) as pool:
+        res = pool.map(_oracle_steps, [budget_s] * n)
+    rate = sum(m / t for m, t in res)
+    return {"value": rate, "unit": "markers/s", "cores": n, "kind": "port",
+            "sample": "%d processes, each stepping its own convection_tdep copy (1.44M markers) with the C "
+                      "restatement for %.0f s" % (n, budget_s),
+            "cpu": platform.processor() or platform.machine()}
+
+
 def phase_profile(ctx, step, nsteps):
     """Per-phase ms per step and launches per step, HIP events around every launch."""
     import ctypes as C
@@ -561,6 +593,9 @@ def main():
     if a.vep:
         return bench_vep(a)
 
+    # the all-core CPU figure forks its workers before anything touches the GPU
+    all_cores = None if a.no_cpu else cpu_baseline_all_cores(a.cpu_budget)
+
     from fvmic2d_amd import workloads
     from fvmic2d_amd.host import MarkerPath
 
@@ -631,6 +666,8 @@ def main():
         cb = cpu_baseline(a.cpu_budget)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+        out["cpu_baseline_all_cores"] = all_cores
+        out["speedup_vs_cpu_all_cores"] = value / all_cores["value"]
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
