@@ -428,7 +428,7 @@ def bench_vep(a):
                    "markers_per_cell": 25, "active_markers_after": active, "materials": 5,
                    "plasticity": "Drucker-Prager", "yielding_last_step": y, "residual_last_step": r},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(dom + "_vep"),
                      "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": per_launch_ms},
         "step": {"algorithmic_bytes": step_bytes,
                  "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,

@@ -566,7 +566,12 @@ int restartFromMarkers(MarkerSet *markerset, GridData *grid, Materials *material
   TRY(fvm_restart_from_markers(g_ctx, &g_mk, fn, elapsedTime));
   Marker *m = (Marker *)calloc(cap > 0 ? (size_t)cap : 1, sizeof(Marker));
   if (!m) return FVM_ERR_NOMEM;
-  TRY(fvm_markers_download_aos(g_ctx, &g_mk, m));
+  const int rc = fvm_markers_download_aos(g_ctx, &g_mk, m);
+  if (rc != FVM_OK) {
+    free(m);
+    report(rc);
+    return rc;
+  }
   for (int32_t i = n; i < cap; i++) { /* resetMarker (markers.c:250-330) */
     m[i].X = -1.0;
     m[i].Y = -1.0;
