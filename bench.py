@@ -435,6 +435,7 @@ def bench_vep(a):
                  "wall_ms_per_step": wall / a.steps * 1e3,
                  "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
     }
+    out["other_entry_points"] = bench_other_entry_points(mp, w)
     out["nodal_step"] = bench_nodal_step(a)
     out["thermal_solve"] = bench_thermal_solve(a)
     if not a.no_cpu:
@@ -445,6 +446,36 @@ def bench_vep(a):
         out["cpu_baseline"] = None
     print(json.dumps(out))
     return 0
+
+
+def bench_other_entry_points(mp, w, reps=10):
+    """The entry points the step does not call, timed on the same 8M-marker state
+    (HIP events on the library's stream, ms per call): projectMarkersNodesFromScalar
+    (the marker eta column onto the basic nodes), projectNodalFieldToMarkersS (the
+    nodal lastT onto marker T) and the step glue's max T / max |sxx| (each reads its
+    result back)."""
+    res = {}
+    scalar = mp.markers.col("eta")
+
+    def timed(name, fn):
+        fn()
+        mp.ctx.sync()
+        mp.ctx.timer_start()
+        for _ in range(reps):
+            fn()
+        res[name] = mp.ctx.timer_stop() / reps
+    timed("projectMarkersNodesFromScalar_ms", lambda: mp.project_from_scalar(scalar, "etaS"))
+    timed("projectNodalFieldToMarkersS_ms", lambda: mp.project_nodal_to_markers("lastT", "T"))
+    timed("maxMarkerT_ms", mp.max_marker_T)
+    timed("maxAbsSxx_ms", mp.max_abs_sxx)
+    n = w.n_markers
+    # algorithmic bytes: scalar projection reads X, Y, the scalar column and the
+    # cells (28 B per marker); nodal-to-markers reads X, Y, cells, writes T (32 B);
+    # the max passes read cellX and one column (12 B)
+    res["achieved_GBs"] = {"projectMarkersNodesFromScalar": n * 28 / (res["projectMarkersNodesFromScalar_ms"] * 1e6),
+                           "projectNodalFieldToMarkersS": n * 32 / (res["projectNodalFieldToMarkersS_ms"] * 1e6),
+                           "maxMarkerT": n * 12 / (res["maxMarkerT_ms"] * 1e6)}
+    return res
 
 
 def bench_thermal_solve(a, reps=5):
