@@ -652,21 +652,43 @@ def main():
     # end-to-end: every step also takes the host solves' outputs in (vx, vy, wxy,
     # thisT host-to-device) and hands the 17 projected nodal fields back
     # (device-to-host), as a driver with host-side PETSc solves would
-    host_in = {n: mp.get_nodal(n) for n in ("vx", "vy", "wxy", "thisT")}
+    import ctypes as C
+    in_names = ("vx", "vy", "wxy", "thisT")
     outs = [p[4] for p in w.projections]
-    e2e_steps = min(a.steps, 20)
-    mp.ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(e2e_steps):
-        for n, v in host_in.items():
-            mp.set_nodal(n, v)
-        mp.step_thermal()
-        for n in outs:
-            mp.get_nodal(n)
-    e2e_ms = (time.perf_counter() - t0) / e2e_steps * 1e3
     nodes = w.NX * w.NY
+    # packed exchange: the 4 inputs in one pinned block, one copy each way per step
+    h_in = mp.ctx.pinned((len(in_names), nodes))
+    h_out = mp.ctx.pinned((len(outs), nodes))
+    for k, n in enumerate(in_names):
+        h_in[k] = mp.get_nodal(n).ravel()
+    d_in = (C.c_void_p * len(in_names))(*[mp.nodal.ptr(n) for n in in_names])
+    d_out = (C.c_void_p * len(outs))(*[mp.nodal.ptr(n) for n in outs])
+    e2e_steps = min(a.steps, 20)
+
+    def e2e_loop(packed):
+        mp.ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(e2e_steps):
+            if packed:
+                mp.ctx.call("fvm_nodal_upload", C.c_int64(nodes), len(in_names), C.cast(d_in, C.c_void_p),
+                            h_in.ctypes.data_as(C.c_void_p))
+            else:
+                for k, n in enumerate(in_names):
+                    mp.set_nodal(n, h_in[k].reshape(w.NY, w.NX))
+            mp.step_thermal()
+            if packed:
+                mp.ctx.call("fvm_nodal_download", C.c_int64(nodes), len(outs), C.cast(d_out, C.c_void_p),
+                            h_out.ctypes.data_as(C.c_void_p))
+            else:
+                for n in outs:
+                    mp.get_nodal(n)
+        return (time.perf_counter() - t0) / e2e_steps * 1e3
+    e2e_ms = e2e_loop(True)
+    e2e_each_ms = e2e_loop(False)
     end_to_end = {"ms_per_step": e2e_ms, "markers_per_s": n0 * 1e3 / e2e_ms, "steps": e2e_steps,
-                  "bytes_h2d_per_step": 4 * nodes * 8, "bytes_d2h_per_step": len(outs) * nodes * 8}
+                  "bytes_h2d_per_step": len(in_names) * nodes * 8, "bytes_d2h_per_step": len(outs) * nodes * 8,
+                  "transfer": "pinned, one packed copy each way (fvm_nodal_upload / fvm_nodal_download)",
+                  "ms_per_step_one_copy_per_array": e2e_each_ms}
 
     # per-kernel timing pass: every launch bracketed by HIP events on the same stream
     phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_thermal, 5)

@@ -80,7 +80,7 @@ extern "C" int fvm_ctx_destroy(fvm_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   fvm_buf *bufs[] = {&ctx->key,      &ctx->slot,     &ctx->perm,    &ctx->count,   &ctx->start,
                      &ctx->fill,     &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
-                     &ctx->red,      &ctx->payload,   &ctx->payload_xy};
+                     &ctx->red,      &ctx->payload,   &ctx->payload_xy, &ctx->xfer};
   for (fvm_buf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
@@ -129,6 +129,68 @@ extern "C" int fvm_malloc(void **p, size_t bytes) {
   return hipMalloc(p, bytes ? bytes : 1) == hipSuccess ? FVM_OK : FVM_ERR_NOMEM;
 }
 extern "C" int fvm_free(void *p) { return hipFree(p) == hipSuccess ? FVM_OK : FVM_ERR_HIP; }
+extern "C" int fvm_host_alloc(void **ptr, size_t bytes) {
+  if (!ptr) return FVM_ERR_ARG;
+  return hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault) == hipSuccess ? FVM_OK : FVM_ERR_NOMEM;
+}
+extern "C" int fvm_host_free(void *ptr) {
+  if (ptr) (void)hipHostFree(ptr);
+  return FVM_OK;
+}
+
+// ---- packed nodal exchange with host-side solves ------------------------------
+#define XFER_MAX 32
+struct XferTable {
+  double *p[XFER_MAX];
+};
+// array k of `n` doubles <-> words [k*n, (k+1)*n) of the packed buffer
+__global__ void __launch_bounds__(256) k_unpack_nodal(int64_t n, int32_t na, const double *__restrict__ packed,
+                                                      XferTable t) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < na; k++) t.p[k][i] = packed[k * n + i];
+}
+__global__ void __launch_bounds__(256) k_pack_nodal(int64_t n, int32_t na, XferTable t, double *__restrict__ packed) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < na; k++) packed[k * n + i] = t.p[k][i];
+}
+
+extern "C" int fvm_nodal_upload(fvm_ctx *ctx, int64_t n, int32_t na, double *const *dev, const double *host) {
+  if (!ctx || n < 0 || na < 0 || na > XFER_MAX || (na > 0 && (!dev || !host))) return FVM_ERR_ARG;
+  if (n == 0 || na == 0) return FVM_OK;
+  XferTable t;
+  for (int k = 0; k < na; k++) {
+    if (!dev[k]) return fvm_set_err(ctx, FVM_ERR_ARG, "fvm_nodal_upload: array %d is NULL", k);
+    t.p[k] = dev[k];
+  }
+  FVM_TRY(fvm_ws(ctx, ctx->xfer, (size_t)na * n * sizeof(double)));
+  FVM_HIP(ctx, hipMemcpyAsync(ctx->xfer.ptr, host, (size_t)na * n * sizeof(double), hipMemcpyHostToDevice,
+                              ctx->stream));
+  k_unpack_nodal<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, na, (const double *)ctx->xfer.ptr, t);
+  FVM_LAUNCH_CHECK(ctx);
+  // the host buffer may be reused as soon as this returns
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+extern "C" int fvm_nodal_download(fvm_ctx *ctx, int64_t n, int32_t na, const double *const *dev, double *host) {
+  if (!ctx || n < 0 || na < 0 || na > XFER_MAX || (na > 0 && (!dev || !host))) return FVM_ERR_ARG;
+  if (n == 0 || na == 0) return FVM_OK;
+  XferTable t;
+  for (int k = 0; k < na; k++) {
+    if (!dev[k]) return fvm_set_err(ctx, FVM_ERR_ARG, "fvm_nodal_download: array %d is NULL", k);
+    t.p[k] = const_cast<double *>(dev[k]);
+  }
+  FVM_TRY(fvm_ws(ctx, ctx->xfer, (size_t)na * n * sizeof(double)));
+  k_pack_nodal<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, na, t, (double *)ctx->xfer.ptr);
+  FVM_LAUNCH_CHECK(ctx);
+  FVM_HIP(ctx, hipMemcpyAsync(host, ctx->xfer.ptr, (size_t)na * n * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
 extern "C" int fvm_memcpy_htod(fvm_ctx *ctx, void *dst, const void *src, size_t bytes) {
   FVM_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));

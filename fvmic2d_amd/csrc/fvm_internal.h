@@ -24,6 +24,7 @@ struct fvm_ctx {
   char err[512] = {0};
   // workspace buffers, grown on demand
   fvm_buf key, slot, perm, count, start, fill, blocksums, partials, scratch, staging, red;
+  fvm_buf xfer;  // packed nodal arrays of fvm_nodal_upload / fvm_nodal_download
   fvm_buf payload;     // bucket-ordered copies of the marker columns a projection reads
   fvm_buf payload_xy;  // bucket-ordered X, Y, Mat: valid while xy_serial == sort_serial
   uint64_t sort_serial = 0, xy_serial = 0;

@@ -99,6 +99,10 @@ _SIGS = {
     "fvm_solve_thermal_system": [P, P, P, P, P, D, I32, C.POINTER(I32), C.POINTER(D)],
     "fvm_restart_from_markers": [P, P, C.c_char_p, C.POINTER(D)],
     "fvm_save_nodal_fields": [P, P, P, P, C.c_char_p],
+    "fvm_host_alloc": [C.POINTER(P), C.c_size_t],
+    "fvm_host_free": [P],
+    "fvm_nodal_upload": [P, C.c_int64, I32, P, P],
+    "fvm_nodal_download": [P, C.c_int64, I32, P, P],
     "fvm_step_thermal": [P, P, P, P, P, P, P, D, I32, P],
     "fvm_step_vep": [P, P, P, P, P, P, P, P, D, C.POINTER(I32), C.POINTER(D)],
     "fvm_profile_enable": [P, I32],
@@ -186,6 +190,16 @@ class Context:
     def free(self, p):
         if p:
             self.lib.fvm_free(P(p))
+
+    def pinned(self, shape):
+        """A numpy array over pinned host memory (fvm_host_alloc), freed with the array's owner."""
+        nbytes = int(np.prod(shape)) * 8
+        p = P()
+        self._check(self.lib.fvm_host_alloc(C.byref(p), nbytes))
+        buf = (C.c_double * int(np.prod(shape))).from_address(p.value)
+        arr = np.frombuffer(buf, dtype=np.float64).reshape(shape)
+        self._pinned = getattr(self, "_pinned", []) + [p.value]
+        return arr
 
     def htod(self, dst, arr):
         a = np.ascontiguousarray(arr)

@@ -102,6 +102,16 @@ int fvm_memcpy_htod(fvm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int fvm_memcpy_dtoh(fvm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int fvm_memcpy_dtod(fvm_ctx *ctx, void *dst, const void *src, size_t bytes);
 int fvm_memset(fvm_ctx *ctx, void *dst, int value, size_t bytes);
+/* Pinned (page-locked) host memory, for the packed exchange below. */
+int fvm_host_alloc(void **ptr, size_t bytes);
+int fvm_host_free(void *ptr);
+/* Per-step exchange with host-side solves (no counterpart in the reference, whose
+ * nodal Vecs live on the host): na device arrays of n doubles each, packed back to
+ * back in `host` (array k at host[k*n]).  One host<->device copy and one kernel per
+ * call instead of one copy per array; pass pinned memory from fvm_host_alloc for
+ * full bandwidth.  At most 32 arrays; both synchronise. */
+int fvm_nodal_upload(fvm_ctx *ctx, int64_t n, int32_t na, double *const *dev, const double *host);
+int fvm_nodal_download(fvm_ctx *ctx, int64_t n, int32_t na, const double *const *dev, double *host);
 
 /* Allocate every column of a marker store in one device slab (freed by
  * fvm_markers_free).  Columns are zero-filled, cellX/cellY = -1. */

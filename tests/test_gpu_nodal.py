@@ -121,3 +121,25 @@ def test_nodal_step_minimal_grids(shape):
     mp.retrieve_solutions(DeviceArray(mp.ctx, S), 2.0)
     assert_bitwise(mp.get_nodal("p"), st.nodal["p"], "p")
     assert mp.limit_displacement_timestep(1e30) == st.limit_displacement_timestep(1e30)
+
+
+def test_packed_nodal_exchange():
+    """fvm_nodal_upload / fvm_nodal_download: packed pinned host block <-> separate
+    device arrays, bitwise, and the 32-array limit."""
+    import ctypes as C
+    w = workloads.nodal_box(37, 23)
+    mp = MarkerPath(w)
+    names = ["vx", "vy", "etaN", "muN", "soxx"]
+    n = w.NX * w.NY
+    h = mp.ctx.pinned((len(names), n))
+    h[:] = np.random.default_rng(9).normal(size=h.shape)
+    d = (C.c_void_p * len(names))(*[mp.nodal.ptr(x) for x in names])
+    mp.ctx.call("fvm_nodal_upload", C.c_int64(n), len(names), C.cast(d, C.c_void_p), h.ctypes.data_as(C.c_void_p))
+    for k, x in enumerate(names):
+        assert_bitwise(mp.get_nodal(x).ravel(), h[k], x)
+    out = mp.ctx.pinned((len(names), n))
+    mp.ctx.call("fvm_nodal_download", C.c_int64(n), len(names), C.cast(d, C.c_void_p),
+                out.ctypes.data_as(C.c_void_p))
+    assert_bitwise(out.ravel(), h.ravel(), "round trip")
+    with pytest.raises(FvmError):
+        mp.ctx.call("fvm_nodal_upload", C.c_int64(n), 33, C.cast(d, C.c_void_p), h.ctypes.data_as(C.c_void_p))
