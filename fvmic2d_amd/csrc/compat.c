@@ -32,6 +32,8 @@ static size_t g_pool_cap = 0;      /* doubles per pool buffer */
 static double *g_scalar = NULL;    /* dense per-marker array */
 static size_t g_scalar_cap = 0;
 static int g_last_status = 0;
+static double *g_S = NULL; /* device copy of the interleaved solution vector */
+static size_t g_S_cap = 0;
 
 #define NRES 100 /* residual.c:9 */
 static double g_residuals[NRES];
@@ -412,8 +414,6 @@ int32_t checkConvergence(double atol, double rtol, int32_t iter) {
 }
 
 /* ---- retrieveSolutions.c:4-82 ---------------------------------------------------- */
-static double *g_S = NULL; /* device copy of the interleaved solution vector */
-static size_t g_S_cap = 0;
 int retrieveSolutions(GridData *grid, NodalFields *nf, Vec S, double Kcont, int32_t *foundnan) {
   TRY(begin(grid, NULL));
   const size_t nn = (size_t)grid->NX * grid->NY;
@@ -435,6 +435,30 @@ int retrieveSolutions(GridData *grid, NodalFields *nf, Vec S, double Kcont, int3
   TRY(get(grid, d.vx, nf->vx));
   TRY(get(grid, d.vy, nf->vy));
   TRY(get(grid, d.p, nf->p));
+  return 0;
+}
+
+/* ---- initialPressureGuess.c:6-184 ----------------------------------------------------- */
+/* The reference solves the pressure Poisson problem with the energy_ KSP (a direct LU);
+ * here BiCGStab runs to a relative residual of 1e-12.  Only the pressure entries of S
+ * are written. */
+int initialPressureGuess(GridData *grid, NodalFields *nf, Options *options, Vec S) {
+  TRY(begin(grid, NULL));
+  const size_t nn = (size_t)grid->NX * grid->NY;
+  if (3 * nn > g_S_cap) {
+    if (g_S) fvm_free(g_S);
+    g_S = NULL;
+    g_S_cap = 0;
+    TRY(fvm_malloc((void **)&g_S, 3 * nn * sizeof(double)));
+    g_S_cap = 3 * nn;
+  }
+  double *rho;
+  TRY(put(grid, 0, nf->rho, &rho));
+  TRY(fvm_memcpy_htod(g_ctx, g_S, vec_data(S), 3 * nn * sizeof(double)));
+  int32_t its = 0;
+  double rel = 0.0;
+  TRY(fvm_initial_pressure_guess(g_ctx, &g_grid, rho, options->gy, 0.0, 1e-12, 1000000, g_S, NULL, &its, &rel));
+  TRY(fvm_memcpy_dtoh(g_ctx, vec_data(S), g_S, 3 * nn * sizeof(double)));
   return 0;
 }
 

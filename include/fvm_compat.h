@@ -93,6 +93,8 @@ int updateGlobalStrainRateResidual(GridData *grid, MarkerSet *markerset, int32_t
 double getGlobalStrainRateResidual(int32_t iter);
 int32_t checkConvergence(double atol, double rtol, int32_t iter);
 
+/* initialPressureGuess.h:2: writes the pressure entries of S (a Vec of NX*NY*3) */
+int initialPressureGuess(GridData *grid, NodalFields *nodalFields, Options *options, Vec S);
 /* retrieveSolutions.h: S is the interleaved Stokes solution (a Vec of NX*NY*3) */
 int retrieveSolutions(GridData *grid, NodalFields *nodalFields, Vec S, double Kcont, int32_t *foundnan);
 /* nodalStressStrain.h (nodalStressStrain.c:6-485) */

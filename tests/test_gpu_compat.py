@@ -212,3 +212,26 @@ def test_compat_save_nodal_fields(L, tmp_path, monkeypatch):
         assert_bitwise(v, hp.arr[name].ravel(), name)
     assert L.saveNodalFields(C.byref(hp.nf), C.byref(hp.grid), 1, 8, C.c_double(1.0), C.c_double(2.0), 1) != 0
     L.fvm_compat_finalize()
+
+
+def test_compat_initial_pressure_guess(L):
+    """initialPressureGuess through the reference's prototype: the pressure entries
+    of S match the direct solve of the oracle's system, the velocity entries stay."""
+    import scipy.sparse.linalg as sla
+    from oracle import stokes as OS
+    w = workloads.nodal_box(41, 29)
+    rng = np.random.default_rng(6)
+    w.nodal["rho"] = 3300.0 + 50.0 * rng.normal(size=(w.NY, w.NX))
+    hp = HostProblem(w)
+    n = w.NX * w.NY
+    S = np.full(3 * n, 7.0)
+    Sv = HostVec(3 * n, S.ctypes.data)
+    opt = abi.default_options()
+    opt.gy = 9.81
+    _rc(L.initialPressureGuess(C.byref(hp.grid), C.byref(hp.nf), C.byref(opt), C.byref(Sv)))
+    A, b = OS.pressure_guess_system(w, w.nodal["rho"], 9.81)
+    ref = sla.spsolve(A.tocsc(), b)
+    p = S.reshape(-1, 3)[:, 2]
+    assert np.abs(p - ref).max() <= 1e-8 * np.abs(ref).max()
+    assert (S.reshape(-1, 3)[:, :2] == 7.0).all()
+    L.fvm_compat_finalize()
