@@ -204,6 +204,43 @@ def test_advect_rk4(small, jaumann, bc):
                 b.value[k] = 0.0
 
 
+@pytest.mark.parametrize("case", ["tiny_grid", "tiny_positions", "extreme_velocities"])
+def test_advect_rk4_extreme_magnitudes(case):
+    """RK4 advection stays bit-identical far from the usual magnitudes:
+    tiny_grid scales the whole grid, the markers and dt by 2^-110; tiny_positions
+    puts every seventh marker at |X| < 1e-33 in cell 0; extreme_velocities scales
+    vx, vy by 1e-130 in some columns and by 1e120 in some rows (interpolation
+    numerators near the bottom of the exponent range, then positions thrown far
+    outside the grid)."""
+    w = _with_stress(workloads.convection_small())
+    m = w.markers
+    if case == "tiny_grid":
+        s = 2.0 ** -110
+        for a in ("xp", "yp", "xcp", "ycp"):
+            setattr(w, a, getattr(w, a) * s)
+        w.LX *= s
+        w.LY *= s
+        m["X"] *= s
+        m["Y"] *= s
+        w.dt *= s
+    elif case == "tiny_positions":
+        r = np.random.default_rng(5)
+        sel = np.arange(0, w.n_markers, 7)
+        m["X"][sel] = r.uniform(0.0, 1e-33, len(sel))
+    else:
+        w.nodal["vx"] = w.nodal["vx"].copy()
+        w.nodal["vy"] = w.nodal["vy"].copy()
+        for n in ("vx", "vy"):
+            w.nodal[n][:, 3:9] *= 1e-130
+            w.nodal[n][20:24, :] *= 1e120
+    st, mp = _pair(w)
+    st.find_marker_cells()
+    mp.find_marker_cells()
+    st.advect(w.dt * 1.3, 0)
+    mp.advect(w.dt * 1.3, 0)
+    _cmp_markers(st, mp, exact=("X", "Y", "VX", "VY", "wxy"), what=case + " ")
+
+
 def test_subgrid_temperature(small):
     st, mp = _pair(small)
     for s in (st, mp):
