@@ -358,8 +358,10 @@ int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t r
     int32_t *count = build_order ? (int32_t *)ctx->count.ptr : nullptr;
     const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
     if (lds <= 48 * 1024) {
-      // one tile of 256 markers per LOC_TILE_BYTES of staged grid (1 to 8)
-      const int tiles = std::max(1, std::min(8, (int)(lds / LOC_TILE_BYTES)));
+      // one tile of 256 markers per LOC_TILE_BYTES of staged grid, at least 2
+      // (at 201 x 201, 6.4 KB staged, 2 tiles locate in 37 us against 42 for 1)
+      // and at most 8
+      const int tiles = std::max(2, std::min(8, (int)(lds / LOC_TILE_BYTES)));
       k_locate<true><<<fvm_blocks(n, 256 * tiles), 256, lds, ctx->stream>>>(
           dg, n, mk->X, mk->Y, mk->cellX, mk->cellY, mk->cpu, rank, counters, key, slot, count, dt, dens ? 1 : 0,
           mk->Mat, mk->T, mk->rho, mk->rhodot, tiles);
