@@ -292,9 +292,10 @@ __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ ins
   gl.yc = sgrid + 2 * g.NX + g.NY + 1;
   const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (m >= I.nMark) return;
-  advect_marker(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.cellX,
-                I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, 0, I.T, I.Tdot, deltaT + I.node_off, dTsub + I.mark_off,
-                I.dt);
+  advect_marker(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.cellX, I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, I.T, I.Tdot,
+                deltaT + I.node_off, dTsub + I.mark_off, I.dt);
+  // the rotation fused behind the RK4 stages: the batched launch keeps one kernel per phase
+  rotate_eii_marker(m, g.NX, g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.dt, 0);
 }
 
 // ---- host side ----------------------------------------------------------------------------

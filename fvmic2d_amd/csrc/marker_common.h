@@ -13,13 +13,6 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     double *__restrict__ MVX,
     double *__restrict__ MVY,
     double *__restrict__ Mwxy,
-    double *__restrict__ sxx,
-    double *__restrict__ syy,
-    double *__restrict__ sxy,
-    const double *__restrict__ exx,
-    const double *__restrict__ eyy,
-    const double *__restrict__ exy,
-    double *__restrict__ Eii,
     const int32_t *__restrict__ cellXa,
     const int32_t *__restrict__ cellYa,
     const double *__restrict__ vx,
@@ -27,7 +20,6 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     const double *__restrict__ wxy,
     BoundaryValues bv,
     double dt,
-    int jaumann,
     double *__restrict__ MT,
     double *__restrict__ MTdot,
     const double *__restrict__ dTn,
@@ -185,7 +177,27 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
   Mwxy[m] = wm;
   MX[m] = X0 + dt * VX;
   MY[m] = Y0 + dt * VY;
+}
 
+// ---- A6, second pass: the stress rotation by the marker spin and the strain
+// invariant update (markers.c:2141-2164, 2223), after advect_marker has stored the spin
+// wxy = (k1 + 2 k2 + 2 k3 + k4) / 6.  A pass of its own: the RK4 pass then holds
+// fewer live registers and hides more of its gather latency, and this one streams.
+__device__ __forceinline__ void rotate_eii_marker(int64_t m, int NX, int NY,
+    const int32_t *__restrict__ cellXa,
+    const int32_t *__restrict__ cellYa,
+    const double *__restrict__ Mwxy,
+    double *__restrict__ sxx,
+    double *__restrict__ syy,
+    double *__restrict__ sxy,
+    const double *__restrict__ exx,
+    const double *__restrict__ eyy,
+    const double *__restrict__ exy,
+    double *__restrict__ Eii,
+    double dt,
+    int jaumann) {
+  if (!cell_ok(cellXa[m], cellYa[m], NX, NY)) return;
+  const double wm = Mwxy[m];
   const double msxx0 = sxx[m], msyy0 = syy[m], msxy0 = sxy[m];
   if (jaumann) {
     const double wxy1 = wm * dt;

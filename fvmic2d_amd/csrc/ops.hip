@@ -15,18 +15,16 @@ int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
 // in LDS once per block, and each block walks `tiles` tiles of 256 markers so
 // that the staging is amortised (2 for small grids, one more per ADV_TILE_BYTES
 // of staged grid beyond that, at most 8); the per-marker arithmetic is unchanged.
+// k_rotate_eii then rotates the stresses and updates Eii in a streaming pass.
 #define ADV_TILE_BYTES 4096
 template <bool LDS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_advect_rk4(DGrid g, int32_t n, double *__restrict__ MX, double *__restrict__ MY,
                                                     double *__restrict__ MVX, double *__restrict__ MVY,
-                                                    double *__restrict__ Mwxy, double *__restrict__ sxx,
-                                                    double *__restrict__ syy, double *__restrict__ sxy,
-                                                    const double *__restrict__ exx, const double *__restrict__ eyy,
-                                                    const double *__restrict__ exy, double *__restrict__ Eii,
+                                                    double *__restrict__ Mwxy,
                                                     const int32_t *__restrict__ cellXa, const int32_t *__restrict__ cellYa,
                                                     const double *__restrict__ vx, const double *__restrict__ vy,
                                                     const double *__restrict__ wxy, BoundaryValues bv, double dt,
-                                                    int jaumann, double *__restrict__ MT, double *__restrict__ MTdot,
+                                                    double *__restrict__ MT, double *__restrict__ MTdot,
                                                     const double *__restrict__ dTn, const double *__restrict__ dTsub,
                                                     double dt_thermal, int tiles) {
   if (LDS) {
@@ -46,15 +44,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int t = 0; t < tiles; t++) {
       const int64_t m = m0 + 256 * t;
       if (m < n)
-        advect_marker(m, gl, MX, MY, MVX, MVY, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, cellXa, cellYa, vx, vy, wxy,
-                      bv, dt, jaumann, MT, MTdot, dTn, dTsub, dt_thermal);
+        advect_marker(m, gl, MX, MY, MVX, MVY, Mwxy, cellXa, cellYa, vx, vy, wxy, bv, dt, MT, MTdot, dTn, dTsub,
+                      dt_thermal);
     }
   } else {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
-    advect_marker(m, g, MX, MY, MVX, MVY, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, cellXa, cellYa, vx, vy, wxy, bv,
-                  dt, jaumann, MT, MTdot, dTn, dTsub, dt_thermal);
+    advect_marker(m, g, MX, MY, MVX, MVY, Mwxy, cellXa, cellYa, vx, vy, wxy, bv, dt, MT, MTdot, dTn, dTsub,
+                  dt_thermal);
   }
+}
+
+__global__ void __launch_bounds__(256) k_rotate_eii(int32_t n, int NX, int NY, const int32_t *__restrict__ cellXa,
+                                                    const int32_t *__restrict__ cellYa,
+                                                    const double *__restrict__ Mwxy, double *__restrict__ sxx,
+                                                    double *__restrict__ syy, double *__restrict__ sxy,
+                                                    const double *__restrict__ exx, const double *__restrict__ eyy,
+                                                    const double *__restrict__ exy, double *__restrict__ Eii,
+                                                    double dt, int jaumann) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  rotate_eii_marker(m, NX, NY, cellXa, cellYa, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, dt, jaumann);
 }
 
 static int launch_advect(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const NodalArrays *nf,
@@ -66,15 +76,19 @@ static int launch_advect(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const
   if (lds <= 32 * 1024) {
     const int tiles = std::max(2, std::min(8, (int)(lds / ADV_TILE_BYTES)));
     k_advect_rk4<true><<<fvm_blocks(n, 256 * tiles), 256, lds, ctx->stream>>>(
-        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy,
-        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal,
+        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, T,
+        Tdot, deltaT, dTsub, dt_thermal,
         tiles);
   } else {
     k_advect_rk4<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(
-        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->sxx, mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy,
-        mk->Eii, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, jaumann, T, Tdot, deltaT, dTsub, dt_thermal,
+        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, T,
+        Tdot, deltaT, dTsub, dt_thermal,
         1);
   }
+  FVM_LAUNCH_CHECK(ctx);
+  k_rotate_eii<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, g->NX, g->NY, mk->cellX, mk->cellY, mk->wxy, mk->sxx,
+                                                            mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy, mk->Eii, dt,
+                                                            jaumann);
   FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
