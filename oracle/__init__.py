@@ -88,6 +88,18 @@ class State:
             self.nodal[n] = np.ascontiguousarray(a.copy() if a is not None else np.zeros((w.NY, w.NX)))
         self.nf = self.descriptor()
 
+    def clone(self):
+        """An independent copy (markers and nodal arrays copied, grid shared): the
+        scale computations of the tests run oracle functions on modified copies."""
+        c = State.__new__(State)
+        c.lib, c.w = self.lib, self.w
+        c.markers = self.markers.copy()
+        c.xp, c.yp, c.xcp, c.ycp, c.grid = self.xp, self.yp, self.xcp, self.ycp, self.grid
+        c.ms = abi.MarkerSet(self.ms.nMark, self.ms.maxMark, self.ms.nOut, c.markers.ctypes.data)
+        c.nodal = {k: v.copy() for k, v in self.nodal.items()}
+        c.nf = c.descriptor()
+        return c
+
     def descriptor(self, **alias):
         d = abi.NodalArrays()
         for n in abi.NODAL_NAMES:
@@ -202,42 +214,59 @@ class State:
         return self.lib.orc_max_abs_sxx(C.byref(self.ms))
 
     # whole steps, reference call order (main_subduction.c) --------------
-    def step_thermal(self):
+    # ``hook(stage, state)``, when given, is called between the sub-steps (stage
+    # names: "pre_projection", "projected", "strain_pressure", "subgrid_stress",
+    # "update_stress", "subgrid_t"); the tests use it to derive error scales from
+    # the intermediate states.
+    def step_thermal(self, hook=None):
         w = self.w
+        hook = hook or (lambda stage, st: None)
         self.find_marker_cells()
         self.update_marker_density()
+        hook("pre_projection", self)
         if w.projections == "all2":
             self.project_markers_nodes_all2()
-            nf = self.nf
+            self.thermal_inputs = {"lastT": "lastT", "rho": "rho", "Cp": "Cp", "kThermal": "kThermal"}
         else:
             for src, sx, sy, meth, out in w.projections:
                 self.project_marker_field(src, out, sx, sy, meth)
-            nf = self.descriptor(lastT="T_b", rho="rho_b", Cp="Cp_b", kThermal="k_b")
+            self.thermal_inputs = {"lastT": "T_b", "rho": "rho_b", "Cp": "Cp_b", "kThermal": "k_b"}
+        nf = self.descriptor(**self.thermal_inputs)
+        hook("projected", self)
         self.subgrid_temperature(w.dt, w.options.subgridTemperatureDiffusivity, nf=nf)
+        hook("subgrid_t", self)
         self.advect(w.dt)
 
-    def step_vep(self):
+    def step_vep(self, hook=None):
         """Marker part of one VEP step; host-solve outputs are the workload's nodal inputs
         (kept in nodal[...] under their names, projection outputs under proj_*)."""
         w = self.w
+        hook = hook or (lambda stage, st: None)
         self.find_marker_cells()
         self.update_marker_density()
         y = self.check_plastic_yielding(w.dt)
         self.limit_viscosity()
+        hook("pre_projection", self)
         proj = self.descriptor(rho="proj_rho", etaN="proj_etaN", etaS="proj_etaS",
                                kThermal="proj_kThermal", lastT="proj_lastT", Cp="proj_Cp",
                                soxx="proj_soxx", soyy="proj_soyy", soxy="proj_soxy",
                                rhodot="proj_rhodot", muS="proj_muS", muN="proj_muN")
         self._c("orc_project_markers_nodes_all2", C.byref(self.ms), C.byref(self.grid),
                 C.byref(proj), C.byref(w.materials))
+        hook("projected", self)
         self.update_marker_strain_pressure(w.dt)
         r = self.strain_rate_residual()
+        hook("strain_pressure", self)
         if w.options.subgridStressDiffusivity > 0.0:
             self.subgrid_stress(w.dt, w.options.subgridStressDiffusivity)
+        hook("subgrid_stress", self)
         self.update_marker_stress()
-        th = self.descriptor(lastT="proj_lastT", rho="proj_rho", Cp="proj_Cp",
-                             kThermal="proj_kThermal")
+        hook("update_stress", self)
+        self.thermal_inputs = {"lastT": "proj_lastT", "rho": "proj_rho", "Cp": "proj_Cp",
+                               "kThermal": "proj_kThermal"}
+        th = self.descriptor(**self.thermal_inputs)
         self.subgrid_temperature(w.dt, w.options.subgridTemperatureDiffusivity, nf=th)
+        hook("subgrid_t", self)
         self.advect(w.dt)
         return y, r
 

@@ -41,10 +41,3 @@ def assert_close(gpu, ref, what, rtol=RTOL, scale=None):
     e = rel_err(g, r, scale)
     assert e <= rtol, "%s: max relative error %.3e > %.1e" % (what, e, rtol)
     return e
-
-
-def field_scale(values):
-    """A per-field magnitude: RMS of the finite reference values."""
-    v = np.asarray(values, dtype=np.float64)
-    v = v[np.isfinite(v)]
-    return float(np.sqrt(np.mean(v * v))) if v.size else 0.0

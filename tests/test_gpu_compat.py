@@ -9,7 +9,8 @@ import pytest
 
 import oracle
 from fvmic2d_amd import abi, lib, workloads
-from helpers import assert_bitwise, assert_close, field_scale
+import scales as S
+from helpers import assert_bitwise, assert_close
 
 pytestmark = pytest.mark.gpu
 
@@ -81,13 +82,16 @@ def test_compat_thermal_step(L):
     assert_bitwise(hp.markers["cellX"], st.markers["cellX"], "cellX")
     _rc(L.projectMarkersNodesAll2(C.byref(hp.ms), C.byref(hp.grid), C.byref(hp.nf),
                                   C.byref(w.materials), C.byref(w.options)))
+    sc = S.abs_projections(st, projections="all2")
     st.project_markers_nodes_all2()
     for n in ("rho", "etaN", "etaS", "kThermal", "lastT", "Cp", "soxx", "soyy", "soxy", "muS"):
-        assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
+        assert_close(hp.arr[n], st.nodal[n], n, scale=sc.get(n))
+    t_sc = S.subgrid_t_scales(st, st.markers["T"].copy(), sc["lastT"], w.dt)
     _rc(L.subgridTemperatureChanges(hp.vec("thisT"), C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms),
                                     C.byref(w.materials), C.c_double(w.dt), C.byref(w.options)))
     st.subgrid_temperature(w.dt, w.options.subgridTemperatureDiffusivity)
-    assert_close(hp.markers["T"], st.markers["T"], "T", scale=field_scale(st.markers["T"]))
+    act = st.markers["cellX"] != -1
+    assert_close(hp.markers["T"][act], st.markers["T"][act], "T", scale=t_sc["T"][act])
     _rc(L.advectMarkersRK(C.byref(hp.ms), C.byref(hp.nf), C.byref(hp.grid), C.byref(w.options),
                           C.byref(w.bv), C.c_double(w.dt)))
     st.advect(w.dt)
@@ -113,6 +117,7 @@ def test_compat_vep_functions(L):
     assert y.value == st.check_plastic_yielding(w.dt)
     L.limitViscosity(C.byref(hp.ms), C.byref(w.options), C.byref(w.materials))
     st.limit_viscosity()
+    sc = S.strain_pressure_scales(st, w.dt)
     _rc(L.updateMarkerStrainPressure(C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms),
                                      C.byref(w.materials), C.byref(w.options), C.c_double(w.dt)))
     st.update_marker_strain_pressure(w.dt)
@@ -120,16 +125,21 @@ def test_compat_vep_functions(L):
     L.getGlobalStrainRateResidual.restype = C.c_double
     r = L.getGlobalStrainRateResidual(0)
     assert abs(r - st.strain_rate_residual()) <= 1e-12 * abs(r)
+    s_mag = S.stress_l1(st.markers)
+    inc, ds_mag = S.subgrid_stress_scales(st, w.dt, 1.0)
     _rc(L.subgridStressChanges(C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms), C.byref(w.materials),
                                C.c_double(w.dt), C.c_double(1.0)))
     st.subgrid_stress(w.dt, 1.0)
+    s_mag = s_mag + inc + S.update_stress_scales(st, ds_mag)
     _rc(L.updateMarkerStress(C.byref(hp.grid), C.byref(hp.nf), C.byref(hp.ms), C.byref(w.materials)))
     st.update_marker_stress()
     act = st.markers["cellX"] != -1
+    sc.update({f: s_mag for f in S.STRESS})
     for f in ("eta", "mu", "sxx", "syy", "sxy", "exx", "eyy", "exy", "p", "wxy", "strainRateResidual"):
-        assert_close(hp.markers[f][act], st.markers[f][act], f, scale=field_scale(st.markers[f][act]))
+        sf = sc[f][act] if f in sc else None  # eta, mu: pointwise
+        assert_close(hp.markers[f][act], st.markers[f][act], f, scale=sf)
     for n in ("dsxx", "dsyy", "dsxy"):
-        assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
+        assert_close(hp.arr[n], st.nodal[n], n, scale=ds_mag[n])
     L.fvm_compat_finalize()
 
 
@@ -157,8 +167,8 @@ def test_compat_nodal_step(L):
     h_ref = np.zeros((w.NY, w.NX))
     st.nodal_stress_strain(w.dt, h_ref)
     for n in ("edotxx", "edotyy", "edotxy", "wxy", "soxx", "soyy", "soxy", "dsxx", "dsyy", "dsxy", "eii", "sii"):
-        assert_close(hp.arr[n], st.nodal[n], n, scale=field_scale(st.nodal[n]))
-    assert_close(heat, h_ref, "nodalHeating", scale=field_scale(h_ref))
+        assert_close(hp.arr[n], st.nodal[n], n)  # same expression order: pointwise
+    assert_close(heat, h_ref, "nodalHeating")
     d = C.c_double(w.dt)
     _rc(L.limitDisplacementTimestep(C.byref(hp.grid), C.byref(hp.nf), C.byref(d), C.byref(w.options)))
     assert d.value == st.limit_displacement_timestep(w.dt)

@@ -1,8 +1,9 @@
 """The nodal step between the Stokes solve and the marker updates on the GPU
 (fvm_retrieve_solutions, fvm_nodal_stress_strain, fvm_limit_displacement_timestep)
 against the oracle: strain rates, stresses, heating and invariants to 1e-12
-relative to each field's scale, the retrieved velocities, pressure and the time
-step bitwise."""
+relative entry by entry (the kernels evaluate each node's expressions in the
+reference's order, so no sum is reassociated), the retrieved velocities,
+pressure and the time step bitwise."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ import oracle
 from fvmic2d_amd import workloads
 from fvmic2d_amd.host import MarkerPath
 from fvmic2d_amd.lib import DeviceArray, FvmError
-from helpers import assert_bitwise, assert_close, field_scale
+from helpers import assert_bitwise, assert_close
 
 pytestmark = pytest.mark.gpu
 
@@ -38,8 +39,8 @@ def test_nodal_stress_strain_matches_oracle(shape, graded, bc):
     h = DeviceArray(mp.ctx, np.full((w.NY, w.NX), 5.0))
     mp.nodal_stress_strain(w.dt, h)
     for n in OUT:
-        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
-    assert_close(h.get().reshape(w.NY, w.NX), h_ref, "nodalHeating", scale=field_scale(h_ref))
+        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=None)
+    assert_close(h.get().reshape(w.NY, w.NX), h_ref, "nodalHeating")
 
 
 def test_nodal_stress_strain_rejects_undefined_ghost():
@@ -100,7 +101,7 @@ def test_nodal_step_in_sequence_at_full_size():
     mp.nodal_stress_strain(w.dt, h)
     assert mp.limit_displacement_timestep(w.dt) == d_ref
     for n in OUT + ["p"]:
-        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
+        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=None)
 
 
 @pytest.mark.parametrize("shape", [(2, 2), (3, 2), (2, 5), (3, 3)])
@@ -113,8 +114,8 @@ def test_nodal_step_minimal_grids(shape):
     h = DeviceArray(mp.ctx, np.full((w.NY, w.NX), 3.0))
     mp.nodal_stress_strain(w.dt, h)
     for n in OUT:
-        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=field_scale(st.nodal[n]))
-    assert_close(h.get().reshape(w.NY, w.NX), h_ref, "nodalHeating", scale=field_scale(h_ref))
+        assert_close(mp.get_nodal(n), st.nodal[n], n, scale=None)
+    assert_close(h.get().reshape(w.NY, w.NX), h_ref, "nodalHeating")
     S = np.random.default_rng(1).normal(size=3 * w.NX * w.NY)
     S_ref = S.copy()
     st.retrieve_solutions(S_ref, 2.0)
