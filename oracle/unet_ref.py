@@ -80,15 +80,18 @@ def up_bwd(dY):
     return (dY[:, 0::2, 0::2] + dY[:, 0::2, 1::2]) + (dY[:, 1::2, 0::2] + dY[:, 1::2, 1::2])
 
 
-def forward_backward(params, arch, X, T, mask):
+def forward_backward(params, arch, X, T, mask, exact=False):
     """Loss and gradients of the U-Net (same naming as fvmic2d_amd.unet.UNet).
 
     params: dict name -> (W [tap][Cout][Cin], b [Cout]) in float32 (the fp32
     master values; the forward pass uses their bf16 rounding).  arch: (levels,
-    base channels, input channels, output channels, real output channels)."""
+    base channels, input channels, output channels, real output channels).
+    exact=True drops every rounding point (float64 throughout), which makes the
+    loss a smooth function of the parameters for a finite-difference check."""
     L, c, cin, cout, creal = arch
-    P = {k: (bf16(w).astype(np.float64), b.astype(np.float64)) for k, (w, b) in params.items()}
-    r = lambda a: bf16(a).astype(np.float64)
+    rnd = (lambda a: np.asarray(a, np.float64)) if exact else bf16
+    P = {k: (rnd(w).astype(np.float64), b.astype(np.float64)) for k, (w, b) in params.items()}
+    r = lambda a: rnd(a).astype(np.float64)
     acts = {}
     x = r(X)
     enc = []
@@ -106,7 +109,9 @@ def forward_backward(params, arch, X, T, mask):
         h1 = r(conv(cat, *P["d%d_1" % i], 3, True))
         d = r(conv(h1, *P["d%d_2" % i], 3, True))
         dec[i] = (cat, h1, d)
-    out = conv(d, *P["out"], 1, False).astype(np.float32).astype(np.float64)
+    out = conv(d, *P["out"], 1, False)
+    if not exact:
+        out = out.astype(np.float32).astype(np.float64)
     npix = X.shape[0] * X.shape[1] * X.shape[2]
     diff = (out[..., :creal] - T) * mask
     inv = 1.0 / (npix * creal)
