@@ -1,4 +1,10 @@
-"""Build libfvmic2d_hip.so (HIP kernels for gfx950 + C host layer) in-tree.
+"""Build the libraries in-tree:
+
+* libfvmic2d_hip.so    -- HIP kernels for gfx950 and the C ABI (fvm_abi.h);
+* libfvmic2d_compat.so -- the reference-named wrappers (fvm_compat.h, csrc/compat.c)
+                          over the stand-in host Vec, linked against the first.
+                          A PETSc build compiles compat.c itself with
+                          -DFVM_WITH_PETSC and links libfvmic2d_hip.so only.
 
     python -m fvmic2d_amd.build            # library
     python -m fvmic2d_amd.build --oracle   # library + the CPU oracle (test checker)
@@ -20,6 +26,7 @@ CSRC = os.path.join(PKG, "csrc")
 INC = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfvmic2d_hip.so")
+COMPAT_LIB = os.path.join(PKG, "libfvmic2d_compat.so")
 ARCH = os.environ.get("FVM_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -39,22 +46,25 @@ def _run(cmd):
 def build_library(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hips = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
-    cs = sorted(f for f in os.listdir(CSRC) if f.endswith(".c"))
     jobs = []
     for f in hips:
         jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, f), "-o",
                                            os.path.join(BUILD, f + ".o")])
-    for f in cs:
-        jobs.append(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, f), "-o",
-                                         os.path.join(BUILD, f + ".o")])
+    compat_o = os.path.join(BUILD, "compat.c.o")
+    jobs.append(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, "compat.c"), "-o", compat_o])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and r.stderr.strip():
                 print(r.stderr)
-    objs = [os.path.join(BUILD, f + ".o") for f in hips + cs]
+    objs = [os.path.join(BUILD, f + ".o") for f in hips]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-Wl,-soname,libfvmic2d_hip.so", "-o", tmp]
+         + objs + ["-lm"])
     os.replace(tmp, LIB)
+    tmp = COMPAT_LIB + ".tmp"
+    _run(["gcc", "-shared", "-fPIC", "-o", tmp, compat_o, "-L" + PKG, "-lfvmic2d_hip",
+          "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, COMPAT_LIB)
     return LIB
 
 

@@ -43,14 +43,35 @@ static int ensure_ctx(void) {
   return fvm_ctx_create(&g_ctx, NULL);
 }
 
-static double *vec_data(Vec v) {
+/* A Vec's host array is checked out only around the copy that uses it.  With
+ * PETSc, reads go through VecGetArrayRead and writes hold VecGetArray until the
+ * device-to-host copy has completed (fvm_memcpy_dtoh synchronises); the
+ * VecRestoreArray that follows advances the Vec's object state, so PETSc drops
+ * its cached norms and extrema of the old values.  One rank: the array holds
+ * every node in natural order. */
+static int vec_to_device(double *dev, Vec v, size_t n) {
+#ifdef FVM_WITH_PETSC
+  const PetscScalar *a = NULL;
+  if (VecGetArrayRead(v, &a)) return FVM_ERR_ARG;
+  int rc = fvm_memcpy_htod(g_ctx, dev, a, n * sizeof(double));
+  VecRestoreArrayRead(v, &a);
+  return rc;
+#else
+  if (!v || !v->a || (size_t)v->n < n) return FVM_ERR_ARG;
+  return fvm_memcpy_htod(g_ctx, dev, v->a, n * sizeof(double));
+#endif
+}
+
+static int vec_from_device(Vec v, const double *dev, size_t n) {
 #ifdef FVM_WITH_PETSC
   PetscScalar *a = NULL;
-  VecGetArray(v, &a);
-  VecRestoreArray(v, &a); /* single rank: the array stays valid for the call */
-  return (double *)a;
+  if (VecGetArray(v, &a)) return FVM_ERR_ARG;
+  int rc = fvm_memcpy_dtoh(g_ctx, a, dev, n * sizeof(double));
+  VecRestoreArray(v, &a);
+  return rc;
 #else
-  return v ? v->a : NULL;
+  if (!v || !v->a || (size_t)v->n < n) return FVM_ERR_ARG;
+  return fvm_memcpy_dtoh(g_ctx, v->a, dev, n * sizeof(double));
 #endif
 }
 
@@ -115,11 +136,11 @@ static int pool(const GridData *g, int i, double **out) {
 static int put(const GridData *g, int i, Vec v, double **dev) {
   int rc = pool(g, i, dev);
   if (rc) return rc;
-  return fvm_memcpy_htod(g_ctx, *dev, vec_data(v), (size_t)g->NX * g->NY * sizeof(double));
+  return vec_to_device(*dev, v, (size_t)g->NX * g->NY);
 }
 
 static int get(const GridData *g, const double *dev, Vec v) {
-  return fvm_memcpy_dtoh(g_ctx, vec_data(v), dev, (size_t)g->NX * g->NY * sizeof(double));
+  return vec_from_device(v, dev, (size_t)g->NX * g->NY);
 }
 
 /* a double field of Marker, named by its address in markers[0] (markerProjection.c:99-102) */
@@ -424,14 +445,14 @@ int retrieveSolutions(GridData *grid, NodalFields *nf, Vec S, double Kcont, int3
     TRY(fvm_malloc((void **)&g_S, 3 * nn * sizeof(double)));
     g_S_cap = 3 * nn;
   }
-  TRY(fvm_memcpy_htod(g_ctx, g_S, vec_data(S), 3 * nn * sizeof(double)));
+  TRY(vec_to_device(g_S, S, 3 * nn));
   NodalArrays d;
   memset(&d, 0, sizeof(d));
   TRY(pool(grid, 0, &d.vx));
   TRY(pool(grid, 1, &d.vy));
   TRY(pool(grid, 2, &d.p));
   TRY(fvm_retrieve_solutions(g_ctx, &g_grid, &d, g_S, Kcont, foundnan));
-  TRY(fvm_memcpy_dtoh(g_ctx, vec_data(S), g_S, 3 * nn * sizeof(double))); /* pressure scaled in place */
+  TRY(vec_from_device(S, g_S, 3 * nn)); /* pressure scaled in place */
   TRY(get(grid, d.vx, nf->vx));
   TRY(get(grid, d.vy, nf->vy));
   TRY(get(grid, d.p, nf->p));
@@ -454,11 +475,11 @@ int initialPressureGuess(GridData *grid, NodalFields *nf, Options *options, Vec 
   }
   double *rho;
   TRY(put(grid, 0, nf->rho, &rho));
-  TRY(fvm_memcpy_htod(g_ctx, g_S, vec_data(S), 3 * nn * sizeof(double)));
+  TRY(vec_to_device(g_S, S, 3 * nn));
   int32_t its = 0;
   double rel = 0.0;
   TRY(fvm_initial_pressure_guess(g_ctx, &g_grid, rho, options->gy, 0.0, 1e-12, 1000000, g_S, NULL, &its, &rel));
-  TRY(fvm_memcpy_dtoh(g_ctx, vec_data(S), g_S, 3 * nn * sizeof(double)));
+  TRY(vec_from_device(S, g_S, 3 * nn));
   return 0;
 }
 

@@ -96,11 +96,8 @@ extern "C" int fvm_ctx_destroy(fvm_ctx *ctx) {
 extern "C" const char *fvm_last_error(const fvm_ctx *ctx) { return ctx ? ctx->err : "no context"; }
 extern "C" void *fvm_ctx_stream(fvm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 extern "C" int fvm_ctx_sync(fvm_ctx *ctx) {
-  int32_t *h = (int32_t *)ctx->pinned.ptr + 1000;  // clear of the other pinned slots
-  FVM_HIP(ctx, hipMemcpyAsync(h, (int32_t *)ctx->red.ptr + FVM_FLAG_SCAN_FAIL, sizeof(int32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
+  if (!ctx) return FVM_ERR_ARG;
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (*h) return fvm_set_err(ctx, FVM_ERR_HIP, "bucket scan: look-back wait exceeded its bound (sort invalid)");
   return FVM_OK;
 }
 extern "C" size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx) {
@@ -111,6 +108,13 @@ extern "C" size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx) {
   size_t s = 0;
   for (const fvm_buf *b : bufs) s += b->bytes;
   return s;
+}
+
+extern "C" int fvm_ctx_set_jaumann(fvm_ctx *ctx, int32_t jaumann) {
+  if (!ctx) return FVM_ERR_ARG;
+  if (jaumann != 0 && jaumann != 1) return fvm_set_err(ctx, FVM_ERR_ARG, "jaumann must be 0 or 1");
+  ctx->jaumann = jaumann;
+  return FVM_OK;
 }
 
 extern "C" int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes) {

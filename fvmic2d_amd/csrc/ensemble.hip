@@ -34,6 +34,7 @@ struct InstDev {
   double rho0;  // materialRho[0] (subgridTemperatureChanges.c:62)
   BoundaryValues bv;
   double dt, dsub;
+  int32_t jaumann;
 };
 
 struct fvm_ensemble {
@@ -295,7 +296,8 @@ __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ ins
   advect_marker(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.cellX, I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, I.T, I.Tdot,
                 deltaT + I.node_off, dTsub + I.mark_off, I.dt);
   // the rotation fused behind the RK4 stages: the batched launch keeps one kernel per phase
-  rotate_eii_marker(m, g.NX, g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.dt, 0);
+  rotate_eii_marker(m, g.NX, g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.dt,
+                    I.jaumann);
 }
 
 // ---- host side ----------------------------------------------------------------------------
@@ -531,6 +533,8 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
   if (I0.nproj < 1 || !I0.proj || I0.nproj > 64) return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: projection list");
   for (int i = 0; i < n; i++) {
     FVM_TRY(fvm_grid_check(ctx, &inst[i].grid));
+    if (inst[i].jaumann != 0 && inst[i].jaumann != 1)
+      return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: instance %d: jaumann must be 0 or 1", i);
     if (inst[i].nproj != I0.nproj || !inst[i].proj)
       return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: instance %d has a different projection list", i);
     for (int p = 0; p < I0.nproj; p++) {
@@ -611,6 +615,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
     D.bv = S.bv;
     D.dt = S.dt;
     D.dsub = S.dsubgrid;
+    D.jaumann = S.jaumann;
     const int64_t ncell = (int64_t)(S.grid.NX - 1) * (S.grid.NY - 1);
     E->nmark += S.markers.nMark;
     E->nbucket += 4 * ncell;

@@ -37,6 +37,7 @@ struct fvm_ctx {
   uint64_t payload_serial = 0;
   bool reuse_payload = false;
   int lanes = 8;  // lanes per lattice cell in the projection partials (8, 16 or 32)
+  int jaumann = 0;  // stress rotation of the whole-step entry points
   fvm_buf pinned;  // small pinned host buffer for scalar results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // optional per-phase timing (fvm_profile_*): event pairs around launches
@@ -96,8 +97,8 @@ static inline unsigned fvm_blocks(int64_t n, int bs) { return (unsigned)((n + bs
 // r = nb % 8 -- a bijection on [0, nb).
 #define FVM_NUM_XCD 8
 
-// int32 slot in ctx->red: sticky flag set when the one-pass bucket scan gave up
-// waiting on a predecessor tile (fvm_ctx_sync reports it)
+// int32 slot in ctx->red: sticky diagnostic flag set when the one-pass bucket
+// scan stopped waiting on a predecessor tile and summed the counts itself
 #define FVM_FLAG_SCAN_FAIL 2
 __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
   const int64_t q = nb / FVM_NUM_XCD, r = nb % FVM_NUM_XCD;

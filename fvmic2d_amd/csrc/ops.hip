@@ -182,7 +182,7 @@ int fvm_advect_with_subgrid_t2(fvm_ctx *ctx, fvm_markers *mk, const NodalArrays 
   if (mk->nMark == 0) return FVM_OK;
   {
     FVM_PHASE(ctx, FVM_PHASE_ADVECT);
-    FVM_TRY(launch_advect(ctx, g, mk, nf, bv, dt, 0, deltaT, dTsub, dt_thermal));
+    FVM_TRY(launch_advect(ctx, g, mk, nf, bv, dt, ctx->jaumann, deltaT, dTsub, dt_thermal));
   }
   fvm_markers_moved(ctx);
   return FVM_OK;

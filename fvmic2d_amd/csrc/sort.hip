@@ -129,9 +129,9 @@ __global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *
 // (status 1) and then its inclusive prefix (status 2) as one 64-bit word
 // (status << 32 | value); a successor's first wavefront reads its predecessors'
 // words 64 at a time and sums them back to the nearest inclusive prefix.  The
-// wait is bounded: after SCAN_SPIN_MAX polls the tile sets the sticky flag
-// *fail and stops waiting (its output is then invalid);
-// fvm_ctx_sync reports the flag.  state[0] = ticket, state[2 + t] = word of tile t.
+// wait is bounded: after SCAN_SPIN_MAX polls the tile stops waiting, sums the
+// counts before it directly (exact, only slower) and sets the sticky
+// diagnostic flag *fail.  state[0] = ticket, state[2 + t] = word of tile t.
 #define SCAN_SPIN_MAX (1 << 20)
 
 __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restrict__ in, int32_t *__restrict__ out,
@@ -204,6 +204,17 @@ __global__ void __launch_bounds__(SCAN_BS) k_scan_onepass(const int32_t *__restr
       prefix += v;
       if (incl) break;
       j -= 64;
+    }
+    if (failed) {
+      // the bounded wait ran out: form the exclusive prefix from the counts
+      // themselves (all of them were written before this launch), so the scan
+      // stays exact; the flag only records that the slow path was taken
+      int32_t s = 0;
+      const int64_t end = (int64_t)tile * SCAN_TILE;
+      for (int64_t k = lane; k < end; k += 64) s += in[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      prefix = s;
     }
     if (lane == 0) {
       if (failed) atomicOr(fail, 1);

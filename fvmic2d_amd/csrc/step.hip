@@ -48,7 +48,7 @@ extern "C" int fvm_step_vep(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, co
   FVM_TRY(fvm_subgrid_temperature_changes(ctx, host->thisT, g, &th, mk, mat, dt,  // :415
                                           opt->subgridTemperatureDiffusivity));
   NodalArrays adv = *host;
-  FVM_TRY(fvm_advect_markers_rk(ctx, mk, &adv, g, bv, dt, 0));  // :435
+  FVM_TRY(fvm_advect_markers_rk(ctx, mk, &adv, g, bv, dt, ctx->jaumann));  // :435
   if (isYielding) *isYielding = y;
   if (residual) *residual = r;
   return FVM_OK;

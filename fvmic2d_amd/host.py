@@ -22,6 +22,11 @@ def _mat_table(materials, name):
 
 
 class MarkerPath:
+    # stress rotation used when this path is an ensemble member (0 exact, 1
+    # -DJAUMANN); a single-instance step takes the context's setting
+    # (fvm_ctx_set_jaumann)
+    jaumann = 0
+
     def __init__(self, w, ctx=None, extra=(), capacity=None):
         self.w = w
         self.ctx = ctx or Context()
@@ -263,6 +268,7 @@ class Ensemble:
             e.bv = w.bv
             e.dt = w.dt
             e.dsubgrid = w.options.subgridTemperatureDiffusivity
+            e.jaumann = p.jaumann
         self._arr = arr
         h = P()
         self.ctx.call("fvm_ensemble_create", I32(len(paths)), C.cast(arr, P), C.byref(h))

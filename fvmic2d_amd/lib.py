@@ -13,6 +13,7 @@ from . import abi
 
 _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfvmic2d_hip.so")
+COMPAT_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfvmic2d_compat.so")
 
 ERRORS = {0: "FVM_OK", 1: "FVM_ERR_PERIODIC", 2: "FVM_ERR_CELL_OUT_OF_BOUNDS", 3: "FVM_ERR_BAD_STAG",
           4: "FVM_ERR_BAD_METHOD", 5: "FVM_ERR_NAN_RESIDUAL", 6: "FVM_ERR_VISCOSITY_LAW",
@@ -46,7 +47,8 @@ class fvm_projection(C.Structure):
 class fvm_instance(C.Structure):
     _fields_ = [("grid", fvm_grid), ("markers", fvm_markers), ("nodal", abi.NodalArrays),
                 ("nproj", C.c_int32), ("proj", C.c_void_p), ("materials", abi.Materials),
-                ("bv", abi.BoundaryValues), ("dt", C.c_double), ("dsubgrid", C.c_double)]
+                ("bv", abi.BoundaryValues), ("dt", C.c_double), ("dsubgrid", C.c_double),
+                ("jaumann", C.c_int32)]
 
 
 class fvm_tuning(C.Structure):
@@ -62,6 +64,7 @@ _SIGS = {
     "fvm_last_error": [P],
     "fvm_ctx_stream": [P],
     "fvm_ctx_sync": [P],
+    "fvm_ctx_set_jaumann": [P, I32],
     "fvm_ctx_workspace_bytes": [P],
     "fvm_ctx_set_lanes": [P, I32],
     "fvm_device_count": [C.POINTER(C.c_int)],
@@ -121,8 +124,10 @@ _SIGS = {
     "fvm_ensemble_launch": [P, P],
     "fvm_ensemble_autotune": [P, P, I32, P, C.POINTER(C.c_float)],
     "fvm_ensemble_counters": [P, P, C.POINTER(I32), C.POINTER(I32)],
-    # reference-signature wrappers (fvm_compat.h); the wrappers themselves keep
-    # the reference's names and are bound by callers with their own prototypes
+}
+# libfvmic2d_compat.so: the reference-named wrappers (fvm_compat.h); they keep
+# the reference's names and are bound by callers with their own prototypes
+_COMPAT_SIGS = {
     "fvm_compat_finalize": [],
     "fvm_compat_last_status": [],
 }
@@ -148,7 +153,28 @@ def load(path=None):
 
 
 def exported_symbols():
-    return list(_SIGS)
+    return list(_SIGS) + list(_COMPAT_SIGS)
+
+
+_COMPAT = None
+
+
+def load_compat(path=None):
+    """The reference-named wrappers (libfvmic2d_compat.so, over libfvmic2d_hip.so)."""
+    global _COMPAT
+    if _COMPAT is not None:
+        return _COMPAT
+    load()
+    p = path or COMPAT_LIB_PATH
+    if not os.path.exists(p):
+        raise FileNotFoundError("%s not built; run python -m fvmic2d_amd.build" % p)
+    lib = C.CDLL(p)
+    for name, args in _COMPAT_SIGS.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = C.c_int
+    _COMPAT = lib
+    return lib
 
 
 class Context:

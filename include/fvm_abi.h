@@ -83,9 +83,7 @@ int fvm_ctx_create(fvm_ctx **ctx, void *stream);
 int fvm_ctx_destroy(fvm_ctx *ctx);
 const char *fvm_last_error(const fvm_ctx *ctx);
 void *fvm_ctx_stream(fvm_ctx *ctx);
-/* Waits for the context's stream.  Returns FVM_ERR_HIP if a bucket sort since
- * the context was created gave up its bounded inter-block wait (never expected;
- * the sort's ordering would then be invalid). */
+/* Waits for the context's stream (every call before it has completed). */
 int fvm_ctx_sync(fvm_ctx *ctx);
 /* Bytes of device workspace the context currently holds (sort buffers, partials). */
 size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
@@ -93,6 +91,11 @@ size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
  * lane count fixes the summation tree, so results are bitwise reproducible for
  * a given setting and agree across settings to the projection tolerance. */
 int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes);
+/* Stress rotation of the whole-step entry points (fvm_step_thermal,
+ * fvm_step_vep): 0 (default) the exact rotation of advectMarkersRK, 1 the
+ * -DJAUMANN form (markers.c:2147-2156), as the reference selects it at compile
+ * time.  fvm_advect_markers_rk takes the choice as its own argument. */
+int fvm_ctx_set_jaumann(fvm_ctx *ctx, int32_t jaumann);
 
 /* ---- device memory helpers (so a host program needs no other GPU API) ---- */
 int fvm_device_count(int *n);
@@ -380,6 +383,7 @@ typedef struct {
   BoundaryValues bv;
   double dt;
   double dsubgrid;       /* Options.subgridTemperatureDiffusivity */
+  int32_t jaumann;       /* stress rotation: 0 exact, 1 -DJAUMANN (as fvm_ctx_set_jaumann) */
 } fvm_instance;
 
 /* Launch shape: threads per block {128, 256, 512}; lanes per lattice cell in

@@ -8,7 +8,9 @@
  * PETSc handles (DM, Vec) are replaced by plain pointers: DM is an opaque
  * pointer, and a nodal field is a dense row-major [jy][ix] array of NX*NY doubles.
  *
- * Checked at compile time in fvmic2d_amd/csrc/abi_layout.c (offsets of Marker).
+ * tests/test_abi_cpu.py compiles this header with gcc and checks Marker's size
+ * (216 B) and every field offset against the reference's layout (X 0 ...
+ * isYielding 208), and the other structs against the Python ctypes mirror.
  */
 #ifndef FVM_STRUCTURES_H
 #define FVM_STRUCTURES_H

@@ -1,5 +1,8 @@
-"""The C ABI without a GPU: the library loads, exports every function that
-include/*.h declares, and the ctypes mirror matches the C struct layouts."""
+"""The C ABI without a GPU: the libraries load and export every function that
+include/*.h declares (the C ABI from libfvmic2d_hip.so, the reference-named
+wrappers from libfvmic2d_compat.so only), the ctypes mirror matches the C
+struct layouts, Marker keeps the reference's field offsets, and the PETSc
+variant of the wrappers compiles (a compile check only: PETSc is not here)."""
 import ctypes as C
 import json
 import os
@@ -14,10 +17,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include")
 
 
-def declared_functions():
+def declared_functions(only=None):
     names = set()
     for f in os.listdir(INC):
-        if f.endswith(".h"):
+        if f.endswith(".h") and (only is None or f == only):
             txt = open(os.path.join(INC, f)).read()
             txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
             names |= set(re.findall(r"^\s*(?:int|void|size_t|const char \*|void \*)\s*\**\s*(fvm_\w+)\s*\(",
@@ -27,9 +30,13 @@ def declared_functions():
 
 def test_library_exports_every_declared_symbol():
     names = declared_functions()
-    assert len(names) >= 30
+    compat = set(declared_functions("fvm_compat.h"))
+    assert len(names) >= 30 and compat
     h = C.CDLL(lib.LIB_PATH)
-    missing = [n for n in names if not hasattr(h, n)]
+    missing = [n for n in names if n not in compat and not hasattr(h, n)]
+    assert not missing, missing
+    hc = C.CDLL(lib.COMPAT_LIB_PATH)
+    missing = [n for n in compat if not hasattr(hc, n)]
     assert not missing, missing
     # and the ctypes binding covers all of them
     assert set(names) <= set(lib.exported_symbols()), set(names) - set(lib.exported_symbols())
@@ -46,8 +53,58 @@ def test_reference_named_wrappers_are_exported():
     txt = open(os.path.join(INC, "fvm_compat.h")).read()
     for n in COMPAT_NAMES:
         assert re.search(r"\b%s\s*\(" % n, txt), n
-    h = C.CDLL(lib.LIB_PATH)
-    assert all(hasattr(h, n) for n in COMPAT_NAMES)
+    hc = C.CDLL(lib.COMPAT_LIB_PATH)
+    assert all(hasattr(hc, n) for n in COMPAT_NAMES)
+    # the C ABI library does not define the reference's names, so a PETSc build
+    # that compiles compat.c itself links it without duplicate symbols
+    out = subprocess.run(["nm", "-D", "--defined-only", lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    defined = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert not defined & set(COMPAT_NAMES), defined & set(COMPAT_NAMES)
+
+
+# Marker offsets of the reference's fdcode.h:97-129 layout (PetscScalar = double,
+# PetscInt = int32, no TEXTURE), as a compiler lays that struct out
+REFERENCE_MARKER_OFFSETS = {
+    "X": 0, "Y": 8, "VX": 16, "VY": 24, "Mat": 32, "T": 40, "Tdot": 48, "s": 56, "eta": 80, "mu": 88,
+    "rho": 96, "rhodot": 104, "p": 112, "e": 120, "E": 144, "Eii": 168, "wxy": 176, "cellX": 184,
+    "cellY": 188, "cpu": 192, "strainRateResidual": 200, "isYielding": 208}
+
+
+def test_marker_keeps_reference_offsets(tmp_path):
+    src = tmp_path / "off.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include \"structures.h\"\n"
+                   "int main(void) {\n"
+                   + "".join('  printf("%s %%zu\\n", offsetof(Marker, %s));\n' % (f, f)
+                             for f in REFERENCE_MARKER_OFFSETS)
+                   + '  printf("size %zu\\n", sizeof(Marker));\n  return 0;\n}\n')
+    exe = tmp_path / "off"
+    subprocess.run(["gcc", "-std=c99", "-I" + INC, str(src), "-o", str(exe)], check=True)
+    got = dict(ln.split() for ln in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                     check=True).stdout.splitlines())
+    assert int(got.pop("size")) == 216
+    assert {k: int(v) for k, v in got.items()} == REFERENCE_MARKER_OFFSETS
+
+
+def test_petsc_variant_of_the_wrappers_compiles(tmp_path):
+    """compat.c with -DFVM_WITH_PETSC against a declarations-only petscvec.h:
+    it compiles, and its object needs only the C ABI (fvm_*), PETSc's Vec
+    array accessors and libc -- nothing that would clash with a PETSc build."""
+    obj = tmp_path / "compat_petsc.o"
+    subprocess.run(["gcc", "-std=c99", "-O2", "-Wall", "-Werror", "-DFVM_WITH_PETSC",
+                    "-I" + os.path.join(ROOT, "tests", "petsc_shim"), "-I" + INC, "-c",
+                    os.path.join(ROOT, "fvmic2d_amd", "csrc", "compat.c"), "-o", str(obj)], check=True)
+    out = subprocess.run(["nm", "-u", str(obj)], capture_output=True, text=True, check=True).stdout
+    undefined = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    foreign = {u for u in undefined if not u.startswith("fvm_")
+               and u not in ("VecGetArray", "VecRestoreArray", "VecGetArrayRead", "VecRestoreArrayRead")}
+    libc_path = subprocess.run(["gcc", "-print-file-name=libc.so.6"], capture_output=True, text=True,
+                               check=True).stdout.strip()
+    out = subprocess.run(["nm", "-D", "--defined-only", libc_path], capture_output=True, text=True,
+                         check=True).stdout
+    libc = {ln.split()[-1].split("@")[0] for ln in out.splitlines() if ln.strip()}
+    assert foreign <= libc, foreign - libc
+    assert {"VecGetArray", "VecRestoreArray", "VecGetArrayRead", "VecRestoreArrayRead"} <= undefined
 
 
 def test_binding_loads_without_gpu():

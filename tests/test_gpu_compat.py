@@ -62,7 +62,7 @@ class HostProblem:
 
 @pytest.fixture(scope="module")
 def L():
-    h = lib.load()
+    h = lib.load_compat()
     return h
 
 

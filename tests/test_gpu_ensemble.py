@@ -7,6 +7,8 @@ fvm_step_thermal; with every launch shape it matches the oracle to the
 single-instance bars (positions bitwise, FP64 fields to 1e-12 relative); graph
 replays equal eager steps bit for bit.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -73,6 +75,38 @@ def test_ensemble_equals_single_instance():
         ens.step()
         for i, (a, b) in enumerate(zip(alone, batch)):
             _assert_same(a, b, "step %d instance %d" % (s, i))
+    ens.close()
+
+
+def test_ensemble_jaumann_equals_single_instance():
+    """The -DJAUMANN rotation selected per instance matches the single-instance
+    step with the context's rotation set the same way."""
+    print("CASE ensemble vs single, jaumann rotation", flush=True)
+    ctx = Context()
+    ws = _instances(3, seed=17, empty=False)
+    for w in ws:  # stresses to rotate
+        r = workloads.Rng(3, 11)
+        for f in ("sxx", "syy", "sxy"):
+            w.markers[f] = r.normal(w.n_markers)
+    alone, batch = _paths(ws, ctx), _paths(ws, ctx)
+    for p in batch:
+        p.jaumann = 1
+    ens = Ensemble(batch)
+    ctx.call("fvm_ctx_set_jaumann", C.c_int32(1))
+    for p in alone:
+        p.step_thermal()
+    ens.step()
+    ctx.call("fvm_ctx_set_jaumann", C.c_int32(0))
+    for i, (a, b) in enumerate(zip(alone, batch)):
+        _assert_same(a, b, "jaumann instance %d" % i)
+        ga, gb = a.get_markers(), b.get_markers()
+        for f in ("sxx", "syy", "sxy", "Eii"):
+            assert_bitwise(ga[f], gb[f], "jaumann instance %d %s" % (i, f))
+    # and the rotation differs from the exact one (the flag reaches the kernel)
+    exact = _paths(ws, ctx)
+    for p in exact:
+        p.step_thermal()
+    assert not np.array_equal(exact[0].get_markers()["sxx"], alone[0].get_markers()["sxx"])
     ens.close()
 
 
