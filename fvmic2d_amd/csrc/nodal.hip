@@ -8,6 +8,8 @@
 #include <float.h>
 #include <math.h>
 
+#include <mutex>
+
 #include "fvm_internal.h"
 
 namespace {
@@ -236,9 +238,11 @@ __global__ void __launch_bounds__(256) k_dt_partial(DGrid g, const double *__res
 // limitTimestep.c:112-121 on the combined partials.  VecMax/VecMin start from
 // the first entry and take strictly larger/smaller ones: a NaN first entry
 // stays, later NaNs are skipped.
+// The starting step is d0, or *d0p when d0p is given (then out may equal d0p:
+// one thread reads it before it writes).
 __global__ void __launch_bounds__(256) k_dt_final(int nb, const DtPart *__restrict__ part,
                                                   const double *__restrict__ rhodot, double d0,
-                                                  double *__restrict__ out) {
+                                                  const double *d0p, double *out) {
   // min and max are exact in any order: the block combines the partials, then
   // one thread applies the reference's final steps
   __shared__ DtPart s_p[4];
@@ -262,7 +266,7 @@ __global__ void __launch_bounds__(256) k_dt_final(int nb, const DtPart *__restri
     if (s_p[k].rmax > r.rmax) r.rmax = s_p[k].rmax;
     if (s_p[k].rmin < r.rmin) r.rmin = s_p[k].rmin;
   }
-  double d = d0;
+  double d = d0p ? *d0p : d0;
   take_min(d, r.cmin);
   const double r0 = rhodot[0];
   double rmax = r0 != r0 ? r0 : r.rmax, rmin = r0 != r0 ? r0 : r.rmin;
@@ -340,12 +344,58 @@ extern "C" int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, 
     k_dt_partial<<<nb, 256, 0, ctx->stream>>>(dgrid(g), nf->vx, nf->vy, nf->rho, nf->rhodot,
                                               opt->displacementStepLimit, part);
     FVM_LAUNCH_CHECK(ctx);
-    k_dt_final<<<1, 256, 0, ctx->stream>>>(nb, part, nf->rhodot, *displacementdt, dout);
+    k_dt_final<<<1, 256, 0, ctx->stream>>>(nb, part, nf->rhodot, *displacementdt, nullptr, dout);
     FVM_LAUNCH_CHECK(ctx);
   }
   double *h = (double *)ctx->pinned.ptr + 24;
   FVM_HIP(ctx, hipMemcpyAsync(h, dout, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *displacementdt = *h;
+  return FVM_OK;
+}
+
+// ---- limitDisplacementTimestep without a context and without allocating --------------
+// The per-block partials of k_dt_partial live in `scratch`, a file-scope device
+// array of DT_BLOCKS entries (24 KiB; one copy on every GPU the library runs
+// on, held until the process ends).  Host code never passes the array's name:
+// its address on each device is looked up once, with that device current, by
+// hipGetSymbolAddress and kept in g_scratch_addr.  Any grid size fits, since
+// the partial pass runs at most DT_BLOCKS blocks over a grid-stride loop.
+// Every call on one GPU shares the array, so calls that overlap on one GPU
+// race; each call writes every partial it reads (no clearing is needed).
+static __device__ __align__(256) DtPart scratch[DT_BLOCKS];
+
+#define FVM_NOALLOC_MAX_DEVICES 64
+static void *g_scratch_addr[FVM_NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_on_current_device(DtPart **out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return FVM_ERR_HIP;
+  if (dev < 0 || dev >= FVM_NOALLOC_MAX_DEVICES) return FVM_ERR_ARG;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[dev]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FVM_ERR_HIP;
+    g_scratch_addr[dev] = p;
+  }
+  *out = (DtPart *)g_scratch_addr[dev];
+  return FVM_OK;
+}
+
+extern "C" int fvm_limit_displacement_timestep_noalloc(void *stream, const fvm_grid *g, const NodalArrays *nf,
+                                                       const Options *opt, double *dt_device) {
+  if (!g || !nf || !opt || !dt_device || !nf->vx || !nf->vy || !nf->rho || !nf->rhodot) return FVM_ERR_ARG;
+  if (g->xperiodic) return FVM_ERR_PERIODIC;
+  if (g->NX < 2 || g->NY < 2 || !g->x || !g->y) return FVM_ERR_ARG;
+  DtPart *part = nullptr;
+  FVM_TRY(scratch_on_current_device(&part));
+  const hipStream_t s = (hipStream_t)stream;
+  const int64_t nn = (int64_t)g->NX * g->NY;
+  const int nb = (int)std::min<int64_t>(fvm_blocks(nn, 256), DT_BLOCKS);
+  k_dt_partial<<<nb, 256, 0, s>>>(dgrid(g), nf->vx, nf->vy, nf->rho, nf->rhodot, opt->displacementStepLimit, part);
+  if (hipGetLastError() != hipSuccess) return FVM_ERR_HIP;
+  k_dt_final<<<1, 256, 0, s>>>(nb, part, nf->rhodot, 0.0, dt_device, dt_device);
+  if (hipGetLastError() != hipSuccess) return FVM_ERR_HIP;
   return FVM_OK;
 }

@@ -240,6 +240,17 @@ int fvm_nodal_stress_strain(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *
  * synchronises.  Called at main_subduction.c:349. */
 int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
                                     double *displacementdt, const Options *opt);
+/* No counterpart in the reference: fvm_limit_displacement_timestep for callers
+ * that cannot allocate (code under HIP graph capture, tight loops, harnesses
+ * that forbid hipMalloc).  It takes no context and allocates nothing: its
+ * per-block partials live in a static device array of 24 KiB per GPU (1024
+ * partials; any grid size fits), held until the process ends.  *dt_device is
+ * device memory: in, the step to limit; out, the limited step, bitwise the
+ * value fvm_limit_displacement_timestep returns.  Enqueued on `stream` (a
+ * hipStream_t, NULL for the default stream) without synchronising.  Calls that
+ * overlap on one GPU share the array and race. */
+int fvm_limit_displacement_timestep_noalloc(void *stream, const fvm_grid *g, const NodalArrays *nf,
+                                            const Options *opt, double *dt_device);
 
 /* ---- energy equation ------------------------------------------------------ */
 /* formThermalSystem.c:6-249 formThermalSystem(problem, grid, nodalFields, LHS,
