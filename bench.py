@@ -1,12 +1,19 @@
 #!/usr/bin/env python3
 """Throughput of the fvmic2d marker step on one MI355X.
 
-A step is one pass of the marker hot path over the convection_tdep workload
-(201x201 uniform grid, 36 markers per cell = 1.44M markers, eta = exp(-ln(1e5) T)):
-findMarkerCells, the density loop, projection of T, eta, rho, k onto all four
-staggered node families plus Cp on basic nodes (17 nodal fields),
-subgridTemperatureChanges and advectMarkersRK -- every kernel of libfvmic2d_hip.so,
-inputs resident in HBM.  Successive steps continue the simulation (markers move).
+The default line is the batched ensemble: 64 independent convection instances
+(grids 65^2..257^2 nodes, 16-36 markers per cell, viscosity contrast 1e2-1e6,
+~47M markers) stepped together -- findMarkerCells, the density loop,
+projection of T, eta, rho, k onto all four staggered node families plus Cp
+(17 nodal fields), subgridTemperatureChanges and advectMarkersRK -- one launch
+per phase over all instances, replayed from a captured HIP graph, inputs
+resident in HBM.  ``sequential`` is the same instances stepped one
+fvm_step_thermal call after another.  Beside it the line carries the
+single-instance steps (``convection_tdep``: 201x201 nodes, 1.44M markers;
+``vep_subduction``: 801x401 tanh-stretched, 8M markers, the full
+visco-elasto-plastic step) and a small learned Stokes initial guess run
+(``learned_guess``: host solver iterations to 1e-8 from the plain and the
+learned guess).
 
 Prints ONE JSON line.  ``value`` = active markers x steps / seconds, timed with
 HIP events on the library's stream between two synchronisations.  ``roofline``
@@ -33,6 +40,10 @@ PHASES = ["locate", "sort", "partials", "nodes", "subgrid_t1", "subgrid_t2", "ad
           "marker_update", "other"]
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(w):
     """Compulsory state traffic per launch of each phase of one convection_tdep step.
 
@@ -54,28 +65,38 @@ STEP_BYTES_PER_MARKER = 268  # locate + projections 60, subgrid T pass 1 32, pas
 STEP_BYTES_PER_NODE = 224
 
 
-def cpu_baseline(budget_s=12.0):
-    """The oracle's full step on the same workload, single thread, until budget_s."""
-    import oracle
-    from fvmic2d_amd import workloads
-    w = workloads.convection_tdep()
-    st = oracle.State(w, extra=[p[4] for p in w.projections])
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        st.step_thermal()
-        steps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    return {"value": w.n_markers * steps / el, "unit": "markers/s", "cores": 1, "kind": "port",
-            "sample": "%d full convection_tdep steps (1.44M markers each) of the C restatement, "
-                      "gcc -O3 -ffp-contract=off, one thread, %.1f s" % (steps, el),
-            "cpu": platform.processor() or platform.machine()}
+# ---- host CPU and the CPU baselines (the oracle, forked before the GPU is touched) ----
+def host_cpu():
+    """The host CPU model (/proc/cpuinfo) and the size of this process's affinity set."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = os.cpu_count() or 1
+    return model, aff
 
 
-def _oracle_steps(budget_s):
-    """One worker of the all-core baseline: oracle steps of convection_tdep for
-    budget_s seconds; returns (markers stepped, seconds)."""
+def cpu_workers():
+    """Worker processes for the all-core baselines: the affinity set, or the
+    thread share the environment grants this job (OMP_NUM_THREADS) if smaller."""
+    _, aff = host_cpu()
+    n = aff
+    share = os.environ.get("OMP_NUM_THREADS", "")
+    if share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n), aff
+
+
+def _tdep_steps(budget_s):
+    """Oracle steps of convection_tdep for budget_s seconds: (markers stepped, s)."""
     import oracle
     from fvmic2d_amd import workloads
     w = workloads.convection_tdep()
@@ -89,20 +110,86 @@ def _oracle_steps(budget_s):
             return w.n_markers * steps, el
 
 
-def cpu_baseline_all_cores(budget_s=12.0, max_workers=16):
-    """The oracle's step on every host core at once: one independent copy of the
-    workload per process (the reference's ranks each step their own share; this
-    leaves out their communication, so it bounds the reference's multi-rank rate
-    from above).  Forks before this process touches the GPU."""
+def _ensemble_steps(args):
+    """Oracle steps of ensemble instances first, first + stride, ... (one step
+    each, generation not timed) until budget_s of stepping: (markers stepped, s)."""
+    first, stride, budget_s = args
+    import oracle
+    from fvmic2d_amd import workloads
+    markers, el, i = 0, 0.0, first
+    while el < budget_s:
+        w = workloads.ensemble(1, seed=4, nmin=65, nmax=257, first=i % 64)[0]
+        st = oracle.State(w, extra=[p[4] for p in w.projections])
+        t0 = time.perf_counter()
+        st.step_thermal()
+        el += time.perf_counter() - t0
+        markers += w.n_markers
+        i += stride
+    return markers, el
+
+
+def cpu_baseline_ensemble(budget_s):
+    m, el = _ensemble_steps((0, 1, budget_s))
+    return {"value": m / el, "unit": "markers/s", "cores": 1, "kind": "port",
+            "sample": "one oracle thermal step of ensemble instances 0, 1, 2, ... in turn, %d markers "
+                      "stepped in %.1f s (C restatement, gcc -O3 -ffp-contract=off, one thread)" % (m, el),
+            "cpu": host_cpu()[0]}
+
+
+def cpu_baseline_tdep(budget_s):
+    m, el = _tdep_steps(budget_s)
+    return {"value": m / el, "unit": "markers/s", "cores": 1, "kind": "port",
+            "sample": "%d full convection_tdep steps (1.44M markers each) of the C restatement, "
+                      "gcc -O3 -ffp-contract=off, one thread, %.1f s" % (m // 1440000, el),
+            "cpu": host_cpu()[0]}
+
+
+def cpu_all_cores(kind, budget_s):
+    """The oracle on every core of this job at once, one independent problem per
+    process (the reference's ranks each step their own share; leaving out their
+    communication bounds the reference's multi-rank rate from above)."""
     import multiprocessing
-    n = max(1, min(max_workers, os.cpu_count() or 1))
+    n, aff = cpu_workers()
     with multiprocessing.get_context("fork").Pool(n) as pool:
-        res = pool.map(_oracle_steps, [budget_s] * n)
-    rate = sum(m / t for m, t in res)
-    return {"value": rate, "unit": "markers/s", "cores": n, "kind": "port",
-            "sample": "%d processes, each stepping its own convection_tdep copy (1.44M markers) with the C "
-                      "restatement for %.0f s" % (n, budget_s),
-            "cpu": platform.processor() or platform.machine()}
+        if kind == "ensemble":
+            res = pool.map(_ensemble_steps, [(k, n, budget_s) for k in range(n)])
+            sample = ("%d processes, worker k stepping ensemble instances k, k+%d, ... with the C restatement "
+                      "for %.0f s" % (n, n, budget_s))
+        else:
+            res = pool.map(_tdep_steps, [budget_s] * n)
+            sample = ("%d processes, each stepping its own convection_tdep copy (1.44M markers) with the C "
+                      "restatement for %.0f s" % (n, budget_s))
+    return {"value": sum(m / t for m, t in res), "unit": "markers/s", "cores": n, "affinity_cores": aff,
+            "kind": "port", "sample": sample, "cpu": host_cpu()[0]}
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of `kernel` from its committed counter summary, if any:
+    profiles/traffic_<kernel>.md holds 'bytes_per_launch: N' (FETCH_SIZE +
+    WRITE_SIZE of a rocprofv3 --pmc run, scaled by a copy of known size) and a
+    'measured:' line.  This run measures no counters; the figure carries its source."""
+    tf = os.path.join(ROOT, "profiles", "traffic_%s.md" % kernel)
+    if not os.path.exists(tf):
+        return None
+    b, when = None, "?"
+    for line in open(tf):
+        t = line.strip()
+        if t.startswith("bytes_per_launch:"):
+            b = float(t.split(":", 1)[1])
+        elif t.startswith("measured:"):
+            when = t.split(":", 1)[1].strip()
+    if b is None:
+        return None
+    return {"bytes": b, "source": "profiles/traffic_%s.md (counter run, %s; not measured in this run)" % (
+        kernel, when)}
+
+
+def roofline(kernel_label, traffic_key, ab, ms):
+    achieved = ab / (ms / 1e3) / 1e9
+    tr = traffic_per_launch(traffic_key)
+    return {"bound": "hbm", "kernel": kernel_label, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": tr and tr["bytes"], "traffic_source": tr and tr["source"],
+            "algorithmic_bytes_per_launch": ab, "avg_launch_ms": ms}
 
 
 def phase_profile(ctx, step, nsteps):
@@ -120,7 +207,7 @@ def phase_profile(ctx, step, nsteps):
     return ms, la
 
 
-def bench_ensemble(a):
+def run_ensemble(a):
     """The batched ensemble: 64 convection_tdep-style instances, grids 65^2..257^2
     (--ens-nmin/--ens-nmax), 16-36 markers per cell, viscosity contrast 1e2-1e6,
     against the same instances stepped one fvm_step_thermal call after another."""
@@ -129,19 +216,17 @@ def bench_ensemble(a):
     from fvmic2d_amd.lib import Context
 
     t0 = time.perf_counter()
-    ws = []
-    for i in range(0, a.instances, 8):
-        ws += workloads.ensemble(min(8, a.instances - i), seed=4, nmin=a.ens_nmin, nmax=a.ens_nmax, first=i)
-        print("ensemble: generated %d instances (%.0f s)" % (len(ws), time.perf_counter() - t0),
-              file=sys.stderr, flush=True)
     ctx = Context()
-    paths = []
-    for w in ws:
-        paths.append(MarkerPath(w, ctx=ctx))
-        w.markers = None  # host copy no longer needed
+    paths, shapes = [], []
+    for i in range(0, a.instances, 8):
+        for w in workloads.ensemble(min(8, a.instances - i), seed=4, nmin=a.ens_nmin, nmax=a.ens_nmax, first=i):
+            paths.append(MarkerPath(w, ctx=ctx))
+            shapes.append((w.NX, w.NY))
+            w.markers = None  # host copy no longer needed
+        log("ensemble: %d instances generated and uploaded (%.0f s)" % (len(paths), time.perf_counter() - t0))
     ctx.sync()
     n0 = sum(p.markers.n for p in paths)
-    print("ensemble: %d markers on the GPU" % n0, file=sys.stderr, flush=True)
+    nodes = sum(nx * ny for nx, ny in shapes)
 
     def seq_step():
         for p in paths:
@@ -153,12 +238,11 @@ def bench_ensemble(a):
     for _ in range(a.steps):
         seq_step()
     seq_ms = ctx.timer_stop() / a.steps
-    print("ensemble: sequential %.3f ms/step" % seq_ms, file=sys.stderr, flush=True)
+    log("ensemble: sequential %.3f ms/step" % seq_ms)
 
     ens = Ensemble(paths)
     (block, lanes), tuned_ms = ens.autotune(reps=3)
-    print("ensemble: autotuned block=%d lanes=%d (%.3f ms)" % (block, lanes, tuned_ms),
-          file=sys.stderr, flush=True)
+    log("ensemble: autotuned block=%d lanes=%d (%.3f ms)" % (block, lanes, tuned_ms))
     for _ in range(a.warmup):
         ens.launch()
     ctx.sync()
@@ -166,35 +250,116 @@ def bench_ensemble(a):
     for _ in range(a.steps):
         ens.launch()
     ens_ms = ctx.timer_stop() / a.steps
+    log("ensemble: batched %.3f ms/step" % ens_ms)
     nout, nbad = ens.counters()
     active = n0 - nout
     phase_ms, phase_la = phase_profile(ctx, ens.step, 3)
-    step_bytes = n0 * STEP_BYTES_PER_MARKER + sum(w.NX * w.NY for w in ws) * STEP_BYTES_PER_NODE
-    nodes = sum(w.NX * w.NY for w in ws)
-    ab = {"advect": n0 * (96 + 88) + nodes * 32}
+    step_bytes = n0 * STEP_BYTES_PER_MARKER + nodes * STEP_BYTES_PER_NODE
+    ab = n0 * (96 + 88) + nodes * 32
     adv_ms = phase_ms["advect"] / phase_la["advect"]
-    achieved = ab["advect"] / (adv_ms / 1e3) / 1e9
     value = active / (ens_ms / 1e3)
     out = {
         "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ens_ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "ensemble%d" % len(ws), "grids": "%dx%d..%dx%d" % (
-            min(w.NX for w in ws), min(w.NX for w in ws), max(w.NX for w in ws), max(w.NX for w in ws)),
-            "markers": n0, "tuning": {"block": block, "lanes": lanes}, "graph": True},
+        "config": {"workload": "ensemble%d" % len(paths),
+                   "grids": "%dx%d..%dx%d" % (min(s[0] for s in shapes), min(s[0] for s in shapes),
+                                              max(s[0] for s in shapes), max(s[0] for s in shapes)),
+                   "markers": n0, "markers_per_cell": "16-36", "viscosity_contrast": "1e2-1e6",
+                   "tuning": {"block": block, "lanes": lanes}, "graph": True},
         "sequential": {"ms_per_step": seq_ms, "markers_per_s": active / (seq_ms / 1e3)},
         "speedup_vs_sequential": seq_ms / ens_ms,
-        "roofline": {"bound": "hbm", "kernel": "advect", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": ab["advect"], "avg_launch_ms": adv_ms},
+        "roofline": roofline("advect (kb_advect: subgrid T pass 2, RK4, stress rotation)", "kb_advect", ab,
+                             adv_ms),
         "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ens_ms / 1e3) / 1e9,
-                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_la,
-                 "bad_cells": nbad},
-        "cpu_baseline": None,
+                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_la, "bad_cells": nbad},
     }
-    print(json.dumps(out))
     ens.close()
-    return 0
+    del ens, paths
+    return out
+
+
+def run_thermal(a):
+    """convection_tdep (201x201, 1.44M markers): the single-instance step, its
+    per-phase times, the dominant kernel's roofline and the PCIe-inclusive rate."""
+    import ctypes as C
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import MarkerPath
+
+    w = workloads.convection_tdep()
+    mp = MarkerPath(w)
+    if a.lanes != 8:
+        mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(a.lanes))
+    n0 = w.n_markers
+    for _ in range(a.warmup):
+        mp.step_thermal()
+    mp.ctx.sync()
+    t0 = time.perf_counter()
+    mp.ctx.timer_start()
+    for _ in range(a.steps):
+        mp.step_thermal()
+    ms = mp.ctx.timer_stop()
+    wall = time.perf_counter() - t0
+    nout = mp.find_marker_cells()
+    active = n0 - nout
+    value = active * a.steps / (ms / 1e3)
+    log("convection_tdep: %.3f ms/step" % (ms / a.steps))
+
+    # end-to-end: every step also takes the host solves' outputs in (vx, vy, wxy,
+    # thisT host-to-device) and hands the 17 projected nodal fields back
+    # (device-to-host), as a driver with host-side PETSc solves would
+    in_names = ("vx", "vy", "wxy", "thisT")
+    outs = [p[4] for p in w.projections]
+    nodes = w.NX * w.NY
+    h_in = mp.ctx.pinned((len(in_names), nodes))
+    h_out = mp.ctx.pinned((len(outs), nodes))
+    for k, n in enumerate(in_names):
+        h_in[k] = mp.get_nodal(n).ravel()
+    d_in = (C.c_void_p * len(in_names))(*[mp.nodal.ptr(n) for n in in_names])
+    d_out = (C.c_void_p * len(outs))(*[mp.nodal.ptr(n) for n in outs])
+    e2e_steps = min(a.steps, 20)
+
+    def e2e_loop(packed):
+        mp.ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(e2e_steps):
+            if packed:
+                mp.ctx.call("fvm_nodal_upload", C.c_int64(nodes), len(in_names), C.cast(d_in, C.c_void_p),
+                            h_in.ctypes.data_as(C.c_void_p))
+            else:
+                for k, n in enumerate(in_names):
+                    mp.set_nodal(n, h_in[k].reshape(w.NY, w.NX))
+            mp.step_thermal()
+            if packed:
+                mp.ctx.call("fvm_nodal_download", C.c_int64(nodes), len(outs), C.cast(d_out, C.c_void_p),
+                            h_out.ctypes.data_as(C.c_void_p))
+            else:
+                for n in outs:
+                    mp.get_nodal(n)
+        return (time.perf_counter() - t0) / e2e_steps * 1e3
+    e2e_ms = e2e_loop(True)
+    e2e_each_ms = e2e_loop(False)
+
+    phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_thermal, 5)
+    ab = algorithmic_bytes(w)
+    cands = {k: v for k, v in phase_ms.items() if ab.get(k)}
+    dom = max(cands, key=cands.get)
+    per_launch_ms = phase_ms[dom] / phase_launches[dom]
+    step_bytes = n0 * STEP_BYTES_PER_MARKER + nodes * STEP_BYTES_PER_NODE
+    return {
+        "value": value, "unit": "markers/s", "steps": a.steps, "ms_per_step": ms / a.steps,
+        "config": {"workload": "convection_tdep", "grid": "201x201 uniform", "markers": n0,
+                   "markers_per_cell": 36, "active_markers_after": active, "projections": 17,
+                   "viscosity_contrast": 1e5},
+        "roofline": roofline(dom, dom, ab[dom], per_launch_ms),
+        "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
+                 "wall_ms_per_step": wall / a.steps * 1e3,
+                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
+        "end_to_end": {"ms_per_step": e2e_ms, "markers_per_s": n0 * 1e3 / e2e_ms, "steps": e2e_steps,
+                       "bytes_h2d_per_step": len(in_names) * nodes * 8, "bytes_d2h_per_step": len(outs) * nodes * 8,
+                       "transfer": "pinned, one packed copy each way (fvm_nodal_upload / fvm_nodal_download)",
+                       "ms_per_step_one_copy_per_array": e2e_each_ms},
+    }
 
 
 def copy_calibration(nbytes=512 << 20, reps=5):
@@ -217,42 +382,28 @@ def copy_calibration(nbytes=512 << 20, reps=5):
     return 0
 
 
-def traffic_per_launch(kernel):
-    """HBM bytes per launch of `kernel` from its committed counter summary, if any
-    (a line 'bytes_per_launch: N' in profiles/traffic_<kernel>.md)."""
-    tf = os.path.join(ROOT, "profiles", "traffic_%s.md" % kernel)
-    if not os.path.exists(tf):
-        return None
-    for line in open(tf):
-        if line.strip().startswith("bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
-def bench_learned_guess(a):
+def run_learned_guess(a, N, ntrain, ntest, epochs):
     """The learned Stokes initial guess: plain-method data, U-Net training on the
     GPU kernels, and the host Stokes solve's FGMRES iterations to 1e-8 from the plain,
     previous-solution and learned guesses on held-out states."""
     from fvmic2d_amd import learned_guess as lg
-    N = a.lg_grid
-    t0 = time.perf_counter()
-    raw_tr = lg.generate(N, list(range(a.lg_train)), workers=a.lg_workers)
-    raw_te = lg.generate(N, list(range(10 ** 6, 10 ** 6 + a.lg_test)), workers=a.lg_workers)
-    t_gen = time.perf_counter() - t0
-    print("learned guess: %d + %d states at %dx%d solved on the host in %.0f s" % (
-        a.lg_train, a.lg_test, N, N, t_gen), file=sys.stderr, flush=True)
     from fvmic2d_amd.lib import Context
+    from fvmic2d_amd.unet import Buf
+    workers = a.lg_workers or cpu_workers()[0]
+    t0 = time.perf_counter()
+    # spawned workers: in the default run this process has used the GPU already
+    raw_tr = lg.generate(N, list(range(ntrain)), workers=workers, start="spawn")
+    raw_te = lg.generate(N, list(range(10 ** 6, 10 ** 6 + ntest)), workers=workers, start="spawn")
+    t_gen = time.perf_counter() - t0
+    log("learned guess: %d + %d states at %dx%d solved on the host in %.0f s" % (ntrain, ntest, N, N, t_gen))
     ctx = Context()
     t0 = time.perf_counter()
     tr, te = lg.Dataset(ctx, N, raw_tr), lg.Dataset(ctx, N, raw_te)
     t_pg = time.perf_counter() - t0
     X, T = tr.arrays()
     model = lg.LearnedGuess(ctx, levels=4, base=32, seed=0)
-    log = lambda m: print("learned guess: " + m, file=sys.stderr, flush=True)
-    hist = model.train(X, T, epochs=a.lg_epochs, batch=8, lr=2e-3, log=log)
-    steps = a.lg_epochs * max(1, len(X) // 8)
-    # forward timing at the training batch shape
-    from fvmic2d_amd.unet import Buf
+    hist = model.train(X, T, epochs=epochs, batch=8, lr=2e-3, log=lambda m: log("learned guess: " + m))
+    steps = epochs * max(1, len(X) // 8)
     dX = Buf(ctx, (8, N - 1, N - 1, model.net.cin), np.uint16)
     model.net.forward(dX)
     ctx.sync()
@@ -265,14 +416,13 @@ def bench_learned_guess(a):
     t_eval = time.perf_counter() - t0
     summ = lg.summarize(rows)
     flops_fwd = model_flops(model.net, N - 1) * 8
-    out = {
+    return {
         "metric": "KSP iterations to 1e-8 relative residual: learned vs plain initial guess",
-        "value": summ["mean_iterations"]["learned"], "unit": "iterations", "n_gpus": 1,
-        "higher_is_better": False, "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
-        "data": "synthetic", "scaling": "weak", "steps": steps, "warmup": 0,
+        "value": summ["mean_iterations"]["learned"], "unit": "iterations", "higher_is_better": False,
+        "dtype": "bf16 (fp32 accumulate)", "steps": steps,
         "config": {"workload": "learned_guess_box%d" % N, "grid": "%dx%d" % (N, N),
-                   "viscosity_contrast": lg.CONTRAST, "train_states": a.lg_train, "test_states": a.lg_test,
-                   "unet": "4 levels, 32 base channels, 3x3 convs", "epochs": a.lg_epochs,
+                   "viscosity_contrast": lg.CONTRAST, "train_states": ntrain, "test_states": ntest,
+                   "unet": "4 levels, 32 base channels, 3x3 convs", "epochs": epochs,
                    "stokes_solve": "FGMRES(100) + u/p field split (LU velocity block, diagonal Schur), rtol 1e-8"},
         "learned_guess": summ,
         "train_loss_first_last": [hist[0], hist[-1]],
@@ -282,10 +432,7 @@ def bench_learned_guess(a):
         "roofline": {"bound": "mfma", "kernel": "unet forward (batch 8)",
                      "achieved": flops_fwd / (fwd_ms / 1e3) / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": flops_fwd / (fwd_ms / 1e3) / 1e12 / 2500.0, "traffic": None},
-        "cpu_baseline": None,
     }
-    print(json.dumps(out))
-    return 0
 
 
 def bench_pressure_guess(a):
@@ -368,7 +515,7 @@ def vep_algorithmic_bytes(w):
     }
 
 
-def cpu_baseline_vep(w, budget_s=12.0):
+def cpu_baseline_vep(w, budget_s):
     """The oracle's visco-elasto-plastic step on the same workload, one thread,
     whole steps until budget_s (at least one)."""
     import oracle
@@ -383,10 +530,10 @@ def cpu_baseline_vep(w, budget_s=12.0):
     return {"value": w.n_markers * steps / el, "unit": "markers/s", "cores": 1, "kind": "port",
             "sample": "%d full vep_subduction steps (%d markers each) of the C restatement, "
                       "gcc -O3 -ffp-contract=off, one thread, %.1f s" % (steps, w.n_markers, el),
-            "cpu": platform.processor() or platform.machine()}
+            "cpu": host_cpu()[0]}
 
 
-def bench_vep(a):
+def run_vep(a, extras=True):
     """The visco-elasto-plastic marker step at full size: 801 x 401 tanh-stretched
     grid, 25 markers per cell (8M markers), five materials with Drucker-Prager
     yielding -- locate + density, yielding, viscosity limits, projections,
@@ -397,7 +544,7 @@ def bench_vep(a):
     from fvmic2d_amd.host import MarkerPath
     t0 = time.perf_counter()
     w = workloads.vep_subduction()
-    print("vep: workload generated in %.0f s" % (time.perf_counter() - t0), file=sys.stderr, flush=True)
+    log("vep: workload generated in %.0f s" % (time.perf_counter() - t0))
     mp = MarkerPath(w)
     n0 = w.n_markers
     for _ in range(a.warmup):
@@ -412,40 +559,30 @@ def bench_vep(a):
     nout = mp.find_marker_cells()
     active = n0 - nout
     value = active * a.steps / (ms / 1e3)
-    print("vep: %.3f ms per step" % (ms / a.steps), file=sys.stderr, flush=True)
+    log("vep: %.3f ms per step" % (ms / a.steps))
     phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_vep, 3)
     ab = vep_algorithmic_bytes(w)
     cands = {k: v / phase_launches[k] for k, v in phase_ms.items() if ab.get(k)}
     dom = max(cands, key=cands.get)
-    per_launch_ms = cands[dom]
-    achieved = ab[dom] / (per_launch_ms / 1e3) / 1e9
     step_bytes = n0 * VEP_STEP_BYTES_PER_MARKER + w.NX * w.NY * VEP_STEP_BYTES_PER_NODE
     out = {
-        "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "value": value, "unit": "markers/s", "steps": a.steps, "ms_per_step": ms / a.steps,
         "config": {"workload": "vep_subduction", "grid": "801x401 tanh-stretched", "markers": n0,
                    "markers_per_cell": 25, "active_markers_after": active, "materials": 5,
                    "plasticity": "Drucker-Prager", "yielding_last_step": y, "residual_last_step": r},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(dom + "_vep"),
-                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": per_launch_ms},
-        "step": {"algorithmic_bytes": step_bytes,
-                 "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
+        "roofline": roofline(dom, dom + "_vep", ab[dom], cands[dom]),
+        "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
                  "wall_ms_per_step": wall / a.steps * 1e3,
                  "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
     }
-    out["other_entry_points"] = bench_other_entry_points(mp, w)
-    out["nodal_step"] = bench_nodal_step(a)
-    out["thermal_solve"] = bench_thermal_solve(a)
+    if extras:
+        out["other_entry_points"] = bench_other_entry_points(mp, w)
+        out["nodal_step"] = bench_nodal_step(a)
+        out["thermal_solve"] = bench_thermal_solve(a)
     if not a.no_cpu:
-        cb = cpu_baseline_vep(w, a.cpu_budget)
-        out["cpu_baseline"] = cb
-        out["speedup_vs_cpu"] = value / cb["value"]
-    else:
-        out["cpu_baseline"] = None
-    print(json.dumps(out))
-    return 0
+        out["cpu_baseline"] = cpu_baseline_vep(w, a.vep_cpu_budget)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    return out
 
 
 def bench_other_entry_points(mp, w, reps=10):
@@ -584,18 +721,22 @@ def model_flops(net, H):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--ensemble", action="store_true",
-                    help="time the batched 64-instance ensemble instead (own JSON line)")
+    ap.add_argument("--vep-cpu-budget", type=float, default=5.0)
     ap.add_argument("--lanes", type=int, default=8, help="lanes per lattice cell in the projections")
     ap.add_argument("--instances", type=int, default=64)
     ap.add_argument("--ens-nmin", type=int, default=65)
     ap.add_argument("--ens-nmax", type=int, default=257)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default line: the ensemble only (no single-instance or learned-guess keys)")
+    ap.add_argument("--thermal", action="store_true",
+                    help="the single-instance convection_tdep step as the line (201x201, 1.44M markers)")
+    ap.add_argument("--ensemble", action="store_true", help="the ensemble line only (no extra keys)")
     ap.add_argument("--vep", action="store_true",
-                    help="the visco-elasto-plastic step at 801x401, 8M markers")
+                    help="the visco-elasto-plastic step at 801x401, 8M markers, as the line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration)")
     ap.add_argument("--learned-guess", action="store_true",
@@ -604,126 +745,58 @@ def main():
     ap.add_argument("--lg-train", type=int, default=512)
     ap.add_argument("--lg-test", type=int, default=24)
     ap.add_argument("--lg-epochs", type=int, default=40)
-    ap.add_argument("--lg-workers", type=int, default=12)
+    ap.add_argument("--lg-workers", type=int, default=0, help="host solver processes (0: this job's cores)")
     ap.add_argument("--pressure-guess", action="store_true",
                     help="time the plain initial pressure guess (own JSON line)")
     ap.add_argument("--pg-grids", type=int, nargs="+", default=[129, 257, 513])
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: the marker path of a single instance runs on one MI355X "
-              "and this library has no multi-GPU path.", file=sys.stderr)
+        print("bench.py measures one GPU: every GPU box here has one MI355X, and this library has no "
+              "multi-GPU path (the design for one is described, not built).", file=sys.stderr)
         return 2
     if a.copy_calibration:
         return copy_calibration()
     if a.learned_guess:
-        return bench_learned_guess(a)
+        out = run_learned_guess(a, a.lg_grid, a.lg_train, a.lg_test, a.lg_epochs)
+        out.update({"n_gpus": 1, "warmup": 0, "vs_baseline": None, "data": "synthetic", "scaling": "weak",
+                    "cpu_baseline": None})
+        print(json.dumps(out))
+        return 0
     if a.pressure_guess:
         return bench_pressure_guess(a)
-    if a.ensemble:
-        return bench_ensemble(a)
-    if a.vep:
-        return bench_vep(a)
 
-    # the all-core CPU figure forks its workers before anything touches the GPU
-    all_cores = None if a.no_cpu else cpu_baseline_all_cores(a.cpu_budget)
+    common = {"metric": METRIC, "n_gpus": 1, "warmup": a.warmup, "higher_is_better": True, "scaling": "weak",
+              "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    if a.thermal or a.vep:
+        cpu_all = None if a.no_cpu or a.vep else cpu_all_cores("tdep", a.cpu_budget)
+        out = run_thermal(a) if a.thermal else run_vep(a)
+        line = dict(common, **out)
+        if a.thermal and not a.no_cpu:
+            line["cpu_baseline"] = cpu_baseline_tdep(a.cpu_budget)
+            line["cpu_baseline_all_cores"] = cpu_all
+        line.setdefault("cpu_baseline", None)
+        print(json.dumps(line))
+        return 0
 
-    from fvmic2d_amd import workloads
-    from fvmic2d_amd.host import MarkerPath
-
-    w = workloads.convection_tdep()
-    mp = MarkerPath(w)
-    if a.lanes != 8:
-        import ctypes as C
-        mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(a.lanes))
-    n0 = w.n_markers
-    for _ in range(a.warmup):
-        mp.step_thermal()
-    mp.ctx.sync()
-    t0 = time.perf_counter()
-    mp.ctx.timer_start()
-    for _ in range(a.steps):
-        mp.step_thermal()
-    ms = mp.ctx.timer_stop()
-    wall = time.perf_counter() - t0
-    nout = mp.find_marker_cells()
-    active = n0 - nout
-    value = active * a.steps / (ms / 1e3)
-
-    # end-to-end: every step also takes the host solves' outputs in (vx, vy, wxy,
-    # thisT host-to-device) and hands the 17 projected nodal fields back
-    # (device-to-host), as a driver with host-side PETSc solves would
-    import ctypes as C
-    in_names = ("vx", "vy", "wxy", "thisT")
-    outs = [p[4] for p in w.projections]
-    nodes = w.NX * w.NY
-    # packed exchange: the 4 inputs in one pinned block, one copy each way per step
-    h_in = mp.ctx.pinned((len(in_names), nodes))
-    h_out = mp.ctx.pinned((len(outs), nodes))
-    for k, n in enumerate(in_names):
-        h_in[k] = mp.get_nodal(n).ravel()
-    d_in = (C.c_void_p * len(in_names))(*[mp.nodal.ptr(n) for n in in_names])
-    d_out = (C.c_void_p * len(outs))(*[mp.nodal.ptr(n) for n in outs])
-    e2e_steps = min(a.steps, 20)
-
-    def e2e_loop(packed):
-        mp.ctx.sync()
-        t0 = time.perf_counter()
-        for _ in range(e2e_steps):
-            if packed:
-                mp.ctx.call("fvm_nodal_upload", C.c_int64(nodes), len(in_names), C.cast(d_in, C.c_void_p),
-                            h_in.ctypes.data_as(C.c_void_p))
-            else:
-                for k, n in enumerate(in_names):
-                    mp.set_nodal(n, h_in[k].reshape(w.NY, w.NX))
-            mp.step_thermal()
-            if packed:
-                mp.ctx.call("fvm_nodal_download", C.c_int64(nodes), len(outs), C.cast(d_out, C.c_void_p),
-                            h_out.ctypes.data_as(C.c_void_p))
-            else:
-                for n in outs:
-                    mp.get_nodal(n)
-        return (time.perf_counter() - t0) / e2e_steps * 1e3
-    e2e_ms = e2e_loop(True)
-    e2e_each_ms = e2e_loop(False)
-    end_to_end = {"ms_per_step": e2e_ms, "markers_per_s": n0 * 1e3 / e2e_ms, "steps": e2e_steps,
-                  "bytes_h2d_per_step": len(in_names) * nodes * 8, "bytes_d2h_per_step": len(outs) * nodes * 8,
-                  "transfer": "pinned, one packed copy each way (fvm_nodal_upload / fvm_nodal_download)",
-                  "ms_per_step_one_copy_per_array": e2e_each_ms}
-
-    # per-kernel timing pass: every launch bracketed by HIP events on the same stream
-    phase_ms, phase_launches = phase_profile(mp.ctx, mp.step_thermal, 5)
-    ab = algorithmic_bytes(w)
-    cands = {k: v for k, v in phase_ms.items() if ab.get(k)}
-    dom = max(cands, key=cands.get)
-    per_launch_ms = phase_ms[dom] / phase_launches[dom]
-    achieved = ab[dom] / (per_launch_ms / 1e3) / 1e9
-    traffic = traffic_per_launch(dom)
-    step_bytes = n0 * STEP_BYTES_PER_MARKER + w.NX * w.NY * STEP_BYTES_PER_NODE
-    out = {
-        "metric": METRIC, "value": value, "unit": "markers/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms / a.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "convection_tdep", "grid": "201x201 uniform", "markers": n0,
-                   "markers_per_cell": 36, "active_markers_after": active,
-                   "projections": 17, "viscosity_contrast": 1e5},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_launch": ab[dom], "avg_launch_ms": per_launch_ms},
-        "step": {"algorithmic_bytes": step_bytes,
-                 "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
-                 "wall_ms_per_step": wall / a.steps * 1e3,
-                 "phase_ms_per_step": phase_ms, "phase_launches_per_step": phase_launches},
-        "end_to_end": end_to_end,
-    }
-    if not a.no_cpu:
-        cb = cpu_baseline(a.cpu_budget)
-        out["cpu_baseline"] = cb
-        out["speedup_vs_cpu"] = value / cb["value"]
-        out["cpu_baseline_all_cores"] = all_cores
-        out["speedup_vs_cpu_all_cores"] = value / all_cores["value"]
-    else:
-        out["cpu_baseline"] = None
-    print(json.dumps(out))
+    # default: the batched ensemble; the CPU baselines fork before the GPU is touched
+    cpu_all = None if a.no_cpu else cpu_all_cores("ensemble", a.cpu_budget)
+    cpu_one = None if a.no_cpu else cpu_baseline_ensemble(a.cpu_budget)
+    line = dict(common, **run_ensemble(a))
+    line["cpu_baseline"] = cpu_one
+    if cpu_one:
+        line["speedup_vs_cpu"] = line["value"] / cpu_one["value"]
+        line["cpu_baseline_all_cores"] = cpu_all
+        line["speedup_vs_cpu_all_cores"] = line["value"] / cpu_all["value"]
+    if not (a.no_extras or a.ensemble):
+        line["convection_tdep"] = run_thermal(a)
+        log("single instance convection_tdep done")
+        line["vep_subduction"] = run_vep(a, extras=False)
+        log("vep_subduction done")
+        try:
+            line["learned_guess"] = run_learned_guess(a, 65, 48, 6, 20)
+        except Exception as e:  # the marker-step figures stand on their own
+            line["learned_guess"] = {"error": "%s: %s" % (type(e).__name__, e)}
+    print(json.dumps(line))
     return 0
 
 

@@ -80,7 +80,8 @@ extern "C" int fvm_ctx_destroy(fvm_ctx *ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   fvm_buf *bufs[] = {&ctx->key,      &ctx->slot,     &ctx->perm,    &ctx->count,   &ctx->start,
                      &ctx->fill,     &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
-                     &ctx->red,      &ctx->payload,   &ctx->payload_xy, &ctx->xfer};
+                     &ctx->red,      &ctx->payload,   &ctx->payload_xy, &ctx->xfer,    &ctx->popc,
+                     &ctx->popc2};
   for (fvm_buf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
@@ -104,7 +105,7 @@ extern "C" size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx) {
   if (!ctx) return 0;
   const fvm_buf *bufs[] = {&ctx->key,  &ctx->slot,    &ctx->perm,     &ctx->count,   &ctx->start,
                            &ctx->fill, &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
-                           &ctx->red,  &ctx->payload,   &ctx->payload_xy};
+                           &ctx->red,  &ctx->payload,   &ctx->payload_xy, &ctx->popc, &ctx->popc2};
   size_t s = 0;
   for (const fvm_buf *b : bufs) s += b->bytes;
   return s;

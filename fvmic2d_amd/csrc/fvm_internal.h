@@ -25,6 +25,7 @@ struct fvm_ctx {
   // workspace buffers, grown on demand
   fvm_buf key, slot, perm, count, start, fill, blocksums, partials, scratch, staging, red;
   fvm_buf xfer;  // packed nodal arrays of fvm_nodal_upload / fvm_nodal_download
+  fvm_buf popc, popc2;  // population control: per-quarter plan and out list; added markers
   fvm_buf payload;     // bucket-ordered copies of the marker columns a projection reads
   fvm_buf payload_xy;  // bucket-ordered X, Y, Mat: valid while xy_serial == sort_serial
   uint64_t sort_serial = 0, xy_serial = 0;

@@ -6,6 +6,7 @@
 #include <float.h>
 
 #include "marker_common.h"
+#include "rheology.h"
 #include "proj_common.h"
 
 int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
@@ -443,50 +444,6 @@ extern "C" int fvm_update_marker_strain_pressure(fvm_ctx *ctx, const fvm_grid *g
 
 // ---- A11: checkPlasticYielding + computeRheology (markers.c:1381-1505,
 //           viscosity.c:12-75) ------------------------------------------------
-struct YieldParams {
-  Materials mat;
-  double etamin, etamax, fractionalEtamin;
-  double slabAngle, decoupleThickness, minDecoupleDepth, maxDecoupleDepth, decoupleEtaScaler;
-  double dt;
-  // sin(F[m][k] / 180 * pi) and tan(slabAngle), evaluated once on the host with
-  // the C library the reference calls (markers.c:1444-1462, viscosity.c:55)
-  double sinF[FVM_MAXMAT][2];
-  double tanSlab;
-};
-
-__device__ __forceinline__ int rheology(const YieldParams &P, int mm, double T, double X, double Y, double *eta,
-                                        double *mu) {
-  const Materials &mat = P.mat;
-  const double thismu = mat.materialMu[mm];
-  const int law = mat.hasEtaT[mm];
-  double thiseta;
-  if (law == 1) {
-    thiseta = mat.materialEta[mm] * exp(mat.QonR[mm] * (1.0 / T - 1.0 / mat.Tref[mm]));
-  } else if (law == 2) {
-    const double T0 = mat.Tref[mm];
-    thiseta = mat.materialEta[mm] * exp(mat.QonR[mm] / T0 * (1.0 - (T - T0) / T0));
-  } else if (law == 3) {
-    thiseta = mat.materialEta[mm] * T;
-  } else if (law == 5) {
-    thiseta = mat.materialEta[mm] * exp(-mat.QonR[mm] * T);
-  } else if (law == 0) {
-    thiseta = mat.materialEta[mm];
-  } else if (law == 6) {
-    const double depth = Y, distance = X;
-    thiseta = mat.materialEta[mm];
-    if (distance > depth / P.tanSlab) {
-      if (distance < (depth / P.tanSlab + P.decoupleThickness)) {
-        if (depth > P.minDecoupleDepth && depth < P.maxDecoupleDepth) thiseta *= P.decoupleEtaScaler;
-      }
-    }
-  } else {
-    return 1;
-  }
-  *eta = thiseta;
-  *mu = thismu;
-  return 0;
-}
-
 // checkPlasticYielding's per-marker body (markers.c:1381-1505 with the viscosity
 // law of viscosity.c:12-104): 0, 1 when the marker yields, 2 when its material
 // or viscosity law is undefined.
@@ -595,20 +552,7 @@ extern "C" int fvm_check_plastic_yielding(fvm_ctx *ctx, const fvm_grid *g, fvm_m
   (void)g;
   (void)elapsedTime;
   if (!ctx || !mk || !mat || !opt) return FVM_ERR_ARG;
-  YieldParams P;
-  P.mat = *mat;
-  P.etamin = opt->etamin;
-  P.etamax = opt->etamax;
-  P.fractionalEtamin = opt->fractionalEtamin;
-  P.slabAngle = opt->slabAngle;
-  P.decoupleThickness = opt->decoupleThickness;
-  P.minDecoupleDepth = opt->minDecoupleDepth;
-  P.maxDecoupleDepth = opt->maxDecoupleDepth;
-  P.decoupleEtaScaler = opt->decoupleEtaScaler;
-  P.dt = dt;
-  for (int i = 0; i < FVM_MAXMAT; i++)
-    for (int k = 0; k < 2; k++) P.sinF[i][k] = sin(mat->F[i][k] / 180.0 * M_PI);
-  P.tanSlab = tan(opt->slabAngle);
+  const YieldParams P = make_yield_params(mat, opt, dt);
   FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
   int32_t *flags = (int32_t *)ctx->red.ptr + 16;
   FVM_HIP(ctx, hipMemsetAsync(flags, 0, 2 * sizeof(int32_t), ctx->stream));

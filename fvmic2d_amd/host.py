@@ -181,6 +181,13 @@ class MarkerPath:
                   C.byref(self.w.materials), dt, C.byref(y), C.byref(self.w.options), 0.0)
         return y.value
 
+    def check_marker_density(self, seed):
+        """checkMarkerDensity (markers.c:455-781); returns (added, removed)."""
+        a, r = I32(0), I32(0)
+        self.call("fvm_check_marker_density", self.grid.ref, self.markers.ref, C.byref(self.w.materials),
+                  C.byref(self.w.options), C.c_uint64(seed), C.byref(a), C.byref(r))
+        return a.value, r.value
+
     def limit_viscosity(self):
         self.call("fvm_limit_viscosity", self.markers.ref, C.byref(self.w.options))
 

@@ -65,6 +65,7 @@ _SIGS = {
     "fvm_ctx_stream": [P],
     "fvm_ctx_sync": [P],
     "fvm_ctx_set_jaumann": [P, I32],
+    "fvm_check_marker_density": [P, P, P, P, P, C.c_uint64, C.POINTER(I32), C.POINTER(I32)],
     "fvm_limit_displacement_timestep_noalloc": [P, P, P, P, P],
     "fvm_ctx_workspace_bytes": [P],
     "fvm_ctx_set_lanes": [P, I32],

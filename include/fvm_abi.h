@@ -50,7 +50,9 @@ enum {
   FVM_ERR_TOO_MANY_FIELDS = 10,  /* more projected fields in one group than supported */
   FVM_ERR_NOT_CONVERGED = 11,    /* an iterative solve missed its tolerance */
   FVM_ERR_BC = 12,               /* a boundary type whose ghost value the reference leaves undefined */
-  FVM_ERR_IO = 13                /* a marker file could not be opened, read or written, or has another layout */
+  FVM_ERR_IO = 13,               /* a marker file could not be opened, read or written, or has another layout */
+  FVM_ERR_CAPACITY = 14,         /* markers.c:772-775: more markers to add than free slots (reference aborts) */
+  FVM_ERR_NEGATIVE_ADD = 15      /* markers.c:548-551: an under-full quarter's target is not above its count (aborts) */
 };
 
 /* ---- device marker store: one device column per Marker field --------------- */
@@ -303,6 +305,32 @@ int fvm_restart_from_markers(fvm_ctx *ctx, fvm_markers *mk, const char *path, do
  * written.  PETSc's companion .info file is not written.  Synchronises. */
 int fvm_save_nodal_fields(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, const double *ha,
                           const char *path);
+
+/* ---- marker population control ------------------------------------------ */
+/* markers.c:455-781 checkMarkerDensity(problem, markerset, grid, options, r), one
+ * rank; called at main_subduction.c:250 after findMarkerCells, whose cells it
+ * reads.  Quarter cells with fewer than opt->minMarkers markers are refilled to
+ * ceil(NMX*(NX-1)/2.0*dx/LX) * ceil(NMY*(NY-1)/2.0*dy/LY) (dx, dy the quarter's
+ * size), those with more than opt->maxMarkers lose the excess; added markers
+ * recycle the slots of markers without a cell (last first) and then grow
+ * mk->nMark up to mk->maxMark.  An added marker left of x[1] is a slab-inflow
+ * marker, one right of x[NX-2] below plate_depth(LX) a mantle-inflow marker
+ * (benchmarkInitialConditions.c:52-92, viscosity from material 0's law);
+ * any other copies every field of the nearest marker within one cell width
+ * and height of that marker's cell (slots refilled earlier in the call
+ * included), keeping its own position, or stays as it is when there is none.
+ * Random numbers: the reference's drand48 / PetscRandom are replaced by the
+ * counter stream u(seed, s, c) = (mix(mix(seed + G (s + 1)) ^ c) >> 11) / 2^53,
+ * mix the SplitMix64 output function, G = 0x9E3779B97F4A7C15: stream 1 with
+ * c = q << 32 | k keys the k-th member (in index order) of over-full quarter q
+ * (q = i + 2(NX-1) j, the reference's order) and the lowest keys leave; streams
+ * 2 and 3 with c = the added marker's number give its x and y,
+ * (hi - lo) * u + lo within its quarter.  Synchronises.  *nAdded is 0 when
+ * nAdd + nMark - nOut equals maxMark (the reference then neither adds nor
+ * aborts); FVM_ERR_CAPACITY when it exceeds it, FVM_ERR_NEGATIVE_ADD when a
+ * target does not exceed its count (both abort in the reference). */
+int fvm_check_marker_density(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const Materials *mat,
+                             const Options *opt, uint64_t seed, int32_t *nAdded, int32_t *nRemoved);
 
 /* ---- whole marker step, in the reference's call order (main_subduction.c) -- */
 /* Thermal-convection step: findMarkerCells (:243), density loop (:255-264),

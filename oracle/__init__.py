@@ -42,8 +42,10 @@ _SIGS = {
     "orc_limit_displacement_timestep": [P, P, C.POINTER(D), D],
     "orc_save_markers_binary": [P, C.c_char_p, D],
     "orc_restart_from_markers": [P, C.c_char_p, C.POINTER(D)],
+    "orc_check_marker_density": [P, P, P, P, C.c_uint64, C.POINTER(I), C.POINTER(I)],
+    "orc_pc_u01": [C.c_uint64, C.c_uint64, C.c_uint64],
 }
-_RES = {"orc_max_marker_T": D, "orc_max_abs_sxx": D, "orc_limit_viscosity": None,
+_RES = {"orc_max_marker_T": D, "orc_max_abs_sxx": D, "orc_limit_viscosity": None, "orc_pc_u01": D,
         "orc_update_marker_density": None}
 
 
@@ -68,10 +70,15 @@ class OracleError(RuntimeError):
 class State:
     """Host copy of a workload: AoS markers, padded grid, nodal arrays."""
 
-    def __init__(self, w, nodal_names=abi.NODAL_NAMES, extra=()):
+    def __init__(self, w, nodal_names=abi.NODAL_NAMES, extra=(), capacity=None):
         self.lib = load()
         self.w = w
-        self.markers = np.ascontiguousarray(w.markers.copy())
+        n = len(w.markers)
+        cap = max(capacity or n, n)
+        self.markers = np.zeros(cap, dtype=abi.MARKER_DTYPE)
+        self.markers[:n] = w.markers
+        self.markers["cellX"][n:] = -1  # unused slots: zeros, no cell
+        self.markers["cellY"][n:] = -1
         self.xp, self.yp = w.xp.copy(), w.yp.copy()
         self.xcp, self.ycp = w.xcp.copy(), w.ycp.copy()
         s = 8
@@ -81,7 +88,7 @@ class State:
         self.grid.y = self.yp.ctypes.data + 2 * s
         self.grid.xc = self.xcp.ctypes.data + 2 * s
         self.grid.yc = self.ycp.ctypes.data + 2 * s
-        self.ms = abi.MarkerSet(len(self.markers), len(self.markers), 0, self.markers.ctypes.data)
+        self.ms = abi.MarkerSet(n, cap, 0, self.markers.ctypes.data)
         self.nodal = {}
         for n in list(nodal_names) + list(extra):
             a = w.nodal.get(n)
@@ -194,6 +201,14 @@ class State:
         self._c("orc_restart_from_markers", C.byref(self.ms), str(path).encode(), C.byref(t))
         self.markers = self.markers[:capacity]
         return t.value
+
+    def check_marker_density(self, seed):
+        """checkMarkerDensity (markers.c:455-781) with the counter stream orc_pc_u01;
+        returns (added, removed).  The active markers are markers[:ms.nMark]."""
+        a, r = I(0), I(0)
+        self._c("orc_check_marker_density", C.byref(self.ms), C.byref(self.grid), C.byref(self.w.options),
+                C.byref(self.w.materials), C.c_uint64(seed), C.byref(a), C.byref(r))
+        return a.value, r.value
 
     def limit_viscosity(self):
         self.lib.orc_limit_viscosity(C.byref(self.ms), C.byref(self.w.options))

@@ -1170,3 +1170,237 @@ int orc_restart_from_markers(MarkerSet *ms, const char *path, double *elapsedTim
   ms->nOut = 0;
   return ok ? ORC_OK : ORC_ERR_IO;
 }
+
+/* ---- markers.c:455-781 checkMarkerDensity (one rank) ---------------------
+ * The reference draws its random numbers from drand48 (which markers leave an
+ * over-full quarter cell) and PetscRandom (where added markers go), neither of
+ * which can be reproduced here; both are replaced by the counter-based stream
+ * orc_pc_u01 below, which the GPU path uses as well:
+ *   u(seed, stream, ctr) = (mix(mix(seed + G*(stream+1)) ^ ctr) >> 11) * 2^-53,
+ *   mix = the SplitMix64 output function, G = 0x9E3779B97F4A7C15;
+ *   stream 1, ctr (q << 32 | k): removal key of the k-th member of quarter q;
+ *   stream 2 / 3, ctr iMark: x / y draw of the iMark-th added marker.
+ * Everything else follows the reference: quarter cells in its order (j over
+ * 2(NY-1) rows outside, i over 2(NX-1) columns inside), its target count
+ * ceil(NMX*(NX-1)/2.0*dx/LX)*ceil(...) per under-full quarter, removal of the
+ * lowest keys (first index on ties) with geometric membership, the out list in
+ * index order followed by the removed markers, recycling from its end, the
+ * slab and inflow markers of benchmarkInitialConditions.c:52-92, and the
+ * nearest-neighbour copy over the original markers m1 < nMark with the
+ * one-cell-width window (recycled slots already filled count as candidates).
+ * Two departures, both where the reference has no defined result:
+ *   - a quarter with fewer geometric members than markers to remove (members
+ *     on x = LX or y = LY are counted but not members) loses only its members;
+ *     the reference would re-append its first member or read an unset index;
+ *   - cpu and strainRateResidual of a slab or inflow marker are 0 (the
+ *     reference copies an uninitialised local).
+ * The reference aborts where this returns ORC_ERR_NEGATIVE_ADD (the target of
+ * an under-full quarter does not exceed its count) or ORC_ERR_CAPACITY. */
+static uint64_t pc_mix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+double orc_pc_u01(uint64_t seed, uint64_t stream, uint64_t ctr) {
+  const uint64_t key = pc_mix(seed + 0x9E3779B97F4A7C15ull * (stream + 1));
+  return (double)(pc_mix(key ^ ctr) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* benchmarkInitialConditions.c:80-92 plate_depth */
+static double pc_plate_depth(double x, const Options *o) {
+  if (fabs(x - o->rootCenter) < o->rootWidth)
+    return o->overridingPlateThickness +
+           o->rootThickness * pow(cos(M_PI * (x - o->rootCenter) / o->rootWidth / 2.0), 2.0);
+  return o->overridingPlateThickness;
+}
+
+/* markers.c:255-330 resetMarker */
+static void pc_reset_marker(Marker *m) {
+  memset(m, 0, sizeof(*m)); /* fields resetMarker leaves unset: 0 here */
+  m->X = -1;
+  m->Y = -1;
+  m->Mat = -1;
+}
+
+/* benchmarkInitialConditions.c:52-72 new_slab_marker / new_inflow_marker */
+static int pc_new_marker(Marker *m, double x, double y, int slab, const Options *o, const Materials *mat) {
+  pc_reset_marker(m);
+  m->X = x;
+  m->Y = y;
+  m->Mat = 0;
+  if (slab) {
+    const double Ts = 273, T0 = 1623, kappa = 0.7272e-6, s_in_year = 3.15576e7;
+    const double slab_age_s = o->slabAge * 1.0e6 * s_in_year;
+    m->T = Ts + (T0 - Ts) * erf((y - x * tan(o->slabAngle)) / (2.0 * sqrt(kappa * slab_age_s)));
+  } else {
+    m->T = 1573.0;
+  }
+  return orc_compute_rheology(m, o, mat, &m->eta, &m->mu);
+}
+
+int orc_check_marker_density(MarkerSet *ms, const GridData *g, const Options *o, const Materials *mat,
+                             uint64_t seed, int *nAdded, int *nRemoved) {
+  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  const int NX = g->NX, NY = g->NY;
+  const int m = NX - 1, n = NY - 1, W = 2 * m;
+  const int nq = 4 * m * n;
+  const int nMark = ms->nMark;
+  Marker *mk = ms->markers;
+  int *cnt = (int *)calloc((size_t)nq, sizeof(int));
+  int *isout = (int *)malloc((size_t)(nMark > 0 ? nMark : 1) * sizeof(int));
+  if (!cnt || !isout) return ORC_ERR_NOMEM;
+  int nOut = 0;
+  for (int m1 = 0; m1 < nMark; m1++) {
+    if (mk[m1].cellX != -1) {
+      const int cx = 2 * mk[m1].cellX + (mk[m1].X >= g->xc[mk[m1].cellX + 1] ? 1 : 0);
+      const int cy = 2 * mk[m1].cellY + (mk[m1].Y >= g->yc[mk[m1].cellY + 1] ? 1 : 0);
+      cnt[cx + cy * W] += 1;
+    } else {
+      isout[nOut++] = m1;
+    }
+  }
+  int nAdd = 0, nRemove = 0, err = ORC_OK;
+  for (int j = 0; j < 2 * n; j++) {
+    for (int i = 0; i < W; i++) {
+      int *c = &cnt[i + W * j];
+      if (*c < o->minMarkers) {
+        double dy = g->y[j / 2 + 1] - g->y[j / 2];
+        double dx = g->x[i / 2 + 1] - g->x[i / 2];
+        dx /= 2.0;
+        dy /= 2.0;
+        const int mynmx = (int)ceil(o->NMX * (g->NX - 1) / 2.0 * dx / g->LX);
+        const int mynmy = (int)ceil(o->NMY * (g->NY - 1) / 2.0 * dy / g->LY);
+        const int mynadd = mynmx * mynmy - *c;
+        if (mynadd <= 0) err = ORC_ERR_NEGATIVE_ADD;
+        nAdd += mynadd;
+        *c = mynadd;
+      } else if (*c > o->maxMarkers) {
+        const int r = *c - o->maxMarkers;
+        nRemove += r;
+        *c = -r;
+      } else {
+        *c = 0;
+      }
+    }
+  }
+  if (err) {
+    free(cnt);
+    free(isout);
+    return err;
+  }
+  /* removals */
+  const int nOutBeforeRemoval = nOut;
+  isout = (int *)realloc(isout, (size_t)(nOut + nRemove + 1) * sizeof(int));
+  if (!isout) return ORC_ERR_NOMEM;
+  if (nRemove > 0) {
+    int *idx = (int *)malloc((size_t)(nMark > 0 ? nMark : 1) * sizeof(int));
+    double *key = (double *)malloc((size_t)(nMark > 0 ? nMark : 1) * sizeof(double));
+    if (!idx || !key) return ORC_ERR_NOMEM;
+    for (int j = 0; j < 2 * n; j++) {
+      const double ymin = !(j % 2) ? g->y[j / 2] : g->yc[(j + 1) / 2];
+      const double ymax = !(j % 2) ? g->yc[j / 2 + 1] : g->y[(j + 1) / 2];
+      for (int i = 0; i < W; i++) {
+        const double xmin = !(i % 2) ? g->x[i / 2] : g->xc[(i + 1) / 2];
+        const double xmax = !(i % 2) ? g->xc[i / 2 + 1] : g->x[(i + 1) / 2];
+        const int q = i + W * j;
+        if (cnt[q] >= 0) continue;
+        const int nToRemove = -cnt[q];
+        int nCell = 0;
+        for (int im = 0; im < nMark; im++)
+          if (mk[im].X >= xmin && mk[im].X < xmax && mk[im].Y >= ymin && mk[im].Y < ymax) idx[nCell++] = im;
+        for (int k = 0; k < nCell; k++) key[k] = orc_pc_u01(seed, 1, ((uint64_t)q << 32) | (uint64_t)k);
+        for (int irem = 0; irem < nToRemove && irem < nCell; irem++) {
+          double minv = 2.0;
+          int minidx = -1;
+          for (int k = 0; k < nCell; k++)
+            if (key[k] < minv) {
+              minv = key[k];
+              minidx = k;
+            }
+          key[minidx] = 2.0;
+          Marker *r = &mk[idx[minidx]];
+          r->X = -1.0;
+          r->Y = -1.0;
+          r->cellX = -1;
+          r->cellY = -1;
+          isout[nOut++] = idx[minidx];
+        }
+      }
+    }
+    free(idx);
+    free(key);
+  }
+  const int removedActual = nOut - nOutBeforeRemoval;
+  int added = 0;
+  if (nAdd > 0 && (nAdd + ms->nMark - nOut) < ms->maxMark) {
+    double *newX = (double *)malloc((size_t)nAdd * sizeof(double));
+    double *newY = (double *)malloc((size_t)nAdd * sizeof(double));
+    if (!newX || !newY) return ORC_ERR_NOMEM;
+    int iMark = 0;
+    for (int j = 0; j < 2 * n; j++) {
+      for (int i = 0; i < W; i++) {
+        const int q = i + W * j;
+        for (int k = 0; k < cnt[q]; k++) {
+          double lo, hi;
+          if (!(i % 2)) lo = g->x[i / 2], hi = g->xc[i / 2 + 1];
+          else lo = g->xc[(i + 1) / 2], hi = g->x[(i + 1) / 2];
+          newX[iMark] = (hi - lo) * orc_pc_u01(seed, 2, (uint64_t)iMark) + lo;
+          if (!(j % 2)) lo = g->y[j / 2], hi = g->yc[j / 2 + 1];
+          else lo = g->yc[(j + 1) / 2], hi = g->y[(j + 1) / 2];
+          newY[iMark] = (hi - lo) * orc_pc_u01(seed, 3, (uint64_t)iMark) + lo;
+          iMark++;
+        }
+      }
+    }
+    const double pd = pc_plate_depth(g->LX, o);
+    for (int i = 0; i < nAdd; i++) {
+      int id;
+      if (nOut > 0) {
+        id = isout[nOut - 1];
+        nOut--;
+      } else {
+        id = ms->nMark;
+        ms->nMark++;
+      }
+      if (newX[i] < g->x[1] || (newX[i] >= g->x[NX - 2] && newY[i] >= pd)) {
+        const int e = pc_new_marker(&mk[id], newX[i], newY[i], newX[i] < g->x[1], o, mat);
+        if (e && !err) err = e;
+        find_marker_cell(&mk[id], g);
+      } else {
+        double mindist = g->LX * g->LY;
+        int minidx = -1;
+        for (int m1 = 0; m1 < nMark; m1++) {
+          if (m1 != id && mk[m1].cellX != -1) {
+            if (fabs(mk[m1].X - newX[i]) <= (g->x[mk[m1].cellX + 1] - g->x[mk[m1].cellX]) &&
+                fabs(mk[m1].Y - newY[i]) <= (g->y[mk[m1].cellY + 1] - g->y[mk[m1].cellY])) {
+              const double x1 = (mk[m1].X - newX[i]);
+              const double y1 = (mk[m1].Y - newY[i]);
+              const double r1 = sqrt(x1 * x1 + y1 * y1);
+              if (r1 < mindist) {
+                mindist = r1;
+                minidx = m1;
+              }
+            }
+          }
+        }
+        if (minidx != -1) {
+          mk[id] = mk[minidx];
+          mk[id].X = newX[i];
+          mk[id].Y = newY[i];
+        }
+      }
+    }
+    added = nAdd;
+    free(newX);
+    free(newY);
+  } else if ((nAdd + ms->nMark - nOut) > ms->maxMark) {
+    err = ORC_ERR_CAPACITY;
+  }
+  free(cnt);
+  free(isout);
+  if (nAdded) *nAdded = added;
+  if (nRemoved) *nRemoved = removedActual;
+  return err;
+}

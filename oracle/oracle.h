@@ -18,6 +18,7 @@
 #define FVM_ORACLE_H
 
 #include <stddef.h>
+#include <stdint.h>
 #include "structures.h"
 
 #ifdef __cplusplus
@@ -34,7 +35,9 @@ enum {
   ORC_ERR_VISCOSITY_LAW = 6,
   ORC_ERR_NOMEM = 7,
   ORC_ERR_BC = 8,
-  ORC_ERR_IO = 9
+  ORC_ERR_IO = 9,
+  ORC_ERR_NEGATIVE_ADD = 10,
+  ORC_ERR_CAPACITY = 11
 };
 
 /* markers.c:85-106 findMarkerCells (+ findMarkerCell 41-83) */
@@ -117,6 +120,15 @@ int orc_limit_displacement_timestep(const GridData *g, const NodalArrays *nf, do
  * the path is the caller's: the reference names it output/Markers.<iMonte>.<iTime>) */
 int orc_save_markers_binary(const MarkerSet *ms, const char *path, double elapsedTime);
 int orc_restart_from_markers(MarkerSet *ms, const char *path, double *elapsedTime);
+
+/* markers.c:455-781 checkMarkerDensity on one rank, with the counter-based
+ * random stream orc_pc_u01 in place of drand48 / PetscRandom (see oracle.c).
+ * Reads the cells that findMarkerCells left; may raise ms->nMark up to
+ * ms->maxMark.  *nAdded = markers added (0 when the reference would neither add
+ * nor abort), *nRemoved = markers removed. */
+int orc_check_marker_density(MarkerSet *ms, const GridData *g, const Options *o, const Materials *mat,
+                             uint64_t seed, int *nAdded, int *nRemoved);
+double orc_pc_u01(uint64_t seed, uint64_t stream, uint64_t ctr);
 
 #ifdef __cplusplus
 }
