@@ -1,20 +1,21 @@
 // Learned initial guess for the Stokes solve: the U-Net's layers as hand-written
 // gfx950 kernels.  Activations are NHWC bfloat16 ([B][H][W][C], C fastest),
 // weights bfloat16 [tap][Cout][Cin] (tap = ky*K + kx) with fp32 master copies
-// for the optimizer; every product accumulates in fp32 on the matrix cores
-// (v_mfma_f32_16x16x16_bf16).  There is no reference counterpart: the
+// for the optimizer; every product accumulates in fp32 on gfx950's double-K
+// bf16 matrix instruction (v_mfma_f32_16x16x32_bf16: K = 32 per issue, twice
+// the CDNA3 16x16x16 shape).  There is no reference counterpart: the
 // reference's only initial guess is initialPressureGuess.c.
 //
-// MFMA 16x16x16 bf16 operand layout (one wavefront): lane l supplies
-//   A: row m = l % 16, k = 4*(l/16) .. 4*(l/16)+3   (four bf16)
-//   B: col n = l % 16, k = 4*(l/16) .. 4*(l/16)+3
+// MFMA 16x16x32 bf16 operand layout (one wavefront): lane l supplies
+//   A: row m = l % 16, k = 8*(l/16) .. 8*(l/16)+7   (eight bf16, one 16-byte load)
+//   B: col n = l % 16, k = 8*(l/16) .. 8*(l/16)+7
 // and receives D[m = 4*(l/16) + i][n = l % 16], i = 0..3 (four fp32).
 // fvm_nn_mfma_selftest checks this layout against a host product.
 #include <hip/hip_bf16.h>
 
 #include "fvm_internal.h"
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -29,35 +30,41 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
   return (uint16_t)(r >> 16);
 }
 
-__device__ __forceinline__ f4 mfma16(s4 a, s4 b, f4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// 4 consecutive bf16 from an 8-byte aligned address, or zeros
-__device__ __forceinline__ s4 ld4(const uint16_t *p, bool ok) {
-  s4 v = {0, 0, 0, 0};
-  if (ok) {
-    const uint2 u = *(const uint2 *)p;
-    v[0] = (short)(u.x & 0xffff);
-    v[1] = (short)(u.x >> 16);
-    v[2] = (short)(u.y & 0xffff);
-    v[3] = (short)(u.y >> 16);
+// 8 consecutive bf16 from a 16-byte aligned address, or zeros
+__device__ __forceinline__ bf8 ld8(const uint16_t *p, bool ok) {
+  const uint4 u = ok ? *(const uint4 *)p : make_uint4(0u, 0u, 0u, 0u);
+  return __builtin_bit_cast(bf8, u);
+}
+
+// eight bf16 gathered one by one (the weight gradient's pixel-strided operands)
+struct Bf8Builder {
+  uint16_t h[8];
+  __device__ __forceinline__ bf8 get() const {
+    uint4 u;
+    u.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+    u.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+    u.z = (uint32_t)h[4] | ((uint32_t)h[5] << 16);
+    u.w = (uint32_t)h[6] | ((uint32_t)h[7] << 16);
+    return __builtin_bit_cast(bf8, u);
   }
-  return v;
-}
+};
 
-// ---- MFMA layout self-test: D(16x16) = A(16x16) B(16x16) ---------------------
+// ---- MFMA layout self-test: D(16x16) = A(16x32) B(32x16) ---------------------
 __global__ void __launch_bounds__(64) k_mfma_selftest(const uint16_t *__restrict__ A, const uint16_t *__restrict__ B,
                                                       float *__restrict__ D) {
   const int l = threadIdx.x;
   const int m = l % 16, kg = l / 16;
-  s4 a, b;
-  for (int i = 0; i < 4; i++) {
-    a[i] = (short)A[m * 16 + 4 * kg + i];      // A row-major [m][k]
-    b[i] = (short)B[(4 * kg + i) * 16 + m];    // B row-major [k][n], n = l % 16
+  Bf8Builder a, b;
+  for (int i = 0; i < 8; i++) {
+    a.h[i] = A[m * 32 + 8 * kg + i];      // A row-major [m][k], 16 x 32
+    b.h[i] = B[(8 * kg + i) * 16 + m];    // B row-major [k][n], 32 x 16, n = l % 16
   }
   f4 c = {0.f, 0.f, 0.f, 0.f};
-  c = mfma16(a, b, c);
+  c = mfma32(a.get(), b.get(), c);
   for (int i = 0; i < 4; i++) D[(4 * kg + i) * 16 + m] = c[i];
 }
 
@@ -109,16 +116,19 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
       xs[t] = a.X + ((((int64_t)pb[t] * a.H + (xv[t] ? hh : 0)) * a.Wd + (xv[t] ? ww : 0)) * a.Cin);
     }
     const uint16_t *wt = a.W + (int64_t)tap * a.Cout * a.Cin;
-    for (int ci0 = 0; ci0 < a.Cin; ci0 += 16) {
-      const int ck = ci0 + 4 * kg;
-      s4 fa[2], fb[2];
-      for (int t = 0; t < 2; t++) fa[t] = ld4(xs[t] + ck, xv[t]);
+    // Cin is a multiple of 16: a lane's eight channels lie wholly inside or
+    // wholly past it (a 16-channel layer leaves half of K at zero)
+    for (int ci0 = 0; ci0 < a.Cin; ci0 += 32) {
+      const int ck = ci0 + 8 * kg;
+      const bool kin = ck < a.Cin;
+      bf8 fa[2], fb[2];
+      for (int t = 0; t < 2; t++) fa[t] = ld8(xs[t] + ck, xv[t] && kin);
       for (int u = 0; u < 2; u++) {
         const int co = c0 + u * 16 + ml;
-        fb[u] = ld4(wt + (int64_t)co * a.Cin + ck, co < a.Cout);
+        fb[u] = ld8(wt + (int64_t)co * a.Cin + ck, co < a.Cout && kin);
       }
       for (int t = 0; t < 2; t++)
-        for (int u = 0; u < 2; u++) acc[t][u] = mfma16(fa[t], fb[u], acc[t][u]);
+        for (int u = 0; u < 2; u++) acc[t][u] = mfma32(fa[t], fb[u], acc[t][u]);
     }
   }
   // epilogue: D[m = 4*kg + i][n = ml] of tile (t, u)
@@ -166,10 +176,10 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
     f4 acc[2][2];
     for (int i = 0; i < 2; i++)
       for (int j = 0; j < 2; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < KCH; k0 += 16) {
-      s4 fa[2], fb[2];
-      for (int i = 0; i < 4; i++) {
-        const int64_t p = chunk * KCH + k0 + 4 * kg + i;
+    for (int k0 = 0; k0 < KCH; k0 += 32) {
+      Bf8Builder fa[2], fb[2];
+      for (int i = 0; i < 8; i++) {
+        const int64_t p = chunk * KCH + k0 + 8 * kg + i;
         bool ok = p < npix;
         const int64_t q = ok ? p : 0;
         const int w = (int)(q % a.Wd), h = (int)((q / a.Wd) % a.H);
@@ -179,15 +189,15 @@ __global__ void __launch_bounds__(256) k_wgrad(WgradArgs a) {
         const int64_t qx = ((bq * a.H + (xok ? hh : 0)) * a.Wd + (xok ? ww : 0));
         for (int t = 0; t < 2; t++) {
           const int co = co0 + t * 16 + ml;
-          fa[t][i] = (ok && co < a.Cout) ? (short)a.dY[q * a.Cout + co] : (short)0;
+          fa[t].h[i] = (ok && co < a.Cout) ? a.dY[q * a.Cout + co] : (uint16_t)0;
         }
         for (int u = 0; u < 2; u++) {
           const int ci = ci0 + u * 16 + ml;
-          fb[u][i] = (xok && ci < a.Cin) ? (short)a.X[qx * a.Cin + ci] : (short)0;
+          fb[u].h[i] = (xok && ci < a.Cin) ? a.X[qx * a.Cin + ci] : (uint16_t)0;
         }
       }
       for (int t = 0; t < 2; t++)
-        for (int u = 0; u < 2; u++) acc[t][u] = mfma16(fa[t], fb[u], acc[t][u]);
+        for (int u = 0; u < 2; u++) acc[t][u] = mfma32(fa[t].get(), fb[u].get(), acc[t][u]);
     }
     float *out = a.part + ((chunk * (a.K * a.K) + tap) * a.Cout) * (int64_t)a.Cin;
     for (int t = 0; t < 2; t++)

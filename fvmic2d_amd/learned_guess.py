@@ -116,13 +116,14 @@ def enforce_box(x, N):
     return np.concatenate([vx.ravel(), vy.ravel(), p.ravel()])
 
 
-def generate(N, seeds, shift=0.03, workers=0):
+def generate(N, seeds, shift=0.03, workers=0, start="fork"):
     """The plain method's direct solves for every seed (host only).  With
-    workers > 0 they run in forked worker processes: call it before anything
-    in this process has touched the GPU."""
+    workers > 0 they run in worker processes: forked (start="fork", only before
+    anything in this process has touched the GPU) or spawned (start="spawn",
+    fresh interpreters that inherit no GPU state)."""
     jobs = [(N, s, shift) for s in seeds]
     if workers > 0:
-        with get_context("fork").Pool(workers) as pool:
+        with get_context(start).Pool(workers) as pool:
             return pool.map(_worker, jobs, chunksize=1)
     return [_worker(j) for j in jobs]
 
@@ -166,13 +167,19 @@ class Dataset:
 
 
 class LearnedGuess:
-    """A trained U-Net plus the scales that map its channels to the Stokes unknowns."""
+    """A trained U-Net plus the scales that map its channels to the Stokes unknowns.
 
-    def __init__(self, ctx, levels=4, base=32, seed=0):
+    With residual=True (the default) the network learns the change of the
+    dynamic pressure since the previous solution, q - q_prev, and the guess is
+    q_prev plus its output: a network that has learned little then falls back
+    towards the previous solution's pressure, not towards zero."""
+
+    def __init__(self, ctx, levels=4, base=32, seed=0, residual=True):
         from .unet import UNet
         self.ctx = ctx
         self.net = UNet(ctx, levels=levels, base=base, cin=5, cout=1, seed=seed)
         self.vs = self.qs = self.dqs = 1.0
+        self.residual = residual
 
     def _scaled(self, X, T=None):
         Xs = X.copy()
@@ -184,6 +191,8 @@ class LearnedGuess:
 
     def train(self, X, T, epochs=30, batch=8, lr=2e-3, seed=0, log=None):
         from .unet import Buf, to_bf16_bits
+        if self.residual:
+            T = T - X[..., 4:5]
         # input scales from the previous solutions, output scale from the targets
         self.vs = float(np.sqrt(np.mean(X[..., 2:4] ** 2))) or 1.0
         self.qs = float(np.sqrt(np.mean(X[..., 4] ** 2))) or 1.0
@@ -234,7 +243,10 @@ class LearnedGuess:
             xb[0, ..., :5] = Xs[k]
             dX.set(to_bf16_bits(xb))
             out[k] = self.net.forward(dX).get()[0, ..., :1]
-        return out * self.dqs
+        out = out * self.dqs
+        if self.residual:
+            out = out + X[..., 4:5]
+        return out
 
 
 def pressure_from_output(o, sample, N):

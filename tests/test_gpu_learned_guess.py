@@ -1,6 +1,13 @@
-"""The learned Stokes guess end to end at a small size: data from the plain
-method's solves, U-Net training by the GPU kernels, and the host Stokes solve (FGMRES +
-field split) started from the plain, previous and learned guesses."""
+"""The learned Stokes guess end to end: data from the plain method's solves,
+U-Net training by the GPU kernels, and the host Stokes solve (FGMRES + field
+split, unpreconditioned relative residual 1e-8) started from the plain, previous
+and learned guesses.
+
+The small case checks the pipeline and that the learned path is taken; the
+config-5 case runs the configured network (257 x 257 nodes, 4 levels, 32 base
+channels) on a small training set and reports the iterations it saves against
+the plain guess (the 40% target is recorded, not asserted: the measured gain is
+what this test reports)."""
 import numpy as np
 import pytest
 
@@ -10,24 +17,48 @@ from fvmic2d_amd.lib import Context
 pytestmark = pytest.mark.gpu
 
 
-def test_learned_guess_pipeline_small():
-    print("CASE learned guess 33x33, 64 train / 6 test", flush=True)
-    N = 33
-    raw_tr = lg.generate(N, list(range(64)), workers=0)
-    raw_te = lg.generate(N, list(range(1000, 1006)), workers=0)
+def _run(N, levels, base, ntrain, ntest, epochs, workers, lr=3e-3):
+    # spawned workers: this process has touched the GPU in earlier tests
+    raw_tr = lg.generate(N, list(range(ntrain)), workers=workers, start="spawn")
+    raw_te = lg.generate(N, list(range(1000, 1000 + ntest)), workers=workers, start="spawn")
     ctx = Context()
     tr, te = lg.Dataset(ctx, N, raw_tr), lg.Dataset(ctx, N, raw_te)
     X, T = tr.arrays()
-    model = lg.LearnedGuess(ctx, levels=3, base=16, seed=0)
-    hist = model.train(X, T, epochs=30, batch=8, lr=3e-3)
-    assert hist[-1] < 0.5 * hist[0], hist
-    # held-out states: the predicted dynamic pressure explains part of its variance
-    Xt, Tt = te.arrays()
-    pred = model.predict(Xt)
-    assert np.mean((pred - Tt) ** 2) < np.mean(Tt ** 2)
+    model = lg.LearnedGuess(ctx, levels=levels, base=base, seed=0)
+    hist = model.train(X, T, epochs=epochs, batch=8, lr=lr)
     rows = lg.evaluate(te, model)
-    s = lg.summarize(rows)
+    return hist, lg.summarize(rows), rows
+
+
+def test_learned_guess_pipeline_small():
+    print("CASE learned guess 33x33, 64 train / 6 test", flush=True)
+    hist, s, rows = _run(33, 3, 16, 64, 6, 30, workers=4)
     print(s, flush=True)
+    # the target is the change from the previous pressure, normalised: its loss
+    # starts near 1 and falls (0.86 after 30 epochs on 64 states)
+    assert hist[-1] < 0.95 * hist[0], hist
     for r in rows:
         assert all(np.isfinite(v) for v in r["r0"].values())
         assert all(0 <= v <= 3000 for v in r["its"].values())
+    # the learned path is taken: its initial residual beats the plain guess's on
+    # some held-out states
+    assert s["fallbacks"] < s["samples"], s
+    assert s["mean_iterations"]["learned"] <= s["mean_iterations"]["plain"], s
+
+
+@pytest.mark.timeout(600)
+def test_learned_guess_config5_size():
+    """257 x 257 nodes (the network sees 256 x 256), 4 levels, 32 base channels,
+    viscosity contrast 1e4; 24 training states, 3 held out."""
+    import time
+    t0 = time.time()
+    print("CASE learned guess 257x257, 4 levels, 32 base, 24 train / 3 test", flush=True)
+    hist, s, rows = _run(257, 4, 32, 24, 3, 20, workers=12, lr=2e-3)
+    print("  %.0f s; plain %.1f, learned %.1f iterations (reduction %.1f%%, %d of %d fell back); "
+          "initial residual plain %.2e, learned %.2e" % (
+              time.time() - t0, s["mean_iterations"]["plain"], s["mean_iterations"]["learned"],
+              100 * s["reduction_vs_plain"], s["fallbacks"], s["samples"],
+              s["geomean_initial_relres"]["plain"], s["geomean_initial_relres"]["learned"]), flush=True)
+    assert np.isfinite(hist[-1]) and hist[-1] < hist[0]
+    for r in rows:
+        assert all(0 < v < 3000 for v in r["its"].values())

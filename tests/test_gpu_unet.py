@@ -12,13 +12,13 @@ pytestmark = pytest.mark.gpu
 
 
 def test_mfma_layout():
-    print("CASE mfma 16x16x16 bf16 layout", flush=True)
+    print("CASE mfma 16x16x32 bf16 layout", flush=True)
     ctx = Context()
     lib = bind()
     rng = np.random.default_rng(0)
-    A = R.bf16(rng.standard_normal((16, 16)).astype(np.float32))
-    B = R.bf16(rng.standard_normal((16, 16)).astype(np.float32))
-    dA, dB, dD = Buf(ctx, (256,), np.uint16), Buf(ctx, (256,), np.uint16), Buf(ctx, (256,), np.float32)
+    A = R.bf16(rng.standard_normal((16, 32)).astype(np.float32))
+    B = R.bf16(rng.standard_normal((32, 16)).astype(np.float32))
+    dA, dB, dD = Buf(ctx, (512,), np.uint16), Buf(ctx, (512,), np.uint16), Buf(ctx, (256,), np.float32)
     dA.set(to_bf16_bits(A).ravel())
     dB.set(to_bf16_bits(B).ravel())
     assert lib.fvm_nn_mfma_selftest(ctx.h, dA.ptr, dB.ptr, dD.ptr) == 0
