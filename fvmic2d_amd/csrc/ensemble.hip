@@ -163,7 +163,10 @@ __global__ void __launch_bounds__(256) kb_order_gather(const InstDev *__restrict
                                                        int32_t *__restrict__ ms, int ncol,
                                                        const double *const *__restrict__ colsrc,
                                                        double *__restrict__ pcols, int64_t nmark) {
-  const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  // the wavefront index is uniform: say so, so that the instance record and
+  // its column pointers are read with scalar loads
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;
   const int64_t tile = w >> 6;
   if (tile >= tile_off[n]) return;  // whole wavefront exits together
   const int ii = tile_inst[tile];
@@ -175,10 +178,12 @@ __global__ void __launch_bounds__(256) kb_order_gather(const InstDev *__restrict
   const int cx = (int)(tl % ntx) * 8 + (l >> 3), cy = (int)(tl / ntx) * 8 + (l & 7);
   if (cx >= ncx || cy >= ncy) return;
   const int64_t c = cell_off[ii] + (int64_t)cy * ncx + cx;
+  const double *__restrict__ X = I.X, *__restrict__ Y = I.Y;
+  const int32_t *__restrict__ Mat = I.Mat;
   bucket_order_cell(c, start, perm, [&](int p, int v) {
-    xs[p] = I.X[v];
-    ys[p] = I.Y[v];
-    ms[p] = I.Mat[v];
+    xs[p] = X[v];
+    ys[p] = Y[v];
+    ms[p] = Mat[v];
     for (int k = 0; k < ncol; k++) pcols[(int64_t)k * nmark + p] = colsrc[(int64_t)k * n + ii][v];
   });
 }
@@ -293,10 +298,10 @@ __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ ins
   gl.yc = sgrid + 2 * g.NX + g.NY + 1;
   const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
   if (m >= I.nMark) return;
-  advect_marker(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.cellX, I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, I.T, I.Tdot,
+  advect_marker<false>(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.cellX, I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, I.T, I.Tdot,
                 deltaT + I.node_off, dTsub + I.mark_off, I.dt);
   // the rotation fused behind the RK4 stages: the batched launch keeps one kernel per phase
-  rotate_eii_marker(m, g.NX, g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.dt,
+  rotate_eii_marker(m, g.ncx(), g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.dt,
                     I.jaumann);
 }
 
@@ -532,7 +537,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
   const fvm_instance &I0 = inst[0];
   if (I0.nproj < 1 || !I0.proj || I0.nproj > 64) return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: projection list");
   for (int i = 0; i < n; i++) {
-    FVM_TRY(fvm_grid_check(ctx, &inst[i].grid));
+    FVM_TRY(fvm_grid_check_nonperiodic(ctx, &inst[i].grid, "ensemble instance"));
     if (inst[i].jaumann != 0 && inst[i].jaumann != 1)
       return fvm_set_err(ctx, FVM_ERR_ARG, "ensemble: instance %d: jaumann must be 0 or 1", i);
     if (inst[i].nproj != I0.nproj || !inst[i].proj)

@@ -80,12 +80,23 @@ int fvm_ws(fvm_ctx *ctx, fvm_buf &b, size_t bytes);  // ensure workspace capacit
 
 #define FVM_LAUNCH_CHECK(ctx) FVM_HIP(ctx, hipGetLastError())
 
+// Marker-path entry points take periodic grids (x periodic, NX >= 4: the two
+// ghost columns on each side then wrap to distinct nodes); the nodal solves and
+// population control call fvm_grid_check_nonperiodic instead.
 static inline int fvm_grid_check(fvm_ctx *ctx, const fvm_grid *g) {
   if (!g || !g->x || !g->y || !g->xc || !g->yc || g->NX < 2 || g->NY < 2)
     return fvm_set_err(ctx, FVM_ERR_ARG, "invalid grid descriptor");
-  if (g->xperiodic) return fvm_set_err(ctx, FVM_ERR_PERIODIC, "periodic grids are not supported");
+  if (g->xperiodic && g->NX < 4) return fvm_set_err(ctx, FVM_ERR_PERIODIC, "a periodic grid needs NX >= 4");
   return FVM_OK;
 }
+static inline int fvm_grid_check_nonperiodic(fvm_ctx *ctx, const fvm_grid *g, const char *what) {
+  FVM_TRY(fvm_grid_check(ctx, g));
+  if (g->xperiodic) return fvm_set_err(ctx, FVM_ERR_PERIODIC, "%s: periodic grids are not supported", what);
+  return FVM_OK;
+}
+// Basic cells along x: NX - 1, or NX on a periodic grid (x[NX] = LX closes the
+// last cell; makePeriodic, gridSpacing.c:159-168).
+static inline int fvm_ncx(const fvm_grid *g) { return g->NX - 1 + (g->xperiodic ? 1 : 0); }
 
 static inline unsigned fvm_blocks(int64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
@@ -138,13 +149,16 @@ __device__ __forceinline__ int wave_count_slot(int k, int32_t *__restrict__ coun
 // ---- device-side grid view -------------------------------------------------
 struct DGrid {
   int NX, NY;
+  int per;  // 1: x periodic (node index ix stands for ix mod NX)
   double LX, LY;
   const double *x, *y, *xc, *yc;
+  __host__ __device__ int ncx() const { return NX - 1 + per; }
 };
 static inline DGrid dgrid(const fvm_grid *g) {
   DGrid d;
   d.NX = g->NX;
   d.NY = g->NY;
+  d.per = g->xperiodic ? 1 : 0;
   d.LX = g->LX;
   d.LY = g->LY;
   d.x = g->x;
@@ -157,25 +171,34 @@ static inline DGrid dgrid(const fvm_grid *g) {
 // A marker takes part only if its cell is inside the grid: cellX == -1 marks an
 // inactive marker (fdcode.h:120); any other out-of-range index (a caller that
 // skipped findMarkerCells) is skipped too, so no kernel indexes outside its arrays.
-__device__ __forceinline__ bool cell_ok(int cx, int cy, int NX, int NY) {
-  return (unsigned)cx <= (unsigned)(NX - 2) && (unsigned)cy <= (unsigned)(NY - 2);
+// ncx = basic cells along x (NX - 1, or NX on a periodic grid)
+__device__ __forceinline__ bool cell_ok(int cx, int cy, int ncx, int NY) {
+  return (unsigned)cx < (unsigned)ncx && (unsigned)cy <= (unsigned)(NY - 2);
+}
+
+// x node index of a periodic local array: ghost columns -2, -1 and NX, NX + 1
+// stand for NX - 2, NX - 1 and 0, 1 (DM_BOUNDARY_PERIODIC, nodalFields.c:11-20)
+__device__ __forceinline__ int wrap_x(int ix, int NX, int per) {
+  return per ? (ix < 0 ? ix + NX : (ix >= NX ? ix - NX : ix)) : ix;
 }
 
 // Read of a ghosted local array: indices outside the grid read 0 (a fresh,
-// zeroed DMDA local vector; DMGlobalToLocal fills only owned nodes).
-__device__ __forceinline__ double gat(const double *__restrict__ f, int NX, int NY, int jy, int ix) {
+// zeroed DMDA local vector; DMGlobalToLocal fills only owned nodes); on a
+// periodic grid the x ghosts read the wrapped column.
+__device__ __forceinline__ double gat(const double *__restrict__ f, int NX, int NY, int per, int jy, int ix) {
+  ix = wrap_x(ix, NX, per);
   return (ix >= 0 && ix < NX && jy >= 0 && jy < NY) ? f[(size_t)jy * NX + ix] : 0.0;
 }
 
 // The reference's bilinear gather written as four accumulations
 // (updateMarkerStress.c:84-88 form): v=0; v+=(1-mdx)(1-mdy)f00; v+=(1-mdx)mdy f10; ...
-__device__ __forceinline__ double bil_acc(const double *__restrict__ f, int NX, int NY, int cy, int cx,
+__device__ __forceinline__ double bil_acc(const double *__restrict__ f, int NX, int NY, int per, int cy, int cx,
                                           double mdx, double mdy) {
   double v = 0.0;
-  v += (1.0 - mdx) * (1.0 - mdy) * gat(f, NX, NY, cy, cx);
-  v += (1.0 - mdx) * mdy * gat(f, NX, NY, cy + 1, cx);
-  v += mdx * (1.0 - mdy) * gat(f, NX, NY, cy, cx + 1);
-  v += mdx * mdy * gat(f, NX, NY, cy + 1, cx + 1);
+  v += (1.0 - mdx) * (1.0 - mdy) * gat(f, NX, NY, per, cy, cx);
+  v += (1.0 - mdx) * mdy * gat(f, NX, NY, per, cy + 1, cx);
+  v += mdx * (1.0 - mdy) * gat(f, NX, NY, per, cy, cx + 1);
+  v += mdx * mdy * gat(f, NX, NY, per, cy + 1, cx + 1);
   return v;
 }
 

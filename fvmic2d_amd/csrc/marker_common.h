@@ -7,6 +7,12 @@
 #include "fvm_internal.h"
 
 // ---- A6: advectMarkersRK (markers.c:1865-2250) --------------------------------
+// PER: x periodic (g.per == 1).  The walk may then leave [0, NX-1] by one cell
+// (XCMIN = -1, XCMAX = NX-1, markers.c:1908-1915), the staggered x indices are
+// not clamped (:1965-1971), node columns wrap, the left-wall ghost rule is off
+// (:2014-2023) and the new X is wrapped into [0, LX] (:2225-2231).  The grid
+// coordinates are then read at indices -1 .. NX+1 (set by makePeriodic).
+template <bool PER>
 __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     double *__restrict__ MX,
     double *__restrict__ MY,
@@ -26,9 +32,11 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     const double *__restrict__ dTsub,
     double dt_thermal) {
   const int c0x = cellXa[m], c0y = cellYa[m];
-  if (!cell_ok(c0x, c0y, g.NX, g.NY)) return;
+  if (!cell_ok(c0x, c0y, g.ncx(), g.NY)) return;
   const int NX = g.NX, NY = g.NY;
-  const int XCMAX = NX - 2, XCMIN = 0;
+  const int XCMAX = PER ? NX - 1 : NX - 2, XCMIN = PER ? -1 : 0;
+  // column offset of node ix + 1 from node ix (wraps past the last column)
+  auto right = [&](int ix) -> int { return (PER && ix + 1 >= NX) ? 1 - NX : 1; };
   const double X0 = MX[m], Y0 = MY[m];
   if (MT) {
     // subgridTemperatureChanges.c:89-103 (pass 2) on the same stored cell,
@@ -39,8 +47,9 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     const double mdy = (Y0 - g.y[c0y]) / dy;
     // the stored cell is inside the grid: all four corners are owned nodes
     const double *d0 = dTn + (int64_t)c0y * NX + c0x, *d1 = d0 + NX;
-    const double dtm = (d0[0] * (1.0 - mdy) * (1.0 - mdx) + d1[0] * mdy * (1.0 - mdx) + d0[1] * (1.0 - mdy) * mdx +
-                        d1[1] * mdy * mdx) +
+    const int r0 = right(c0x);
+    const double dtm = (d0[0] * (1.0 - mdy) * (1.0 - mdx) + d1[0] * mdy * (1.0 - mdx) + d0[r0] * (1.0 - mdy) * mdx +
+                        d1[r0] * mdy * mdx) +
                        dTsub[m];
     MT[m] = MT[m] + dtm;
     MTdot[m] = dtm / dt_thermal;
@@ -67,10 +76,12 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     int vyX = cellX;
     if (vyX > -1 && xcur < g.xc[vyX + 1]) vyX--;
     int vyY = cellY;
-    if (vxX < 0) vxX = 0;
-    else if (vxX > NX - 2) vxX = NX - 2;
-    if (vyX < -1) vyX = -1;
-    else if (vyX > NX - 2) vyX = NX - 2;
+    if (!PER) {
+      if (vxX < 0) vxX = 0;
+      else if (vxX > NX - 2) vxX = NX - 2;
+      if (vyX < -1) vyX = -1;
+      else if (vyX > NX - 2) vyX = NX - 2;
+    }
     if (vxY < -1) vxY = -1;
     else if (vxY > NY - 2) vxY = NY - 2;
     if (vyY < 0) vyY = 0;
@@ -80,16 +91,20 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     double deltaym = ycur - g.yc[vxY + 1];
     double dx = g.x[vxX + 1] - g.x[vxX];
     double dy = g.yc[vxY + 2] - g.yc[vxY + 1];
-    // vxX == cellX (cellX stays in [0, NX-2]), so this quotient is also the wxy mdx below
+    // vxX == cellX (cellX stays in [0, NX-2], or [-1, NX-1] when PER), so this
+    // quotient is also the wxy mdx below
     const double qx = deltaxm / dx;
     // vx nodes: rows vxY (-1 is the ghost row, which reads 0) and vxY + 1,
-    // columns vxX and vxX + 1 (both owned: vxX is in [0, NX-2]).  Loads are
-    // unconditional from a valid address; the ghost row is selected away.
+    // columns vxX and vxX + 1 (both owned: vxX is in [0, NX-2]; when PER the
+    // columns wrap).  Loads are unconditional from a valid address; the ghost
+    // row is selected away.
     const bool vrow = vxY >= 0;
-    const double *vr1 = vx + (int64_t)(vxY + 1) * NX + vxX;
+    const int vxc = PER && vxX < 0 ? vxX + NX : vxX;
+    const int vxr = right(vxc);
+    const double *vr1 = vx + (int64_t)(vxY + 1) * NX + vxc;
     const double *vr0 = vrow ? vr1 - NX : vr1;
-    const double vxll = vr1[0], vxlr = vr1[1];
-    const double vx0 = vr0[0], vx1 = vr0[1];
+    const double vxll = vr1[0], vxlr = vr1[vxr];
+    const double vx0 = vr0[0], vx1 = vr0[vxr];
     double vxul, vxur;
     if (vxY == -1 && bv.mechBCTop.type[0] == 0) {
       vxul = 2.0 * bv.mechBCTop.value[0] - vxll;
@@ -107,17 +122,20 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     a += vxll * (1.0 - qx) * (qyv);
     a += vxlr * (deltaxm * deltaym) / dx / dy;
 
-    // vy nodes: columns vyX (-1 is the ghost column, which reads 0) and
-    // vyX + 1, rows vyY and vyY + 1 (both owned: vyY is in [0, NY-2])
-    const bool vcol = vyX >= 0;
-    const double *q0 = vy + (int64_t)vyY * NX + (vyX + 1), *q1 = q0 + NX;
+    // vy nodes: columns vyX (-1 is the ghost column, which reads 0; when PER
+    // it is column NX-1) and vyX + 1, rows vyY and vyY + 1 (both owned: vyY
+    // is in [0, NY-2])
+    const bool vcol = PER || vyX >= 0;
+    const int vyc = PER && vyX + 1 >= NX ? vyX + 1 - NX : vyX + 1;  // column of vyX + 1
+    const int vyl = (PER && vyc == 0) ? NX - 1 : -1;                // offset of column vyX
+    const double *q0 = vy + (int64_t)vyY * NX + vyc, *q1 = q0 + NX;
     const double vyur = q0[0], vylr = q1[0];
-    const double vy0 = (vcol ? q0 - 1 : q0)[0], vy1 = (vcol ? q1 - 1 : q1)[0];
+    const double vy0 = (vcol ? q0 + vyl : q0)[0], vy1 = (vcol ? q1 + vyl : q1)[0];
     double vyul, vyll;
-    if (vyX == -1 && bv.mechBCLeft.type[1] == 0) {
+    if (!PER && vyX == -1 && bv.mechBCLeft.type[1] == 0) {
       vyul = 2.0 * bv.mechBCLeft.value[1] - vyur;
       vyll = 2.0 * bv.mechBCLeft.value[1] - vylr;
-    } else if (vyX == -1 && bv.mechBCLeft.type[1] == 1) {
+    } else if (!PER && vyX == -1 && bv.mechBCLeft.type[1] == 1) {
       vyul = vyur;
       vyll = vylr;
     } else {
@@ -140,12 +158,15 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
     // operands as qx and qy above, hence the same values
     const double mdx = qx;
     const double mdy = qy;
-    // cellX in [0, NX-2], cellY in [0, NY-2]: four owned nodes
-    const double *w0 = wxy + (int64_t)cellY * NX + cellX, *w1 = w0 + NX;
+    // cellX in [0, NX-2] ([-1, NX-1] wrapped when PER), cellY in [0, NY-2]:
+    // four owned nodes
+    const int wc = PER && cellX < 0 ? cellX + NX : cellX;
+    const int wr = right(wc);
+    const double *w0 = wxy + (int64_t)cellY * NX + wc, *w1 = w0 + NX;
     double w = (1.0 - mdx) * (1.0 - mdy) * w0[0];
-    w += (mdx) * (1.0 - mdy) * w0[1];
+    w += (mdx) * (1.0 - mdy) * w0[wr];
     w += (1.0 - mdx) * (mdy) * w1[0];
-    w += (mdx) * (mdy) * w1[1];
+    w += (mdx) * (mdy) * w1[wr];
     if (irk == 0) {
       svx = a;
       svy = b;
@@ -175,7 +196,12 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
   MVX[m] = VX;
   MVY[m] = VY;
   Mwxy[m] = wm;
-  MX[m] = X0 + dt * VX;
+  double Xn = X0 + dt * VX;
+  if (PER) {
+    if (Xn > g.LX) Xn -= g.LX;
+    else if (Xn < 0) Xn += g.LX;
+  }
+  MX[m] = Xn;
   MY[m] = Y0 + dt * VY;
 }
 
@@ -183,7 +209,8 @@ __device__ __forceinline__ void advect_marker(int64_t m, const DGrid &g,
 // invariant update (markers.c:2141-2164, 2223), after advect_marker has stored the spin
 // wxy = (k1 + 2 k2 + 2 k3 + k4) / 6.  A pass of its own: the RK4 pass then holds
 // fewer live registers and hides more of its gather latency, and this one streams.
-__device__ __forceinline__ void rotate_eii_marker(int64_t m, int NX, int NY,
+// ncx: basic cells along x (NX - 1, or NX on a periodic grid)
+__device__ __forceinline__ void rotate_eii_marker(int64_t m, int ncx, int NY,
     const int32_t *__restrict__ cellXa,
     const int32_t *__restrict__ cellYa,
     const double *__restrict__ Mwxy,
@@ -196,7 +223,7 @@ __device__ __forceinline__ void rotate_eii_marker(int64_t m, int NX, int NY,
     double *__restrict__ Eii,
     double dt,
     int jaumann) {
-  if (!cell_ok(cellXa[m], cellYa[m], NX, NY)) return;
+  if (!cell_ok(cellXa[m], cellYa[m], ncx, NY)) return;
   const double wm = Mwxy[m];
   const double msxx0 = sxx[m], msyy0 = syy[m], msxy0 = sxy[m];
   if (jaumann) {
@@ -232,16 +259,16 @@ __device__ __forceinline__ void subgrid_t2_marker(int64_t m, const DGrid &g,
     const double *__restrict__ dTsub,
     double dt) {
   const int cellX = cX[m], cellY = cY[m];
-  if (!cell_ok(cellX, cellY, g.NX, g.NY)) return;
-  const int NX = g.NX, NY = g.NY;
+  if (!cell_ok(cellX, cellY, g.ncx(), g.NY)) return;
+  const int NX = g.NX, NY = g.NY, P = g.per;
   const double dx = g.x[cellX + 1] - g.x[cellX];
   const double dy = g.y[cellY + 1] - g.y[cellY];
   const double mdx = (MX[m] - g.x[cellX]) / dx;
   const double mdy = (MY[m] - g.y[cellY]) / dy;
-  const double dtm = (gat(dT, NX, NY, cellY, cellX) * (1.0 - mdy) * (1.0 - mdx) +
-                      gat(dT, NX, NY, cellY + 1, cellX) * mdy * (1.0 - mdx) +
-                      gat(dT, NX, NY, cellY, cellX + 1) * (1.0 - mdy) * mdx +
-                      gat(dT, NX, NY, cellY + 1, cellX + 1) * mdy * mdx) +
+  const double dtm = (gat(dT, NX, NY, P, cellY, cellX) * (1.0 - mdy) * (1.0 - mdx) +
+                      gat(dT, NX, NY, P, cellY + 1, cellX) * mdy * (1.0 - mdx) +
+                      gat(dT, NX, NY, P, cellY, cellX + 1) * (1.0 - mdy) * mdx +
+                      gat(dT, NX, NY, P, cellY + 1, cellX + 1) * mdy * mdx) +
                      dTsub[m];
   MT[m] = MT[m] + dtm;
   MTdot[m] = dtm / dt;

@@ -284,7 +284,7 @@ __global__ void __launch_bounds__(256) k_dt_final(int nb, const DtPart *__restri
 extern "C" int fvm_nodal_stress_strain(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf,
                                        const BoundaryValues *bv, double dt, double *nodalHeating) {
   if (!ctx || !nf || !bv || !nodalHeating) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "nodalStressStrain"));
   const double *need[] = {nf->vx, nf->vy, nf->etaN, nf->muN, nf->etaS, nf->muS};
   double *outs[] = {nf->edotxx, nf->edotyy, nf->edotxy, nf->wxy,  nf->soxx, nf->soyy, nf->soxy,
                     nf->dsxx,   nf->dsyy,   nf->dsxy,   nf->eii, nf->sii};
@@ -312,7 +312,7 @@ extern "C" int fvm_nodal_stress_strain(fvm_ctx *ctx, const fvm_grid *g, const No
 extern "C" int fvm_retrieve_solutions(fvm_ctx *ctx, const fvm_grid *g, const NodalArrays *nf, double *S,
                                       double Kcont, int32_t *foundnan) {
   if (!ctx || !nf || !S || !nf->vx || !nf->vy || !nf->p) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "retrieveSolutions"));
   const int64_t nn = (int64_t)g->NX * g->NY;
   FVM_TRY(fvm_ws(ctx, ctx->red, 4096));
   double *pm = (double *)((char *)ctx->red.ptr + 512);
@@ -333,7 +333,7 @@ extern "C" int fvm_limit_displacement_timestep(fvm_ctx *ctx, const fvm_grid *g, 
                                                double *displacementdt, const Options *opt) {
   if (!ctx || !nf || !displacementdt || !opt || !nf->vx || !nf->vy || !nf->rho || !nf->rhodot)
     return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "limitDisplacementTimestep"));
   const int64_t nn = (int64_t)g->NX * g->NY;
   const int nb = (int)std::min<int64_t>(fvm_blocks(nn, 256), DT_BLOCKS);
   FVM_TRY(fvm_ws(ctx, ctx->red, 4096 + (size_t)DT_BLOCKS * sizeof(DtPart)));

@@ -2,7 +2,7 @@
 // stress diffusion, marker stress / strain-rate / pressure updates, plastic
 // yielding, viscosity limits, the strain-rate residual and the glue loops.
 // Each kernel keeps the reference's expression order (see the file:line
-// citations); the grid must be non-periodic (checked by the entry points).
+// citations).  Periodic x grids follow the reference's xperiodic branches.
 #include <float.h>
 
 #include "marker_common.h"
@@ -18,7 +18,9 @@ int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);  // project.hip
 // of staged grid beyond that, at most 8); the per-marker arithmetic is unchanged.
 // k_rotate_eii then rotates the stresses and updates Eii in a streaming pass.
 #define ADV_TILE_BYTES 4096
-template <bool LDS>
+// A periodic grid (PER) reads the grid coordinates at ghost indices -1 .. NX+1,
+// so it takes the path without staging.
+template <bool LDS, bool PER>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_advect_rk4(DGrid g, int32_t n, double *__restrict__ MX, double *__restrict__ MY,
                                                     double *__restrict__ MVX, double *__restrict__ MVY,
                                                     double *__restrict__ Mwxy,
@@ -45,18 +47,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     for (int t = 0; t < tiles; t++) {
       const int64_t m = m0 + 256 * t;
       if (m < n)
-        advect_marker(m, gl, MX, MY, MVX, MVY, Mwxy, cellXa, cellYa, vx, vy, wxy, bv, dt, MT, MTdot, dTn, dTsub,
+        advect_marker<PER>(m, gl, MX, MY, MVX, MVY, Mwxy, cellXa, cellYa, vx, vy, wxy, bv, dt, MT, MTdot, dTn, dTsub,
                       dt_thermal);
     }
   } else {
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= n) return;
-    advect_marker(m, g, MX, MY, MVX, MVY, Mwxy, cellXa, cellYa, vx, vy, wxy, bv, dt, MT, MTdot, dTn, dTsub,
+    advect_marker<PER>(m, g, MX, MY, MVX, MVY, Mwxy, cellXa, cellYa, vx, vy, wxy, bv, dt, MT, MTdot, dTn, dTsub,
                   dt_thermal);
   }
 }
 
-__global__ void __launch_bounds__(256) k_rotate_eii(int32_t n, int NX, int NY, const int32_t *__restrict__ cellXa,
+__global__ void __launch_bounds__(256) k_rotate_eii(int32_t n, int ncx, int NY, const int32_t *__restrict__ cellXa,
                                                     const int32_t *__restrict__ cellYa,
                                                     const double *__restrict__ Mwxy, double *__restrict__ sxx,
                                                     double *__restrict__ syy, double *__restrict__ sxy,
@@ -65,7 +67,7 @@ __global__ void __launch_bounds__(256) k_rotate_eii(int32_t n, int NX, int NY, c
                                                     double dt, int jaumann) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
-  rotate_eii_marker(m, NX, NY, cellXa, cellYa, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, dt, jaumann);
+  rotate_eii_marker(m, ncx, NY, cellXa, cellYa, Mwxy, sxx, syy, sxy, exx, eyy, exy, Eii, dt, jaumann);
 }
 
 static int launch_advect(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const NodalArrays *nf,
@@ -74,20 +76,24 @@ static int launch_advect(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const
   const int32_t n = mk->nMark;
   const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
   double *T = deltaT ? mk->T : nullptr, *Tdot = deltaT ? mk->Tdot : nullptr;
-  if (lds <= 32 * 1024) {
+  if (g->xperiodic) {
+    k_advect_rk4<false, true><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(
+        dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, T,
+        Tdot, deltaT, dTsub, dt_thermal, 1);
+  } else if (lds <= 32 * 1024) {
     const int tiles = std::max(2, std::min(8, (int)(lds / ADV_TILE_BYTES)));
-    k_advect_rk4<true><<<fvm_blocks(n, 256 * tiles), 256, lds, ctx->stream>>>(
+    k_advect_rk4<true, false><<<fvm_blocks(n, 256 * tiles), 256, lds, ctx->stream>>>(
         dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, T,
         Tdot, deltaT, dTsub, dt_thermal,
         tiles);
   } else {
-    k_advect_rk4<false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(
+    k_advect_rk4<false, false><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(
         dgrid(g), n, mk->X, mk->Y, mk->VX, mk->VY, mk->wxy, mk->cellX, mk->cellY, nf->vx, nf->vy, nf->wxy, *bv, dt, T,
         Tdot, deltaT, dTsub, dt_thermal,
         1);
   }
   FVM_LAUNCH_CHECK(ctx);
-  k_rotate_eii<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, g->NX, g->NY, mk->cellX, mk->cellY, mk->wxy, mk->sxx,
+  k_rotate_eii<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, fvm_ncx(g), g->NY, mk->cellX, mk->cellY, mk->wxy, mk->sxx,
                                                             mk->syy, mk->sxy, mk->exx, mk->eyy, mk->exy, mk->Eii, dt,
                                                             jaumann);
   FVM_LAUNCH_CHECK(ctx);
@@ -202,13 +208,15 @@ __global__ void __launch_bounds__(256) k_subgrid_s1(DGrid g, int32_t n, const do
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const int c0x = cXa[m], c0y = cYa[m];
-  if (!cell_ok(c0x, c0y, g.NX, g.NY)) {
+  if (!cell_ok(c0x, c0y, g.ncx(), g.NY)) {
     dxy[m] = 0.0;
     dxx[m] = 0.0;
     dyy[m] = 0.0;
     return;
   }
-  const int NX = g.NX, NY = g.NY;
+  // no xperiodic branch in subgridStressChanges.c: the clamps keep both
+  // gathers inside [0, NX-1] on a periodic grid too
+  const int NX = g.NX, NY = g.NY, P = g.per;
   const double X = MX[m], Y = MY[m];
   const double sdm = eta[m] / mu[m];
   double sdif = -dsubgrid * dt / sdm;
@@ -223,10 +231,10 @@ __global__ void __launch_bounds__(256) k_subgrid_s1(DGrid g, int32_t n, const do
   double mdx = (X - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
   double mdy = (Y - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
   double sxym = 0.0;
-  sxym += (1.0 - mdx) * (1.0 - mdy) * gat(soxy, NX, NY, cellY, cellX);
-  sxym += (1.0 - mdx) * mdy * gat(soxy, NX, NY, cellY + 1, cellX);
-  sxym += mdx * (1.0 - mdy) * gat(soxy, NX, NY, cellY, cellX + 1);
-  sxym += mdx * mdy * gat(soxy, NX, NY, cellY + 1, cellX + 1);
+  sxym += (1.0 - mdx) * (1.0 - mdy) * gat(soxy, NX, NY, P, cellY, cellX);
+  sxym += (1.0 - mdx) * mdy * gat(soxy, NX, NY, P, cellY + 1, cellX);
+  sxym += mdx * (1.0 - mdy) * gat(soxy, NX, NY, P, cellY, cellX + 1);
+  sxym += mdx * mdy * gat(soxy, NX, NY, P, cellY + 1, cellX + 1);
   double sxy0 = sxy[m];
   double dsxym = sxym - sxy0;
   dsxym *= sdif;
@@ -243,12 +251,12 @@ __global__ void __launch_bounds__(256) k_subgrid_s1(DGrid g, int32_t n, const do
   else if (cellX > NX - 2) cellX = NX - 2;
   mdx = (X - g.xc[cellX]) / (g.xc[cellX + 1] - g.xc[cellX]);
   mdy = (Y - g.yc[cellY]) / (g.yc[cellY + 1] - g.yc[cellY]);
-  const double sxxm = bil_acc(soxx, NX, NY, cellY, cellX, mdx, mdy);
+  const double sxxm = bil_acc(soxx, NX, NY, P, cellY, cellX, mdx, mdy);
   const double sxx0 = sxx[m];
   double dsxxm = sxxm - sxx0;
   dsxxm *= sdif;
   sxx[m] = sxx0 + dsxxm;
-  const double syym = bil_acc(soyy, NX, NY, cellY, cellX, mdx, mdy);
+  const double syym = bil_acc(soyy, NX, NY, P, cellY, cellX, mdx, mdy);
   const double syy0 = syy[m];
   double dsyym = syym - syy0;
   dsyym *= sdif;
@@ -308,20 +316,22 @@ __global__ void __launch_bounds__(256) k_update_stress(DGrid g, int32_t n, const
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const int c0x = cXa[m], c0y = cYa[m];
-  if (!cell_ok(c0x, c0y, g.NX, g.NY)) return;
-  const int NX = g.NX, NY = g.NY;
+  if (!cell_ok(c0x, c0y, g.ncx(), g.NY)) return;
+  const int NX = g.NX, NY = g.NY, P = g.per;
   const double X = MX[m], Y = MY[m];
   int cellX = c0x, cellY = c0y;
-  if (cellX < NX - 2 && X >= g.xc[cellX + 1]) cellX++;
+  // periodic: the centre cell may step to NX and is then clamped to NX-1
+  // (cellXMax NX, then NX-1); node column NX is column 0
+  if (cellX < (P ? NX : NX - 2) && X >= g.xc[cellX + 1]) cellX++;
   if (cellY < NY - 1 && Y >= g.yc[cellY + 1]) cellY++;
   if (cellY < 1) cellY = 1;
   else if (cellY > NY - 2) cellY = NY - 2;
   if (cellX < 1) cellX = 1;
-  else if (cellX > NX - 2) cellX = NX - 2;
+  else if (cellX > (P ? NX - 1 : NX - 2)) cellX = P ? NX - 1 : NX - 2;
   double mdx = (X - g.xc[cellX]) / (g.xc[cellX + 1] - g.xc[cellX]);
   double mdy = (Y - g.yc[cellY]) / (g.yc[cellY + 1] - g.yc[cellY]);
-  const double mdsxx = bil_acc(dsxx, NX, NY, cellY, cellX, mdx, mdy);
-  const double mdsyy = bil_acc(dsyy, NX, NY, cellY, cellX, mdx, mdy);
+  const double mdsxx = bil_acc(dsxx, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double mdsyy = bil_acc(dsyy, NX, NY, P, cellY, cellX, mdx, mdy);
   sxx[m] = sxx[m] + mdsxx;
   syy[m] = syy[m] + mdsyy;
   cellX = c0x;
@@ -329,10 +339,10 @@ __global__ void __launch_bounds__(256) k_update_stress(DGrid g, int32_t n, const
   if (cellY < 0) cellY = 0;
   else if (cellY > NY - 2) cellY = NY - 2;
   if (cellX < 0) cellX = 0;
-  else if (cellX > NX - 2) cellX = NX - 2;
+  else if (cellX > (P ? NX - 1 : NX - 2)) cellX = P ? NX - 1 : NX - 2;
   mdx = (X - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
   mdy = (Y - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
-  const double mdsxy = bil_acc(dsxy, NX, NY, cellY, cellX, mdx, mdy);
+  const double mdsxy = bil_acc(dsxy, NX, NY, P, cellY, cellX, mdx, mdy);
   sxy[m] = sxy[m] + mdsxy;
 }
 
@@ -367,36 +377,38 @@ __global__ void __launch_bounds__(256) k_strain_pressure(DGrid g, int32_t n, con
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const int c0x = cXa[m], c0y = cYa[m];
-  if (!cell_ok(c0x, c0y, g.NX, g.NY)) return;
-  const int NX = g.NX, NY = g.NY;
+  if (!cell_ok(c0x, c0y, g.ncx(), g.NY)) return;
+  const int NX = g.NX, NY = g.NY, P = g.per;
   const double X = MX[m], Y = MY[m];
   int cellX = c0x, cellY = c0y;
-  if (cellX < NX - 2 && X >= g.xc[cellX + 1]) cellX++;
+  // periodic: the centre cell may step to NX and is then clamped to NX-1
+  // (cellXMax NX, then NX-1); node column NX is column 0
+  if (cellX < (P ? NX : NX - 2) && X >= g.xc[cellX + 1]) cellX++;
   if (cellY < NY - 1 && Y >= g.yc[cellY + 1]) cellY++;
   if (cellY < 1) cellY = 1;
   else if (cellY > NY - 2) cellY = NY - 2;
   if (cellX < 1) cellX = 1;
-  else if (cellX > NX - 2) cellX = NX - 2;
+  else if (cellX > (P ? NX - 1 : NX - 2)) cellX = P ? NX - 1 : NX - 2;
   double mdx = (X - g.xc[cellX]) / (g.xc[cellX + 1] - g.xc[cellX]);
   double mdy = (Y - g.yc[cellY]) / (g.yc[cellY + 1] - g.yc[cellY]);
-  Mp[m] = bil_acc(F.p, NX, NY, cellY, cellX, mdx, mdy);
-  const double mexx = bil_acc(F.exx, NX, NY, cellY, cellX, mdx, mdy);
-  const double msxx = bil_acc(F.soxx, NX, NY, cellY, cellX, mdx, mdy);
-  const double msyy = bil_acc(F.soyy, NX, NY, cellY, cellX, mdx, mdy);
-  const double mdsxx = bil_acc(F.dsxx, NX, NY, cellY, cellX, mdx, mdy);
-  const double mdsyy = bil_acc(F.dsyy, NX, NY, cellY, cellX, mdx, mdy);
+  Mp[m] = bil_acc(F.p, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double mexx = bil_acc(F.exx, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double msxx = bil_acc(F.soxx, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double msyy = bil_acc(F.soyy, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double mdsxx = bil_acc(F.dsxx, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double mdsyy = bil_acc(F.dsyy, NX, NY, P, cellY, cellX, mdx, mdy);
   cellX = c0x;
   cellY = c0y;
   if (cellY < 0) cellY = 0;
   else if (cellY > NY - 2) cellY = NY - 2;
   if (cellX < 0) cellX = 0;
-  else if (cellX > NX - 2) cellX = NX - 2;
+  else if (cellX > (P ? NX - 1 : NX - 2)) cellX = P ? NX - 1 : NX - 2;
   mdx = (X - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
   mdy = (Y - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
-  Mwxy[m] = bil_acc(F.wxy, NX, NY, cellY, cellX, mdx, mdy);
-  const double mexy = bil_acc(F.exy, NX, NY, cellY, cellX, mdx, mdy);
-  const double msxy = bil_acc(F.soxy, NX, NY, cellY, cellX, mdx, mdy);
-  const double mdsxy = bil_acc(F.dsxy, NX, NY, cellY, cellX, mdx, mdy);
+  Mwxy[m] = bil_acc(F.wxy, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double mexy = bil_acc(F.exy, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double msxy = bil_acc(F.soxy, NX, NY, P, cellY, cellX, mdx, mdy);
+  const double mdsxy = bil_acc(F.dsxy, NX, NY, P, cellY, cellX, mdx, mdy);
   const double em = eta[m], um = mu[m];
   const double t11 = msxx / 2.0 / em + mdsxx / 2.0 / dt / um;
   const double t22 = msyy / 2.0 / em + mdsyy / 2.0 / dt / um;
@@ -688,12 +700,14 @@ __global__ void k_nodal_to_markers(DGrid g, int32_t n, const double *__restrict_
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const int cellX = cX[m], cellY = cY[m];
-  if (!cell_ok(cellX, cellY, g.NX, g.NY)) return;
-  const int NX = g.NX, NY = g.NY;
+  if (!cell_ok(cellX, cellY, g.ncx(), g.NY)) return;
+  const int NX = g.NX, NY = g.NY, P = g.per;
   const double mdx = (MX[m] - g.x[cellX]) / (g.x[cellX + 1] - g.x[cellX]);
   const double mdy = (MY[m] - g.y[cellY]) / (g.y[cellY + 1] - g.y[cellY]);
-  out[m] = gat(f, NX, NY, cellY, cellX) * (1 - mdx) * (1 - mdy) + gat(f, NX, NY, cellY + 1, cellX) * (1 - mdx) * mdy +
-           gat(f, NX, NY, cellY, cellX + 1) * (mdx) * (1 - mdy) + gat(f, NX, NY, cellY + 1, cellX + 1) * mdx * mdy;
+  out[m] = gat(f, NX, NY, P, cellY, cellX) * (1 - mdx) * (1 - mdy) +
+           gat(f, NX, NY, P, cellY + 1, cellX) * (1 - mdx) * mdy +
+           gat(f, NX, NY, P, cellY, cellX + 1) * (mdx) * (1 - mdy) +
+           gat(f, NX, NY, P, cellY + 1, cellX + 1) * mdx * mdy;
 }
 
 extern "C" int fvm_project_nodal_field_to_markers(fvm_ctx *ctx, const fvm_grid *g, const double *nodalField,

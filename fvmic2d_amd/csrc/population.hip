@@ -449,7 +449,7 @@ int64_t scan_state_words(int64_t n) { return fvm_scan_tiles(n) + 2; }
 extern "C" int fvm_check_marker_density(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, const Materials *mat,
                                         const Options *opt, uint64_t seed, int32_t *nAdded, int32_t *nRemoved) {
   if (!ctx || !mk || !mat || !opt) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "checkMarkerDensity"));
   if (nAdded) *nAdded = 0;
   if (nRemoved) *nRemoved = 0;
   const int32_t n = mk->nMark;

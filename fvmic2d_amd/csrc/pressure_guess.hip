@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(256) k_store_p(int64_t n, const double *__rest
 extern "C" int fvm_pressure_guess_apply(fvm_ctx *ctx, const fvm_grid *g, const double *rho, double gy, double gx,
                                         const double *x, double *y, double *b) {
   if (!ctx || !rho || !x || !y) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "initialPressureGuess"));
   if (g->NX < 5 || g->NY < 5) return fvm_set_err(ctx, FVM_ERR_ARG, "pressure guess needs NX, NY >= 5");
   const int64_t n = (int64_t)g->NX * g->NY;
   const PGrid pg = pgrid(g);
@@ -393,7 +393,7 @@ extern "C" int fvm_initial_pressure_guess(fvm_ctx *ctx, const fvm_grid *g, const
                                           double rtol, int32_t maxit, double *S, double *p_out, int32_t *iters,
                                           double *relres) {
   if (!ctx || !rho || (!S && !p_out) || !(rtol > 0.0) || maxit < 1) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "initialPressureGuess"));
   if (g->NX < 5 || g->NY < 5) return fvm_set_err(ctx, FVM_ERR_ARG, "pressure guess needs NX, NY >= 5 (pinned cell (3,3))");
   const int64_t n = (int64_t)g->NX * g->NY;
   BcgVecs V;

@@ -13,6 +13,7 @@
 struct Lat {
   int lo, n;  // lattice cell index range [lo, lo+n)
 };
+// N - 1 basic cells along the axis: N = NX (NX + 1 along a periodic x) or NY
 __host__ __device__ __forceinline__ Lat lattice(int stag, int N) {
   Lat L;
   if (stag == 0) {
@@ -193,8 +194,8 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
       qlo = (1 - jx) + 2 * (1 - jy);
       qhi = qlo + 1;
     }
-    ok = ok && live && cx >= 0 && cx <= g.NX - 2 && cy >= 0 && cy <= g.NY - 2;
-    const int kb = (cy * (g.NX - 1) + cx) << 2;
+    ok = ok && live && cx >= 0 && cx < g.ncx() && cy >= 0 && cy <= g.NY - 2;
+    const int kb = (cy * g.ncx() + cx) << 2;
     const int a0 = ok ? start[kb + qlo] : 0;
     const int a1 = ok ? start[kb + qhi] : 0;
     rs[r] = a0;
@@ -210,15 +211,17 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
     for (int k = 0; k < NF; k++) sf[c][k] = 0.0;
   }
   // pass-1 inputs at the corners of basic cell (lx, ly) (SUBT only; a cell with
-  // markers has lx in [0, NX-2], ly in [0, NY-2])
+  // markers has lx in [0, ncx-1], ly in [0, NY-2]; the right column lx + 1 is
+  // node 0 for the last cell of a periodic grid)
   double T00 = 0, T10 = 0, T01 = 0, T11 = 0, r00 = 0, r10 = 0, r01 = 0, r11 = 0;
   double C00 = 0, C10 = 0, C01 = 0, C11 = 0, k00 = 0, k10 = 0, k01 = 0, k11 = 0, sden = 1.0;
   if (SUBT && tot > 0) {
     const int64_t i0 = (int64_t)ly * g.NX + lx, i1 = i0 + g.NX;
-    T00 = sb.T[i0], T10 = sb.T[i0 + 1], T01 = sb.T[i1], T11 = sb.T[i1 + 1];
-    r00 = sb.rho[i0], r10 = sb.rho[i0 + 1], r01 = sb.rho[i1], r11 = sb.rho[i1 + 1];
-    C00 = sb.Cp[i0], C10 = sb.Cp[i0 + 1], C01 = sb.Cp[i1], C11 = sb.Cp[i1 + 1];
-    k00 = sb.k[i0], k10 = sb.k[i0 + 1], k01 = sb.k[i1], k11 = sb.k[i1 + 1];
+    const int64_t e = (lx + 1 == g.NX) ? 1 - g.NX : 1;  // offset of the right column
+    T00 = sb.T[i0], T10 = sb.T[i0 + e], T01 = sb.T[i1], T11 = sb.T[i1 + e];
+    r00 = sb.rho[i0], r10 = sb.rho[i0 + e], r01 = sb.rho[i1], r11 = sb.rho[i1 + e];
+    C00 = sb.Cp[i0], C10 = sb.Cp[i0 + e], C01 = sb.Cp[i1], C11 = sb.Cp[i1 + e];
+    k00 = sb.k[i0], k10 = sb.k[i0 + e], k01 = sb.k[i1], k11 = sb.k[i1 + e];
     const double dx = g.x[lx + 1] - g.x[lx];
     const double dy = g.y[ly + 1] - g.y[ly];
     sden = 2.0 / (dx * dx) + 2.0 / (dy * dy);
@@ -386,9 +389,23 @@ struct NodeGroups {
   unsigned char pg[MAX_NODE_GROUPS * FVM_MAXF], pk[MAX_NODE_GROUPS * FVM_MAXF];
 };
 
-// Pass 2 for node (ix, jy) (flat index i), field k of one group.
-// Pass 2 for node (ix, jy), every field of one group: the corner weights are
-// read once; each field's result is the one node_field gives.
+// The result of one field at node i from its summed weight and value.
+__device__ __forceinline__ void node_store(const NodeGroup &q, int k, int64_t i, double den, double num, double ea,
+                                           double eb) {
+  const double r = q.harm ? den / num : num / den;
+  double *o = q.out[k];
+  if (q.epilogue == EPI_DIVIDE) o[i] = r;
+  // VecAXPBYPCZ(z, 1, -1, -1, x=a, y=b): z = x + (-1)*y + (-1)*z
+  else if (q.epilogue == EPI_SUBGRID_T) o[i] = ea + -1.0 * eb + -1.0 * r;
+  // VecAXPY(out, -1, r)
+  else o[i] = o[i] + -1.0 * r;
+}
+
+// Pass 2 for node (ix, jy) (flat index i), every field of one group: the
+// (lattice cell, corner) pairs touching the node are (ix-1, jy-1) corner 3,
+// (ix, jy-1) corner 2, (ix-1, jy) corner 1, (ix, jy) corner 0, added in that
+// order (global = 0 + local: DMLocalToGlobal ADD into a zeroed vector), then
+// normalised.  The corner weights are read once.
 __device__ __forceinline__ void node_group(const NodeGroup &q, int ix, int jy, int64_t i) {
   const int64_t nL = (int64_t)q.nLx * q.nLy;
   const int st = q.pst;
@@ -396,6 +413,7 @@ __device__ __forceinline__ void node_group(const NodeGroup &q, int ix, int jy, i
   const bool vx0 = ax0 >= 0 && ax0 < q.nLx, vx1 = ax1 >= 0 && ax1 < q.nLx;
   const bool vy0 = ay0 >= 0 && ay0 < q.nLy, vy1 = ay1 >= 0 && ay1 < q.nLy;
   const bool v3 = vx0 && vy0, v2 = vx1 && vy0, v1 = vx0 && vy1, v0 = vx1 && vy1;
+  // absent corners read element 0 (a valid address) and are dropped below
   const int64_t L3 = v3 ? (int64_t)ay0 * q.nLx + ax0 : 0, L2 = v2 ? (int64_t)ay0 * q.nLx + ax1 : 0;
   const int64_t L1 = v1 ? (int64_t)ay1 * q.nLx + ax0 : 0, L0 = v0 ? (int64_t)ay1 * q.nLx + ax1 : 0;
   const double *P = q.part;
@@ -418,48 +436,46 @@ __device__ __forceinline__ void node_group(const NodeGroup &q, int ix, int jy, i
     if (v1) num += f1;
     if (v0) num += f0;
     num = 0.0 + num;
-    const double r = q.harm ? den / num : num / den;
-    double *o = q.out[k];
-    if (q.epilogue == EPI_DIVIDE) o[i] = r;
-    else if (q.epilogue == EPI_SUBGRID_T) o[i] = ea + -1.0 * eb + -1.0 * r;
-    else o[i] = o[i] + -1.0 * r;
+    node_store(q, k, i, den, num, ea, eb);
   }
 }
 
-__device__ __forceinline__ void node_field(const NodeGroup &q, int ix, int jy, int64_t i, int k) {
+// The same on a periodic x axis: node ix also collects the corners that land
+// on its ghost images ix + NX and ix - NX (DM_BOUNDARY_PERIODIC adds a ghost
+// column into the node it wraps to).  Images in the order ix, ix + NX, ix - NX;
+// corners in the order above within each.
+__device__ __forceinline__ void node_group_periodic(const NodeGroup &q, int ix, int jy, int64_t i, int NX) {
   const int64_t nL = (int64_t)q.nLx * q.nLy;
   const int st = q.pst;
-  // (lattice cell, corner) pairs touching node (ix, jy), in a fixed order:
-  // (ix-1, jy-1) corner 3, (ix, jy-1) corner 2, (ix-1, jy) corner 1, (ix, jy) corner 0
-  const int ax0 = ix - 1 - q.lox, ax1 = ix - q.lox, ay0 = jy - 1 - q.loy, ay1 = jy - q.loy;
-  const bool vx0 = ax0 >= 0 && ax0 < q.nLx, vx1 = ax1 >= 0 && ax1 < q.nLx;
+  const int ay0 = jy - 1 - q.loy, ay1 = jy - q.loy;
   const bool vy0 = ay0 >= 0 && ay0 < q.nLy, vy1 = ay1 >= 0 && ay1 < q.nLy;
-  const bool v3 = vx0 && vy0, v2 = vx1 && vy0, v1 = vx0 && vy1, v0 = vx1 && vy1;
-  // absent corners read element 0 (a valid address) and are dropped below
-  const int64_t L3 = v3 ? (int64_t)ay0 * q.nLx + ax0 : 0, L2 = v2 ? (int64_t)ay0 * q.nLx + ax1 : 0;
-  const int64_t L1 = v1 ? (int64_t)ay1 * q.nLx + ax0 : 0, L0 = v0 ? (int64_t)ay1 * q.nLx + ax1 : 0;
+  int64_t L[12];
+  bool v[12];
+  int c[12];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    const int xi = ix + (s == 0 ? 0 : (s == 1 ? NX : -NX));
+    const int ax0 = xi - 1 - q.lox, ax1 = xi - q.lox;
+    const bool vx0 = ax0 >= 0 && ax0 < q.nLx, vx1 = ax1 >= 0 && ax1 < q.nLx;
+    v[4 * s + 0] = vx0 && vy0, c[4 * s + 0] = 3, L[4 * s + 0] = (int64_t)ay0 * q.nLx + ax0;
+    v[4 * s + 1] = vx1 && vy0, c[4 * s + 1] = 2, L[4 * s + 1] = (int64_t)ay0 * q.nLx + ax1;
+    v[4 * s + 2] = vx0 && vy1, c[4 * s + 2] = 1, L[4 * s + 2] = (int64_t)ay1 * q.nLx + ax0;
+    v[4 * s + 3] = vx1 && vy1, c[4 * s + 3] = 0, L[4 * s + 3] = (int64_t)ay1 * q.nLx + ax1;
+  }
   const double *P = q.part;
-  const double w3 = P[(int64_t)(3 * st) * nL + L3], w2 = P[(int64_t)(2 * st) * nL + L2];
-  const double w1 = P[(int64_t)(1 * st) * nL + L1], w0 = P[L0];
-  const double f3 = P[(int64_t)(3 * st + 1 + k) * nL + L3], f2 = P[(int64_t)(2 * st + 1 + k) * nL + L2];
-  const double f1 = P[(int64_t)(1 * st + 1 + k) * nL + L1], f0 = P[(int64_t)(1 + k) * nL + L0];
-  // global = 0 + local (DMLocalToGlobal ADD into a zeroed vector), then divide
-  double den = 0.0, num = 0.0;
-  if (v3) den += w3, num += f3;
-  if (v2) den += w2, num += f2;
-  if (v1) den += w1, num += f1;
-  if (v0) den += w0, num += f0;
+  double den = 0.0;
+#pragma unroll
+  for (int e = 0; e < 12; e++)
+    if (v[e]) den += P[(int64_t)(c[e] * st) * nL + L[e]];
   den = 0.0 + den;
-  num = 0.0 + num;
-  const double r = q.harm ? den / num : num / den;
-  double *o = q.out[k];
-  if (q.epilogue == EPI_DIVIDE) {
-    o[i] = r;
-  } else if (q.epilogue == EPI_SUBGRID_T) {
-    // VecAXPBYPCZ(z, 1, -1, -1, x=a, y=b): z = x + (-1)*y + (-1)*z
-    o[i] = q.ea[i] + -1.0 * q.eb[i] + -1.0 * r;
-  } else {
-    // VecAXPY(out, -1, r)
-    o[i] = o[i] + -1.0 * r;
+  double ea = 0.0, eb = 0.0;
+  if (q.epilogue == EPI_SUBGRID_T) ea = q.ea[i], eb = q.eb[i];
+  for (int k = 0; k < q.nf; k++) {
+    double num = 0.0;
+#pragma unroll
+    for (int e = 0; e < 12; e++)
+      if (v[e]) num += P[(int64_t)(c[e] * st + 1 + k) * nL + L[e]];
+    num = 0.0 + num;
+    node_store(q, k, i, den, num, ea, eb);
   }
 }

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const i
 }
 
 // Several groups of one kind (method, field count) in one launch: the launch
-// covers lattice-cell coordinates (cx, cy) in [-1, NX-1] x [-1, NY-1] and each
+// covers lattice-cell coordinates (cx, cy) in [-1, ncx] x [-1, NY-1] and each
 // lane group computes, family after family, the lattice cell at (cx, cy) of
 // every group whose lattice contains it.  Neighbouring families' cells at one
 // coordinate cover nearly the same markers, so the later families read them
@@ -60,12 +60,13 @@ __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg,
   const int64_t tile = b / mg.ng;
   const int64_t t = (tile * blockDim.x + threadIdx.x) / G;
   const int j = threadIdx.x % G;
-  const int64_t nT = (int64_t)(g.NX + 1) * (g.NY + 1);
+  const int ntx = g.ncx() + 2;
+  const int64_t nT = (int64_t)ntx * (g.NY + 1);
   const bool inside = t < nT;
-  const int cx = inside ? (int)(t % (g.NX + 1)) - 1 : -2;
-  const int cy = inside ? (int)(t / (g.NX + 1)) - 1 : -2;
+  const int cx = inside ? (int)(t % ntx) - 1 : -2;
+  const int cy = inside ? (int)(t / ntx) - 1 : -2;
   const ProjGroup &pg = mg.pg[q];
-  const Lat Lx = lattice(pg.stagx, g.NX), Ly = lattice(pg.stagy, g.NY);
+  const Lat Lx = lattice(pg.stagx, g.ncx() + 1), Ly = lattice(pg.stagy, g.NY);
   const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
   const int64_t nL = (int64_t)Lx.n * Ly.n;
   const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
@@ -100,11 +101,12 @@ __global__ void __launch_bounds__(256) k_gather_payload(int32_t n, const int32_t
 }
 
 // one thread per node; launch row blockIdx.y = group (all its fields)
-__global__ void __launch_bounds__(256) k_nodes(int NX, int NY, NodeGroups G) {
+__global__ void __launch_bounds__(256) k_nodes(int NX, int NY, int per, NodeGroups G) {
   const int64_t i = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= (int64_t)NX * NY) return;
   const int ix = (int)(i % NX), jy = (int)(i / NX);
-  node_group(G.q[blockIdx.y], ix, jy, i);
+  if (per) node_group_periodic(G.q[blockIdx.y], ix, jy, i, NX);
+  else node_group(G.q[blockIdx.y], ix, jy, i);
 }
 
 struct Payload {
@@ -115,7 +117,7 @@ struct Payload {
 template <int METHOD, int NF>
 static int launch_partials(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, const ProjGroup &pg,
                            double *part) {
-  const Lat Lx = lattice(pg.stagx, g->NX), Ly = lattice(pg.stagy, g->NY);
+  const Lat Lx = lattice(pg.stagx, fvm_ncx(g) + 1), Ly = lattice(pg.stagy, g->NY);
   const int64_t nL = (int64_t)Lx.n * Ly.n;
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
   const int G = ctx->lanes;
@@ -132,7 +134,7 @@ static int launch_partials(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, c
 
 template <int METHOD, int NF>
 static int launch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, const MultiGroups &mg) {
-  const int64_t nT = (int64_t)(g->NX + 1) * (g->NY + 1);
+  const int64_t nT = (int64_t)(fvm_ncx(g) + 2) * (g->NY + 1);
   FVM_PHASE(ctx, FVM_PHASE_PARTIALS);
   const int G = ctx->lanes;
 #define LAUNCH_M(GG)                                                                                     \
@@ -189,7 +191,7 @@ static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const Payload &mk, c
   if (sb) {
     if (pg.stagx != 0 || pg.stagy != 0 || pg.method != PM_ARITH || pg.nf != 1)
       return fvm_set_err(ctx, FVM_ERR_ARG, "fused subgrid pass needs one ARITHMETIC field on basic nodes");
-    const Lat Lx = lattice(0, g->NX), Ly = lattice(0, g->NY);
+    const Lat Lx = lattice(0, fvm_ncx(g) + 1), Ly = lattice(0, g->NY);
     const int64_t nL = (int64_t)Lx.n * Ly.n;
     FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
     const int G = ctx->lanes;
@@ -249,7 +251,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
       }
   }
   for (int i = 0; i < ng; i++) {
-    const Lat Lx = lattice(sg[i].stagx, g->NX), Ly = lattice(sg[i].stagy, g->NY);
+    const Lat Lx = lattice(sg[i].stagx, fvm_ncx(g) + 1), Ly = lattice(sg[i].stagy, g->NY);
     off[i] = total;
     total += (size_t)Lx.n * Ly.n * 4 * (sg[i].nf + 1);
   }
@@ -317,7 +319,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
   }
   if (!gathered && n > 0 && (!have_xy || pc.nc > 0)) {
     FVM_PHASE(ctx, FVM_PHASE_SORT);
-    const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+    const int64_t nk = 4LL * fvm_ncx(g) * (g->NY - 1);
     k_gather_payload<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, (const int32_t *)ctx->start.ptr + nk,
                                                                   (const int32_t *)ctx->perm.ptr, mk->X, mk->Y,
                                                                   mk->Mat, xs, ys, ms, pc, have_xy ? 0 : 1);
@@ -355,7 +357,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
     memset(&G, 0, sizeof(G));
     for (int i = i0; i < ng && G.ng < MAX_NODE_GROUPS; i++) {
       const ProjGroup &pg = pgs[i];
-      const Lat Lx = lattice(pg.stagx, g->NX), Ly = lattice(pg.stagy, g->NY);
+      const Lat Lx = lattice(pg.stagx, fvm_ncx(g) + 1), Ly = lattice(pg.stagy, g->NY);
       NodeGroup &q = G.q[G.ng++];
       q.part = part + off[i];
       q.nf = pg.nf;
@@ -376,7 +378,7 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
       q.eb = pg.eb;
     }
     FVM_PHASE(ctx, FVM_PHASE_NODES);
-    k_nodes<<<dim3(fvm_blocks(nn, 256), G.ng), 256, 0, ctx->stream>>>(g->NX, g->NY, G);
+    k_nodes<<<dim3(fvm_blocks(nn, 256), G.ng), 256, 0, ctx->stream>>>(g->NX, g->NY, g->xperiodic ? 1 : 0, G);
     FVM_LAUNCH_CHECK(ctx);
   }
   return FVM_OK;

@@ -1,7 +1,8 @@
 // Marker cell location (findMarkerCells) and the deterministic bucket order
 // used by every projection.
 //
-// Bucket key of an active marker: ((cellY*(NX-1) + cellX) << 2) | bx | (by << 1)
+// Bucket key of an active marker: ((cellY*ncx + cellX) << 2) | bx | (by << 1),
+// ncx = NX - 1 basic cells along x (NX on a periodic grid),
 // with bx = X >= xc[cellX+1], by = Y >= yc[cellY+1] -- the quadrant tests of
 // findCellStag2D (markerProjection.c:258-268).  For every staggered node family
 // all markers of one bucket fall into the same family cell, so a family cell is
@@ -36,18 +37,23 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
   extern __shared__ double sgrid[];
   const double *gx = g.x, *gy = g.y, *gxc = g.xc, *gyc = g.yc;
   if (LDS) {
-    // x[0..NX-1], y[0..NY-1], xc[0..NX], yc[0..NY]
-    for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
-    for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
-    for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
-    for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
+    // x[0..nxn-1], y[0..NY-1], xc[0..NX], yc[0..NY]; nxn = NX + per nodes
+    // bound the cells along x (x[NX] = LX closes a periodic grid's last cell)
+    const int nxn = g.NX + g.per;
+    for (int i = threadIdx.x; i < nxn; i += blockDim.x) sgrid[i] = g.x[i];
+    for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[nxn + i] = g.y[i];
+    for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[nxn + g.NY + i] = g.xc[i];
+    for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[nxn + g.NX + g.NY + 1 + i] = g.yc[i];
     __syncthreads();
     gx = sgrid;
-    gy = sgrid + g.NX;
-    gxc = sgrid + g.NX + g.NY;
-    gyc = sgrid + 2 * g.NX + g.NY + 1;
+    gy = sgrid + nxn;
+    gxc = sgrid + nxn + g.NY;
+    gyc = sgrid + nxn + g.NX + g.NY + 1;
   }
-  const double ginvx = (double)(g.NX - 1) / (gx[g.NX - 1] - gx[0]);
+  // findMarkerCell bisects x over [0, NX - 1], or [0, NX] on a periodic grid
+  // (markers.c:49), and accepts cells up to NX - 2, or NX - 1 (markers.c:73)
+  const int ncx = g.ncx();
+  const double ginvx = (double)ncx / (gx[ncx] - gx[0]);
   const double ginvy = (double)(g.NY - 1) / (gy[g.NY - 1] - gy[0]);
   // each block locates `tiles` consecutive tiles of blockDim.x markers, so that
   // the grid staged in LDS serves more than one tile
@@ -59,14 +65,14 @@ __global__ void __launch_bounds__(256) k_locate(DGrid g, int32_t n, const double
       const double xm = X[m], ym = Y[m];
       int cx = -1, cy = -1;
       if (xm >= 0.0 && ym >= 0.0 && xm <= g.LX && ym <= g.LY) {
-        cx = guess_grid(gx, g.NX, xm, ginvx);
+        cx = guess_grid(gx, ncx + 1, xm, ginvx);
         cy = guess_grid(gy, g.NY, ym, ginvy);
-        if (cx < 0 || cx > g.NX - 2 || cy < 0 || cy > g.NY - 2) {
+        if (cx < 0 || cx > ncx - 1 || cy < 0 || cy > g.NY - 2) {
           bad = 1;
         } else {
           const int bx = xm >= gxc[cx + 1] ? 1 : 0;
           const int by = ym >= gyc[cy + 1] ? 1 : 0;
-          k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+          k = ((cy * ncx + cx) << 2) | bx | (by << 1);
         }
       } else {
         out = 1;
@@ -107,10 +113,10 @@ __global__ void __launch_bounds__(256) k_keys(DGrid g, int32_t n, const double *
   int k = -1;
   if (m < n) {
     const int cx = cellX[m], cy = cellY[m];
-    if (cell_ok(cx, cy, g.NX, g.NY)) {  // inactive (or invalid) markers take no part
+    if (cell_ok(cx, cy, g.ncx(), g.NY)) {  // inactive (or invalid) markers take no part
       const int bx = X[m] >= g.xc[cx + 1] ? 1 : 0;
       const int by = Y[m] >= g.yc[cy + 1] ? 1 : 0;
-      k = ((cy * (g.NX - 1) + cx) << 2) | bx | (by << 1);
+      k = ((cy * g.ncx() + cx) << 2) | bx | (by << 1);
     }
     key[m] = k;
   }
@@ -285,7 +291,7 @@ __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, int32_t ncx
 }
 
 static int sort_buffers(fvm_ctx *ctx, const fvm_grid *g, int32_t n) {
-  const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+  const int64_t nk = 4LL * fvm_ncx(g) * (g->NY - 1);
   const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
   FVM_TRY(fvm_ws(ctx, ctx->key, (size_t)(n > 0 ? n : 1) * 4));
   FVM_TRY(fvm_ws(ctx, ctx->slot, (size_t)(n > 0 ? n : 1) * 4));
@@ -304,7 +310,7 @@ static int zero_counts(fvm_ctx *ctx, const fvm_grid *g) {
   if (!g) {
     k_zero_counts<<<1, 64, 0, ctx->stream>>>(nullptr, 0, counters, nullptr, 0);
   } else {
-    const int64_t nk = 4LL * (g->NX - 1) * (g->NY - 1);
+    const int64_t nk = 4LL * fvm_ncx(g) * (g->NY - 1);
     const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
     k_zero_counts<<<fvm_blocks(nk + 1, 256), 256, 0, ctx->stream>>>(
         (int32_t *)ctx->count.ptr, nk + 1, counters, (unsigned long long *)ctx->blocksums.ptr, ntiles + 2);
@@ -317,7 +323,7 @@ static int zero_counts(fvm_ctx *ctx, const fvm_grid *g) {
 static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, const SortGather *sg) {
   FVM_PHASE(ctx, FVM_PHASE_SORT);
   const int32_t n = mk->nMark;
-  const int64_t ncell = (int64_t)(g->NX - 1) * (g->NY - 1);
+  const int64_t ncell = (int64_t)fvm_ncx(g) * (g->NY - 1);
   const int64_t nk = 4 * ncell;
   const int64_t ntiles = (nk + SCAN_TILE - 1) / SCAN_TILE;
   int32_t *key = (int32_t *)ctx->key.ptr, *slot = (int32_t *)ctx->slot.ptr, *perm = (int32_t *)ctx->perm.ptr;
@@ -330,7 +336,7 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, c
     FVM_LAUNCH_CHECK(ctx);
   }
   if (sg && n > 0) {
-    const int32_t ncx = g->NX - 1, ncy = g->NY - 1;
+    const int32_t ncx = fvm_ncx(g), ncy = g->NY - 1;
     const int64_t ntiles = (int64_t)((ncx + 7) / 8) * ((ncy + 7) / 8);
     k_bucket_order<true><<<fvm_blocks(ntiles * 64 * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, ncx, ncy, start,
                                                                                    perm, *sg);
@@ -367,7 +373,7 @@ int fvm_locate_async(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t r
     int32_t *key = build_order ? (int32_t *)ctx->key.ptr : nullptr;
     int32_t *slot = build_order ? (int32_t *)ctx->slot.ptr : nullptr;
     int32_t *count = build_order ? (int32_t *)ctx->count.ptr : nullptr;
-    const size_t lds = (size_t)(2 * g->NX + 2 * g->NY + 2) * sizeof(double);
+    const size_t lds = (size_t)(2 * g->NX + (g->xperiodic ? 1 : 0) + 2 * g->NY + 2) * sizeof(double);
     if (lds <= 48 * 1024) {
       // one tile of 256 markers per LOC_TILE_BYTES of staged grid, at least 2
       // (at 201 x 201, 6.4 KB staged, 2 tiles locate in 37 us against 42 for 1)

@@ -161,7 +161,7 @@ extern "C" int fvm_form_thermal_system(fvm_ctx *ctx, const fvm_grid *g, const No
 extern "C" int fvm_solve_thermal_system(fvm_ctx *ctx, const fvm_grid *g, const double *coef, const double *rhs,
                                         double *T, double rtol, int32_t maxit, int32_t *iters, double *relres) {
   if (!ctx || !coef || !rhs || !T || !(rtol > 0.0) || maxit < 1) return FVM_ERR_ARG;
-  FVM_TRY(fvm_grid_check(ctx, g));
+  FVM_TRY(fvm_grid_check_nonperiodic(ctx, g, "thermal solve"));
   const int64_t nn = (int64_t)g->NX * g->NY;
   FVM_TRY(fvm_ws(ctx, ctx->partials, (size_t)6 * nn * sizeof(double)));
   double *sc = (double *)ctx->partials.ptr, *srhs = sc + 5 * nn;

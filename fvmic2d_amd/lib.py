@@ -276,7 +276,8 @@ class DeviceGrid:
         self.ctx = ctx
         self.NX, self.NY = w.NX, w.NY
         self._arrs = [DeviceArray(ctx, a) for a in (w.xp, w.yp, w.xcp, w.ycp)]
-        self.desc = fvm_grid(w.NX, w.NY, w.LX, w.LY, 0, *[a.ptr + 16 for a in self._arrs])
+        self.desc = fvm_grid(w.NX, w.NY, w.LX, w.LY, int(getattr(w, "xperiodic", 0)),
+                             *[a.ptr + 16 for a in self._arrs])
 
     @property
     def ref(self):

@@ -113,6 +113,32 @@ def pad_grid(x, L):
     return xp, xcp
 
 
+def pad_periodic(x, L):
+    """Padded node and centre arrays of a periodic x axis (length NX+4, element 2
+    is index 0).  x holds the NX+1 nodes 0 = x[0] < ... < x[NX] = L of the NX
+    cells; centres follow getCellCenters over NX nodes (gridSpacing.c:150-157),
+    then the ghosts of makePeriodic (gridSpacing.c:159-168): x[-2], x[-1],
+    x[NX], x[NX+1], xc[-1], xc[0], xc[NX], xc[NX+1].  xc[-2], which the
+    reference leaves unset, is extrapolated."""
+    NX = len(x) - 1
+    xp = np.empty(NX + 4)
+    xp[2:NX + 2] = x[:NX]
+    xp[NX + 2] = L
+    xp[NX + 3] = L + x[1]
+    xp[1] = x[NX - 1] - L
+    xp[0] = x[NX - 2] - L
+    xc = np.empty(NX + 2)
+    xc[1:NX] = (x[0:NX - 1] + x[1:NX]) / 2.0
+    xc[NX] = (L + x[NX - 1]) / 2.0
+    xc[NX + 1] = xc[1] + L
+    xc[0] = 0.0 - (L - xc[NX])
+    xcp = np.empty(NX + 4)
+    xcp[2:NX + 4] = xc
+    xcp[1] = 0.0 - (L - xc[NX - 1])
+    xcp[0] = xcp[1] - (xcp[2] - xcp[1])
+    return xp, xcp
+
+
 # ------------------------------------------------------------ workload ----
 @dataclass
 class Workload:
@@ -134,6 +160,7 @@ class Workload:
     kind: str                 # "plain" or "vep"
     projections: list = field(default_factory=list)  # (src, stagx, stagy, method, out)
     seed: int = 0
+    xperiodic: int = 0        # 1: x periodic (grid from pad_periodic)
 
     @property
     def x(self):
@@ -501,10 +528,137 @@ def vep_subduction(seed=3, NX=801, NY=401, NM=5):
     return w
 
 
+def _periodic_fourier(rng, X, Y, LX, LY, nmodes=3):
+    """A smooth field, periodic in x over LX: sum of a_ij cos(2 pi i x / LX + p_ij)
+    sin/cos(pi j y / LY) with seeded coefficients."""
+    a = rng.normal(2 * nmodes * nmodes)
+    v = np.zeros(np.broadcast(X, Y).shape)
+    k = 0
+    for i in range(nmodes):
+        for j in range(nmodes):
+            v = v + a[k] / (1.0 + i * i + j * j) * np.cos(2.0 * np.pi * i * X / LX + a[k + 1]) * \
+                np.cos(np.pi * j * Y / LY + 0.3 * j)
+            k += 2
+    return v
+
+
+def periodic_box(kind="plain", NX=64, NY=41, NM=4, seed=8, graded_y=True):
+    """A box periodic in x (the reference's grid for mechBCLeft.type[0] == 2,
+    gridGenerator.c:66-85): NX uniform cells over LX (x[NX] = LX closes the
+    last one), a tanh-stretched or uniform y axis, NM x NM jittered markers per
+    cell plus markers exactly on x = 0 and x = LX and a few outside the box.
+    The velocity has a mean drift along x of about half a cell per step, so
+    markers cross x = LX and x = 0 and are wrapped.
+
+    kind "plain": isoviscous thermal convection inputs (the convection_small
+    fields); kind "vep": five materials with yielding and every nodal input of
+    the marker stress and strain updates (vep_subduction's fields, periodic)."""
+    LX, LY = (2.0, 1.0) if kind == "plain" else (2.0e5, 1.0e5)
+    xf = spacing_uniform(NX + 1, LX)
+    xp, xcp = pad_periodic(xf, LX)
+    y = spacing_tanh_band(NY, LY) if graded_y else spacing_uniform(NY, LY)
+    yp, ycp = pad_grid(y, LY)
+    rng = Rng(seed, 1)
+    X, Y = _markers_in_cells(rng, xf, y, NM, NM)
+    n = len(X)
+    edges = [(0.0, 0.3), (LX, 0.5), (LX, 0.0), (0.0, 1.0), (LX * (1 - 1e-16), 0.7),
+             (xf[NX - 1], 0.2), (xcp[NX + 2], 0.6), (xcp[3], 0.4), (-1e-9 * LX, 0.5), (LX * (1 + 1e-12), 0.5)]
+    for k, (ex, ey) in enumerate(edges):
+        X[(37 * k + 5) % n], Y[(37 * k + 5) % n] = ex, ey * LY
+    mk = _new_markers(n)
+    mk["X"], mk["Y"] = X, Y
+    r2 = Rng(seed, 2)
+    bv = _bc_free_slip()
+    for side in ("mechBCLeft", "mechBCRight"):
+        getattr(bv, side).type[0] = 2  # periodic (gridGenerator.c:67)
+    opt = abi.default_options()
+    opt.NX, opt.NY, opt.LX, opt.LY, opt.NMX, opt.NMY = NX, NY, LX, LY, NM, NM
+    opt.mechBCLeft.type[0] = 2
+    opt.mechBCRight.type[0] = 2
+    Xn, Yn = np.meshgrid(xf[:NX], y)
+    hx = LX / NX
+    if kind == "plain":
+        mk["Mat"] = 0
+        mk["T"] = 1.0 - Y / LY + 0.05 * np.cos(2 * np.pi * X / LX) * np.sin(np.pi * Y / LY)
+        mk["eta"] = 1.0
+        mk["mu"] = 1e99
+        mat = abi.default_materials()
+        mat.materialEta[0] = mat.materialRho[0] = mat.materialkThermal[0] = mat.materialCp[0] = 1.0
+        mat.materialAlpha[0] = 1e-3
+        mat.materialMu[0] = 1e99
+        U = 1.0
+    else:
+        matid = np.clip((Y / LY * 4).astype(int), 0, 3)
+        matid = np.where(np.abs(Y - 0.5 * LY - 0.1 * LY * np.sin(2 * np.pi * X / LX)) < 0.08 * LY, 4, matid)
+        mk["Mat"] = matid
+        mk["T"] = 273.0 + 1300.0 * Y / LY * (1.0 + 0.01 * r2.normal(n))
+        mat = abi.default_materials()
+        mat.nMaterials = 5
+        etas = [1e23, 1e22, 1e21, 1e20, 1e24]
+        for i in range(5):
+            mat.materialEta[i] = etas[i]
+            mat.materialRho[i] = 3300.0 + 20.0 * i
+            mat.materialkThermal[i] = 3.0
+            mat.materialCp[i] = 1000.0 + 50.0 * i
+            mat.materialAlpha[i] = 3e-5
+            mat.materialMu[i] = 1e10
+            mat.materialCohesion[i] = 1e7 * (1 + i)
+            mat.materialFriction[i] = 0.3 + 0.05 * i
+            mat.hasPlasticity[i] = 1
+            mat.hasEtaT[i] = 1 if i < 4 else 0
+            mat.QonR[i] = 2.0e4
+            mat.Tref[i] = 1573.0
+        opt.etamin, opt.etamax, opt.fractionalEtamin = 1e18, 1e25, 1e-4
+        opt.subgridStressDiffusivity = 1.0
+        opt.subgridTemperatureDiffusivity = 1.0
+        r4 = Rng(seed, 4)
+        mk["eta"] = np.array(etas)[matid] * np.exp(0.1 * r4.normal(n))
+        mk["mu"] = 1e10
+        mk["sxx"] = 1e7 * r4.normal(n)
+        mk["syy"] = -mk["sxx"] + 1e6 * r4.normal(n)
+        mk["sxy"] = 5e6 * r4.normal(n)
+        mk["exx"] = 1e-15 * r4.normal(n)
+        mk["eyy"] = -mk["exx"]
+        mk["exy"] = 1e-15 * r4.normal(n)
+        mk["p"] = 3300.0 * 9.8 * Y * (1.0 + 0.05 * r4.normal(n))
+        mk["Eii"] = np.abs(r4.uniform(n, 0.0, 1.5))
+        U = 1e-9
+    w = Workload("periodic_" + kind, NX, NY, LX, LY, xp, yp, xcp, ycp, mk, {}, mat, opt, bv, 0.0,
+                 "plain" if kind == "plain" else "vep", seed=seed, xperiodic=1)
+    # staggered velocity: mean drift U along x plus a periodic perturbation
+    r5 = Rng(seed, 5)
+    Xv, Yv = np.meshgrid(xf[:NX], w.yc[1:NY + 1])          # vx nodes (x[ix], yc[jy+1])
+    Xw, Yw = np.meshgrid(w.xc[1:NX + 1], y)                # vy nodes (xc[ix+1], y[jy])
+    vx = U * (1.0 + 0.5 * _periodic_fourier(r5, Xv, Yv, LX, LY))
+    vy = U * 0.5 * _periodic_fourier(r5, Xw, Yw, LX, LY)
+    vx[NY - 1, :] = 0.0  # ghost row of the vx nodes
+    vy[:, :] = np.where(np.arange(NY)[:, None] == NY - 1, 0.0, vy)
+    w.dt = 0.5 * hx / (U * 2.0)
+    r6 = Rng(seed, 6)
+    nod = {"vx": vx, "vy": vy, "wxy": U / LY * _periodic_fourier(r6, Xn, Yn, LX, LY)}
+    if kind == "plain":
+        nod["thisT"] = 1.0 - Yn / LY + 0.05 * np.cos(2 * np.pi * Xn / LX) * np.sin(np.pi * Yn / LY) + \
+            1e-3 * _periodic_fourier(r6, Xn, Yn, LX, LY)
+    else:
+        Xc, Yc = np.meshgrid(w.xc[0:NX], w.yc[0:NY])
+        for name, scale, (XX, YY) in (("edotxx", 1e-15, (Xc, Yc)), ("edotyy", 1e-15, (Xc, Yc)),
+                                      ("edotxy", 1e-15, (Xn, Yn)), ("soxx", 1e7, (Xc, Yc)),
+                                      ("soyy", 1e7, (Xc, Yc)), ("soxy", 1e7, (Xn, Yn)),
+                                      ("dsxx", 1e6, (Xc, Yc)), ("dsyy", 1e6, (Xc, Yc)),
+                                      ("dsxy", 1e6, (Xn, Yn))):
+            nod[name] = scale * _periodic_fourier(r6, XX, YY, LX, LY)
+        nod["p"] = 3300.0 * 9.8 * Yc * (1.0 + 0.01 * _periodic_fourier(r6, Xc, Yc, LX, LY))
+        nod["thisT"] = 273.0 + 1300.0 * Yn / LY + 5.0 * _periodic_fourier(r6, Xn, Yn, LX, LY)
+    w.nodal = nod
+    w.projections = "all2"
+    return w
+
+
 WORKLOADS = {
     "convection_small": convection_small,
     "convection_tdep": convection_tdep,
     "vep_subduction": vep_subduction,
+    "periodic_box": periodic_box,
 }
 
 

@@ -16,8 +16,18 @@
  * Semantics kept from the reference's PETSc layer: scatters to node indices
  * outside the grid are dropped (DMLocalToGlobal ADD on DM_BOUNDARY_GHOSTED),
  * gathers outside the grid read 0, normalisation is IEEE division (nodes with
- * zero weight become NaN).  Only non-periodic grids are supported
- * (xperiodic != 0 returns FVM_ERR_PERIODIC).
+ * zero weight become NaN).
+ *
+ * Periodic x (xperiodic != 0, DM_BOUNDARY_PERIODIC, nodalFields.c:11-20): the
+ * grid has NX cells along x, the last closed by x[NX] = LX, and the ghost
+ * coordinates are those makePeriodic sets (gridSpacing.c:159-168).  Node column
+ * ix outside [0, NX-1] stands for ix mod NX in scatters and gathers, and the
+ * marker functions follow the reference's xperiodic branches (markers.c:49,73,
+ * 1908-1915, 1965-1971, 2014-2023, 2225-2231; updateMarkerStress.c:66,76,110;
+ * updateMarkerStrainPressure.c:137,146,211).  It needs NX >= 4.  The nodal
+ * functions (nodal stress/strain, solution retrieval, displacement time step,
+ * thermal system, initial pressure guess), marker population control and the
+ * batched ensemble return FVM_ERR_PERIODIC for periodic grids.
  *
  * Projections are deterministic: markers are bucketed by (basic cell, quadrant)
  * into a sort order that is a pure function of the marker positions, and every
@@ -38,7 +48,7 @@ extern "C" {
 /* ---- status codes --------------------------------------------------------- */
 enum {
   FVM_OK = 0,
-  FVM_ERR_PERIODIC = 1,          /* xperiodic grids are not supported */
+  FVM_ERR_PERIODIC = 1,          /* a periodic grid where it is not supported, or with NX < 4 */
   FVM_ERR_CELL_OUT_OF_BOUNDS = 2,/* markers.c:74-77 (reference aborts) */
   FVM_ERR_BAD_STAG = 3,          /* markerProjection.c:137-153 (reference aborts) */
   FVM_ERR_BAD_METHOD = 4,        /* markerProjection.c:224-225, 241-243 (aborts) */

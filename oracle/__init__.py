@@ -84,6 +84,7 @@ class State:
         s = 8
         self.grid = abi.GridData()
         self.grid.LX, self.grid.LY, self.grid.NX, self.grid.NY = w.LX, w.LY, w.NX, w.NY
+        self.grid.xperiodic = int(getattr(w, "xperiodic", 0))
         self.grid.x = self.xp.ctypes.data + 2 * s
         self.grid.y = self.yp.ctypes.data + 2 * s
         self.grid.xc = self.xcp.ctypes.data + 2 * s

@@ -12,15 +12,18 @@
 
 #include "oracle.h"
 
-/* ---- ghosted local arrays (DMDA stencil width 2, nodalFields.c:19) ------- */
+/* ---- ghosted local arrays (DMDA stencil width 2, nodalFields.c:11-20) ----
+ * x is DM_BOUNDARY_PERIODIC when grid->xperiodic, else DM_BOUNDARY_GHOSTED;
+ * y is always DM_BOUNDARY_GHOSTED. */
 typedef struct {
-  int NX, NY;
+  int NX, NY, per;
   double *a; /* (NY+4) x (NX+4), zero-initialised */
 } Local;
 
-static int local_new(Local *l, int NX, int NY) {
+static int local_new(Local *l, int NX, int NY, int per) {
   l->NX = NX;
   l->NY = NY;
+  l->per = per != 0;
   l->a = (double *)calloc((size_t)(NX + 4) * (NY + 4), sizeof(double));
   return l->a ? ORC_OK : ORC_ERR_NOMEM;
 }
@@ -28,18 +31,35 @@ static void local_free(Local *l) { free(l->a); l->a = NULL; }
 static double *local_at(Local *l, int jy, int ix) {
   return &l->a[(size_t)(jy + 2) * (l->NX + 4) + (ix + 2)];
 }
-/* DMGlobalToLocal: interior from the global vector, ghosts stay zero. */
+/* DMGlobalToLocal: interior from the global vector; on a periodic x axis the
+ * two ghost columns on each side hold the wrapped columns (ix -> ix mod NX),
+ * every other ghost stays zero. */
 static void local_fill(Local *l, const double *global) {
-  for (int jy = 0; jy < l->NY; jy++)
-    for (int ix = 0; ix < l->NX; ix++) *local_at(l, jy, ix) = global[(size_t)jy * l->NX + ix];
+  const int NX = l->NX;
+  for (int jy = 0; jy < l->NY; jy++) {
+    for (int ix = 0; ix < NX; ix++) *local_at(l, jy, ix) = global[(size_t)jy * NX + ix];
+    if (l->per)
+      for (int ix = -2; ix < NX + 2; ix += (ix == -1 ? NX + 1 : 1))
+        *local_at(l, jy, ix) = global[(size_t)jy * NX + (ix + NX) % NX];
+  }
 }
-/* DMLocalToGlobal ADD_VALUES on DM_BOUNDARY_GHOSTED: ghosts are dropped. */
+/* DMLocalToGlobal ADD_VALUES: ghosts of a ghosted axis are dropped; on a
+ * periodic x axis a ghost column adds into the node it wraps to.  Every
+ * wrapped node then receives exactly two addends (NX >= 4), so the sum is
+ * independent of the order in which PETSc's scatter applies them. */
 static void local_add_to_global(Local *l, double *global) {
-  for (int jy = 0; jy < l->NY; jy++)
-    for (int ix = 0; ix < l->NX; ix++) {
-      size_t i = (size_t)jy * l->NX + ix;
+  const int NX = l->NX;
+  for (int jy = 0; jy < l->NY; jy++) {
+    for (int ix = 0; ix < NX; ix++) {
+      size_t i = (size_t)jy * NX + ix;
       global[i] = global[i] + *local_at(l, jy, ix);
     }
+    if (l->per)
+      for (int ix = -2; ix < NX + 2; ix += (ix == -1 ? NX + 1 : 1)) {
+        size_t i = (size_t)jy * NX + (ix + NX) % NX;
+        global[i] = global[i] + *local_at(l, jy, ix);
+      }
+  }
 }
 #define LA(l, jy, ix) (*local_at(&(l), (jy), (ix)))
 
@@ -54,7 +74,8 @@ static void set_field_of(Marker *mk, size_t off, double v) { memcpy((char *)mk +
 static int find_marker_cell(Marker *mk, const GridData *g) {
   const int NX = g->NX, NY = g->NY;
   if (mk->X >= 0.0 && mk->Y >= 0.0 && mk->X <= g->LX && mk->Y <= g->LY) {
-    int lo = 0, hi = NX - 1;
+    /* a periodic x axis has NX cells: x[NX] = LX closes the last one */
+    int lo = 0, hi = g->xperiodic ? NX : NX - 1;
     while (hi - lo > 1) {
       int mid = (hi + lo) / 2;
       if (mid == lo) mid++;
@@ -73,7 +94,7 @@ static int find_marker_cell(Marker *mk, const GridData *g) {
       else hi = mid;
     }
     mk->cellY = lo;
-    if (mk->cellX < 0 || mk->cellX > NX - 2 || mk->cellY < 0 || mk->cellY > NY - 2)
+    if (mk->cellX < 0 || mk->cellX > (g->xperiodic ? NX - 1 : NX - 2) || mk->cellY < 0 || mk->cellY > NY - 2)
       return ORC_ERR_CELL_OUT_OF_BOUNDS; /* markers.c:74-77 aborts */
   } else {
     mk->cellX = -1;
@@ -83,7 +104,7 @@ static int find_marker_cell(Marker *mk, const GridData *g) {
 }
 
 int orc_find_marker_cells(MarkerSet *ms, const GridData *g, int rank) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   int nout = 0, err = ORC_OK;
   for (int m = 0; m < ms->nMark; m++) {
     int e = find_marker_cell(&ms->markers[m], g);
@@ -116,14 +137,14 @@ static void find_cell_stag(const Marker *mk, const GridData *g, int stagx, int s
 int orc_project_marker_field(const GridData *g, const MarkerSet *ms, const double *materialProperty,
                              size_t field_offset, double *nodalField, int stagx, int stagy,
                              int method) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   if (stagx < -1 || stagx > 1 || stagy < -1 || stagy > 1) return ORC_ERR_BAD_STAG;
   if (method != ARITHMETIC && method != ARITHMETIC_LOCAL && method != HARMONIC) return ORC_ERR_BAD_METHOD;
   const int NX = g->NX, NY = g->NY;
   const size_t nn = (size_t)NX * NY;
   double *wtg = (double *)calloc(nn, sizeof(double));
   Local wt, nf;
-  if (!wtg || local_new(&wt, NX, NY) || local_new(&nf, NX, NY)) return ORC_ERR_NOMEM;
+  if (!wtg || local_new(&wt, NX, NY, g->xperiodic) || local_new(&nf, NX, NY, g->xperiodic)) return ORC_ERR_NOMEM;
   memset(nodalField, 0, nn * sizeof(double));
   const Marker *mk = ms->markers;
   double *mf = (double *)malloc((size_t)(ms->nMark > 0 ? ms->nMark : 1) * sizeof(double));
@@ -217,12 +238,12 @@ int orc_project_markers_nodes_all2(const MarkerSet *ms, const GridData *g, Nodal
 /* ---- markers.c:1507-1594 ------------------------------------------------ */
 int orc_project_markers_nodes_from_scalar(const MarkerSet *ms, const GridData *g,
                                           const double *markerField, double *nodalField) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   const int NX = g->NX, NY = g->NY;
   const size_t nn = (size_t)NX * NY;
   double *wtng = (double *)calloc(nn, sizeof(double));
   Local wtn, nf;
-  if (!wtng || local_new(&wtn, NX, NY) || local_new(&nf, NX, NY)) return ORC_ERR_NOMEM;
+  if (!wtng || local_new(&wtn, NX, NY, g->xperiodic) || local_new(&nf, NX, NY, g->xperiodic)) return ORC_ERR_NOMEM;
   memset(nodalField, 0, nn * sizeof(double));
   const Marker *mk = ms->markers;
   for (int m = 0; m < ms->nMark; m++) {
@@ -257,9 +278,9 @@ int orc_project_markers_nodes_from_scalar(const MarkerSet *ms, const GridData *g
 /* ---- markers.c:1670-1706 ------------------------------------------------ */
 int orc_project_nodal_field_to_markers(const double *nodalField, MarkerSet *ms, size_t field_offset,
                                        const GridData *g) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   Local f;
-  if (local_new(&f, g->NX, g->NY)) return ORC_ERR_NOMEM;
+  if (local_new(&f, g->NX, g->NY, g->xperiodic)) return ORC_ERR_NOMEM;
   local_fill(&f, nodalField);
   Marker *mk = ms->markers;
   for (int m = 0; m < ms->nMark; m++) {
@@ -278,14 +299,16 @@ int orc_project_nodal_field_to_markers(const double *nodalField, MarkerSet *ms, 
 /* ---- markers.c:1865-2250 ------------------------------------------------ */
 int orc_advect_markers_rk(MarkerSet *ms, const NodalArrays *nfa, const GridData *g,
                           const BoundaryValues *bv, double dt, int jaumann) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   const int NX = g->NX, NY = g->NY;
   Local vx, vy, wxy;
-  if (local_new(&vx, NX, NY) || local_new(&vy, NX, NY) || local_new(&wxy, NX, NY)) return ORC_ERR_NOMEM;
+  if (local_new(&vx, NX, NY, g->xperiodic) || local_new(&vy, NX, NY, g->xperiodic) || local_new(&wxy, NX, NY, g->xperiodic)) return ORC_ERR_NOMEM;
   local_fill(&vx, nfa->vx);
   local_fill(&vy, nfa->vy);
   local_fill(&wxy, nfa->wxy);
-  const int XCMAX = NX - 2, XCMIN = 0;
+  const int per = g->xperiodic != 0;
+  /* markers.c:1908-1915: a periodic walk may leave [0, NX-1] by one cell */
+  const int XCMAX = per ? NX - 1 : NX - 2, XCMIN = per ? -1 : 0;
   Marker *mk = ms->markers;
   for (int m = 0; m < ms->nMark; m++) {
     if (mk[m].cellX == -1) continue;
@@ -308,10 +331,12 @@ int orc_advect_markers_rk(MarkerSet *ms, const NodalArrays *nfa, const GridData 
       int vyX = cellX;
       if (vyX > -1 && xcur < g->xc[vyX + 1]) vyX--;
       int vyY = cellY;
-      if (vxX < 0) vxX = 0;
-      else if (vxX > NX - 2) vxX = NX - 2;
-      if (vyX < -1) vyX = -1;
-      else if (vyX > NX - 2) vyX = NX - 2;
+      if (!per) { /* markers.c:1965-1971: no x clamp on a periodic grid */
+        if (vxX < 0) vxX = 0;
+        else if (vxX > NX - 2) vxX = NX - 2;
+        if (vyX < -1) vyX = -1;
+        else if (vyX > NX - 2) vyX = NX - 2;
+      }
       if (vxY < -1) vxY = -1;
       else if (vxY > NY - 2) vxY = NY - 2;
       if (vyY < 0) vyY = 0;
@@ -343,10 +368,10 @@ int orc_advect_markers_rk(MarkerSet *ms, const NodalArrays *nfa, const GridData 
       const double vyur = LA(vy, vyY, vyX + 1);
       double vyll;
       const double vylr = LA(vy, vyY + 1, vyX + 1);
-      if (vyX == -1 && bv->mechBCLeft.type[1] == 0) {
+      if (!per && vyX == -1 && bv->mechBCLeft.type[1] == 0) {
         vyul = 2.0 * bv->mechBCLeft.value[1] - LA(vy, vyY, vyX + 1);
         vyll = 2.0 * bv->mechBCLeft.value[1] - LA(vy, vyY + 1, vyX + 1);
-      } else if (vyX == -1 && bv->mechBCLeft.type[1] == 1) {
+      } else if (!per && vyX == -1 && bv->mechBCLeft.type[1] == 1) {
         vyul = LA(vy, vyY, vyX + 1);
         vyll = LA(vy, vyY + 1, vyX + 1);
       } else {
@@ -398,6 +423,10 @@ int orc_advect_markers_rk(MarkerSet *ms, const NodalArrays *nfa, const GridData 
       mk[m].s.T12 = 0.5 * (msxx0 - msyy0) * sin(2.0 * thetaz) + msxy0 * cos(2.0 * thetaz);
     }
     mk[m].Eii += dt * sqrt(0.5 * (mk[m].e.T11 * mk[m].e.T11 + mk[m].e.T22 * mk[m].e.T22) + mk[m].e.T12 * mk[m].e.T12);
+    if (per) { /* markers.c:2225-2231 */
+      if (mk[m].X > g->LX) mk[m].X -= g->LX;
+      else if (mk[m].X < 0) mk[m].X += g->LX;
+    }
   }
   local_free(&vx);
   local_free(&vy);
@@ -408,11 +437,11 @@ int orc_advect_markers_rk(MarkerSet *ms, const NodalArrays *nfa, const GridData 
 /* ---- subgridTemperatureChanges.c:5-124 --------------------------------- */
 int orc_subgrid_temperature_changes(const double *thisT, const GridData *g, const NodalArrays *nfa,
                                     MarkerSet *ms, const Materials *mat, double dt, double dsubgrid) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   const int NX = g->NX, NY = g->NY;
   const size_t nn = (size_t)NX * NY;
   Local T, rho, Cp, kT;
-  if (local_new(&T, NX, NY) || local_new(&rho, NX, NY) || local_new(&Cp, NX, NY) || local_new(&kT, NX, NY))
+  if (local_new(&T, NX, NY, g->xperiodic) || local_new(&rho, NX, NY, g->xperiodic) || local_new(&Cp, NX, NY, g->xperiodic) || local_new(&kT, NX, NY, g->xperiodic))
     return ORC_ERR_NOMEM;
   local_fill(&T, nfa->lastT);
   local_fill(&rho, nfa->rho);
@@ -447,7 +476,7 @@ int orc_subgrid_temperature_changes(const double *thisT, const GridData *g, cons
   /* VecAXPBYPCZ(deltaT, 1, -1, -1, thisT, lastT): z = x + (-1)y + (-1)z */
   for (size_t i = 0; i < nn; i++) deltaT[i] = thisT[i] + -1.0 * nfa->lastT[i] + -1.0 * deltaT[i];
   Local dT;
-  if (local_new(&dT, NX, NY)) return ORC_ERR_NOMEM;
+  if (local_new(&dT, NX, NY, g->xperiodic)) return ORC_ERR_NOMEM;
   local_fill(&dT, deltaT);
   for (int m = 0; m < ms->nMark; m++) {
     if (mk[m].cellX == -1) continue;
@@ -476,13 +505,13 @@ int orc_subgrid_temperature_changes(const double *thisT, const GridData *g, cons
 int orc_subgrid_stress_changes(const GridData *g, NodalArrays *nfa, MarkerSet *ms, const Materials *mat,
                                double dt, double dsubgrid) {
   (void)mat;
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   const int NX = g->NX, NY = g->NY;
   const size_t nn = (size_t)NX * NY;
   Local soxx, soyy, soxy, dsxxn, dsyyn, dsxyn, wtetan, wtetas;
-  if (local_new(&soxx, NX, NY) || local_new(&soyy, NX, NY) || local_new(&soxy, NX, NY) ||
-      local_new(&dsxxn, NX, NY) || local_new(&dsyyn, NX, NY) || local_new(&dsxyn, NX, NY) ||
-      local_new(&wtetan, NX, NY) || local_new(&wtetas, NX, NY))
+  if (local_new(&soxx, NX, NY, g->xperiodic) || local_new(&soyy, NX, NY, g->xperiodic) || local_new(&soxy, NX, NY, g->xperiodic) ||
+      local_new(&dsxxn, NX, NY, g->xperiodic) || local_new(&dsyyn, NX, NY, g->xperiodic) || local_new(&dsxyn, NX, NY, g->xperiodic) ||
+      local_new(&wtetan, NX, NY, g->xperiodic) || local_new(&wtetas, NX, NY, g->xperiodic))
     return ORC_ERR_NOMEM;
   local_fill(&soxx, nfa->soxx);
   local_fill(&soyy, nfa->soyy);
@@ -602,10 +631,10 @@ int orc_subgrid_stress_changes(const GridData *g, NodalArrays *nfa, MarkerSet *m
 
 /* ---- updateMarkerStress.c:5-163 ---------------------------------------- */
 int orc_update_marker_stress(const GridData *g, const NodalArrays *nfa, MarkerSet *ms) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   const int NX = g->NX, NY = g->NY;
   Local dsxx, dsyy, dsxy;
-  if (local_new(&dsxx, NX, NY) || local_new(&dsyy, NX, NY) || local_new(&dsxy, NX, NY)) return ORC_ERR_NOMEM;
+  if (local_new(&dsxx, NX, NY, g->xperiodic) || local_new(&dsyy, NX, NY, g->xperiodic) || local_new(&dsxy, NX, NY, g->xperiodic)) return ORC_ERR_NOMEM;
   local_fill(&dsxx, nfa->dsxx);
   local_fill(&dsyy, nfa->dsyy);
   local_fill(&dsxy, nfa->dsxy);
@@ -613,12 +642,12 @@ int orc_update_marker_stress(const GridData *g, const NodalArrays *nfa, MarkerSe
   for (int m = 0; m < ms->nMark; m++) {
     if (mk[m].cellX == -1) continue;
     int cellX = mk[m].cellX, cellY = mk[m].cellY;
-    int cellXMax = NX - 2;
+    int cellXMax = g->xperiodic ? NX : NX - 2; /* updateMarkerStress.c:66 */
     if (cellX < cellXMax && mk[m].X >= g->xc[cellX + 1]) cellX++;
     if (cellY < NY - 1 && mk[m].Y >= g->yc[cellY + 1]) cellY++;
     if (cellY < 1) cellY = 1;
     else if (cellY > NY - 2) cellY = NY - 2;
-    cellXMax = NX - 2;
+    cellXMax = g->xperiodic ? NX - 1 : NX - 2;
     if (cellX < 1) cellX = 1;
     else if (cellX > cellXMax) cellX = cellXMax;
     double mdx = (mk[m].X - g->xc[cellX]) / (g->xc[cellX + 1] - g->xc[cellX]);
@@ -637,7 +666,7 @@ int orc_update_marker_stress(const GridData *g, const NodalArrays *nfa, MarkerSe
     mk[m].s.T22 += mdsyy;
     cellX = mk[m].cellX;
     cellY = mk[m].cellY;
-    cellXMax = NX - 2;
+    cellXMax = g->xperiodic ? NX - 1 : NX - 2;
     if (cellY < 0) cellY = 0;
     else if (cellY > NY - 2) cellY = NY - 2;
     if (cellX < 0) cellX = 0;
@@ -668,13 +697,13 @@ static double bil_c(Local *f, int cellY, int cellX, double mdx, double mdy) {
 }
 
 int orc_update_marker_strain_pressure(const GridData *g, const NodalArrays *nfa, MarkerSet *ms, double dt) {
-  if (g->xperiodic) return ORC_ERR_PERIODIC;
+  if (g->xperiodic && g->NX < 4) return ORC_ERR_PERIODIC;
   const int NX = g->NX, NY = g->NY;
   Local dsxx, dsyy, dsxy, soxx, soyy, soxy, exx, eyy, exy, wxy, p;
-  if (local_new(&dsxx, NX, NY) || local_new(&dsyy, NX, NY) || local_new(&dsxy, NX, NY) ||
-      local_new(&soxx, NX, NY) || local_new(&soyy, NX, NY) || local_new(&soxy, NX, NY) ||
-      local_new(&exx, NX, NY) || local_new(&eyy, NX, NY) || local_new(&exy, NX, NY) ||
-      local_new(&wxy, NX, NY) || local_new(&p, NX, NY))
+  if (local_new(&dsxx, NX, NY, g->xperiodic) || local_new(&dsyy, NX, NY, g->xperiodic) || local_new(&dsxy, NX, NY, g->xperiodic) ||
+      local_new(&soxx, NX, NY, g->xperiodic) || local_new(&soyy, NX, NY, g->xperiodic) || local_new(&soxy, NX, NY, g->xperiodic) ||
+      local_new(&exx, NX, NY, g->xperiodic) || local_new(&eyy, NX, NY, g->xperiodic) || local_new(&exy, NX, NY, g->xperiodic) ||
+      local_new(&wxy, NX, NY, g->xperiodic) || local_new(&p, NX, NY, g->xperiodic))
     return ORC_ERR_NOMEM;
   local_fill(&dsxx, nfa->dsxx);
   local_fill(&dsyy, nfa->dsyy);
@@ -692,12 +721,12 @@ int orc_update_marker_strain_pressure(const GridData *g, const NodalArrays *nfa,
   for (int m = 0; m < ms->nMark; m++) {
     if (mk[m].cellX == -1) continue;
     int cellX = mk[m].cellX, cellY = mk[m].cellY;
-    int cellXMax = NX - 2;
+    int cellXMax = g->xperiodic ? NX : NX - 2; /* updateMarkerStrainPressure.c:137 */
     if (cellX < cellXMax && mk[m].X >= g->xc[cellX + 1]) cellX++;
     if (cellY < NY - 1 && mk[m].Y >= g->yc[cellY + 1]) cellY++;
     if (cellY < 1) cellY = 1;
     else if (cellY > NY - 2) cellY = NY - 2;
-    cellXMax = NX - 2;
+    cellXMax = g->xperiodic ? NX - 1 : NX - 2;
     if (cellX < 1) cellX = 1;
     else if (cellX > cellXMax) cellX = cellXMax;
     double mdx = (mk[m].X - g->xc[cellX]) / (g->xc[cellX + 1] - g->xc[cellX]);
@@ -712,7 +741,7 @@ int orc_update_marker_strain_pressure(const GridData *g, const NodalArrays *nfa,
 
     cellX = mk[m].cellX;
     cellY = mk[m].cellY;
-    cellXMax = NX - 2;
+    cellXMax = g->xperiodic ? NX - 1 : NX - 2;
     if (cellY < 0) cellY = 0;
     else if (cellY > NY - 2) cellY = NY - 2;
     if (cellX < 0) cellX = 0;

@@ -10,9 +10,14 @@
  *     to node indices outside [0,NX-1]x[0,NY-1] are dropped;
  *   - gather from a local vector outside the domain reads 0.0 (fresh, zeroed
  *     local vector);
+ *   - periodic x (grid->xperiodic, DM_BOUNDARY_PERIODIC, nodalFields.c:11-20):
+ *     x node index ix in [-2, NX+1] stands for ix mod NX, both ways (gathers
+ *     read the wrapped column, scatters add into it); y stays ghosted;
  *   - VecPointwiseDivide is IEEE division (0/0 = NaN at zero-weight nodes).
- * Only the non-periodic grid (xperiodic == 0) is restated; periodic input
- * returns ORC_ERR_PERIODIC.
+ * The marker functions take periodic grids (NX >= 4, ghost coordinates as
+ * makePeriodic, gridSpacing.c:159-168, sets them); the nodal functions
+ * (nodal stress/strain, solution retrieval, displacement time step, marker
+ * population control) return ORC_ERR_PERIODIC for them.
  */
 #ifndef FVM_ORACLE_H
 #define FVM_ORACLE_H
