@@ -37,7 +37,7 @@ METRIC = "markers/s per MIC step (proj+advect+subgrid); KSP iters to 1e-8 vs pla
 HBM_PEAK_GBS = 8000.0
 
 PHASES = ["locate", "sort", "partials", "nodes", "subgrid_t1", "subgrid_t2", "advect", "subgrid_s1",
-          "marker_update", "other"]
+          "marker_update", "other", "persistent"]
 
 
 def log(msg):
@@ -241,18 +241,34 @@ def run_ensemble(a):
     log("ensemble: sequential %.3f ms/step" % seq_ms)
 
     ens = Ensemble(paths)
-    (block, lanes), tuned_ms = ens.autotune(reps=3)
-    log("ensemble: autotuned block=%d lanes=%d (%.3f ms)" % (block, lanes, tuned_ms))
-    for _ in range(a.warmup):
-        ens.launch()
-    ctx.sync()
-    ctx.timer_start()
-    for _ in range(a.steps):
-        ens.launch()
-    ens_ms = ctx.timer_stop() / a.steps
+    (block, lanes, persistent), tuned_ms = ens.autotune(reps=3)
+    log("ensemble: autotuned block=%d lanes=%d persistent=%d (%.3f ms)" % (block, lanes, persistent, tuned_ms))
+
+    def timed(nsteps):
+        for _ in range(a.warmup):
+            ens.launch()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(nsteps):
+            ens.launch()
+        return ctx.timer_stop() / nsteps
+
+    ens_ms = timed(a.steps)
     log("ensemble: batched %.3f ms/step" % ens_ms)
     nout, nbad = ens.counters()
     active = n0 - nout
+    # both launch modes at the tuned lane count, for the record; the per-phase
+    # mode last, for the phase profile
+    modes = {}
+    ens.set_tuning(256, lanes, 1)
+    ens.capture()
+    modes["persistent_ms_per_step"] = timed(max(3, a.steps // 4))
+    ens.set_tuning(256 if persistent else block, lanes, 0)
+    ens.capture()
+    modes["per_phase_ms_per_step"] = timed(max(3, a.steps // 4))
+    ens.counters()  # raises if a persistent step stalled
+    log("ensemble: persistent %.3f ms, per-phase %.3f ms" % (modes["persistent_ms_per_step"],
+                                                              modes["per_phase_ms_per_step"]))
     phase_ms, phase_la = phase_profile(ctx, ens.step, 3)
     step_bytes = n0 * STEP_BYTES_PER_MARKER + nodes * STEP_BYTES_PER_NODE
     ab = n0 * (96 + 88) + nodes * 32
@@ -266,9 +282,10 @@ def run_ensemble(a):
                    "grids": "%dx%d..%dx%d" % (min(s[0] for s in shapes), min(s[0] for s in shapes),
                                               max(s[0] for s in shapes), max(s[0] for s in shapes)),
                    "markers": n0, "markers_per_cell": "16-36", "viscosity_contrast": "1e2-1e6",
-                   "tuning": {"block": block, "lanes": lanes}, "graph": True},
+                   "tuning": {"block": block, "lanes": lanes, "persistent": persistent}, "graph": True},
         "sequential": {"ms_per_step": seq_ms, "markers_per_s": active / (seq_ms / 1e3)},
         "speedup_vs_sequential": seq_ms / ens_ms,
+        "launch_modes": modes,
         "roofline": roofline("advect (kb_advect: subgrid T pass 2, RK4, stress rotation)", "kb_advect", ab,
                              adv_ms),
         "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ens_ms / 1e3) / 1e9,

@@ -16,6 +16,7 @@
 
 #include "marker_common.h"
 #include "proj_common.h"
+#include "scan_common.h"
 
 int fvm_fill(fvm_ctx *ctx, double *a, int64_t n, double v);
 
@@ -37,9 +38,54 @@ struct InstDev {
   int32_t jaumann;
 };
 
+// Groups of one (method, field count) share a launch: launch row blockIdx.y
+// is group gi[blockIdx.y]; a row's blocks beyond that group's count exit.
+#define KB_MAX_MULTI 8
+struct KbGroups {
+  int gi[KB_MAX_MULTI], sx[KB_MAX_MULTI], sy[KB_MAX_MULTI];
+};
+
+enum { PS_ZERO, PS_LOCATE, PS_SCAN, PS_SCATTER, PS_ORDER, PS_PART, PS_NODES, PS_SUBT, PS_NODES2, PS_ADVECT, PS_N };
+#ifndef PS_CHUNK
+#define PS_CHUNK 32
+#endif
+#define PS_SPIN_MAX (1 << 22)
+#define PS_PEND 64
+#define PS_SAT 16
+#define PS_ZERO_ITEM 4096
+
+struct PersistArgs {
+  const InstDev *insts;
+  int n, ng, nrow, ncol;
+  int64_t off[PS_N + 1];  // first item of each phase; off[PS_N] = total
+  // rows: markers (256 per block), nodes, dTsub lattice, universal lattice
+  const int64_t *mblk, *nblk, *sblk, *ublk;
+  const int32_t *mbi, *nbi, *sbi, *ubi;
+  const int64_t *cell_off, *tile_off;
+  const int32_t *tile_inst;
+  const int32_t *scan_upto;           // [scan tile]: its counts are complete when instances 0..upto are located
+  const int32_t *scan_lo, *scan_hi;   // [scan tile]: instances whose scan the tile completes
+  int32_t *done;                      // [PS_N][n] items credited
+  const int32_t *need;                // [PS_N][n] items per (phase, instance)
+  unsigned long long *ticket;         // [0] next chunk, [1] workgroups finished
+  const int64_t *chunk;               // [nchunk + 1] first item of each chunk
+  int64_t nchunk;
+  int32_t *key, *slot, *perm, *count, *start, *counters, *scan_fail;
+  unsigned long long *scan_state;
+  int64_t nbucket, nstate, nmark;
+  double *xs, *ys, *pcols, *part, *dTsub, *deltaT;
+  const double *mt;  // payload column of marker T
+  int32_t *ms;
+  const double *const *colsrc;
+  const ProjGroup *groups_sorted, *groups;
+  const int64_t *partoff;
+  const NodeGroup *nodes;
+  KbGroups kg;
+};
+
 struct fvm_ensemble {
   int n = 0, ng = 0;
-  fvm_tuning tune = {256, 8};
+  fvm_tuning tune = {256, 8, 0};
   std::vector<fvm_instance> inst;
   std::vector<InstDev> hinst;
   std::vector<ProjGroup> hgroups;   // [group][instance]; group ng is the dTsub projection
@@ -75,6 +121,18 @@ struct fvm_ensemble {
   size_t lds_locate = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // persistent step (tune.persistent): work-item schedule and completion state
+  int persist_nf = 0;                  // stored fields of the single partials kind, 0: not possible
+  PersistArgs pa;                      // launch arguments, rebuilt with the block rows
+  PersistArgs *d_pa = nullptr;         // their device copy (the kernel reads them through a pointer)
+  int persist_grid = 0;                // resident workgroups
+  int32_t *d_done = nullptr, *d_need = nullptr, *d_scan_upto = nullptr, *d_scan_lo = nullptr,
+          *d_scan_hi = nullptr;
+  unsigned long long *d_ticket = nullptr;
+  size_t scan_cap = 0;
+  int64_t *d_chunk = nullptr;
+  size_t chunk_cap = 0;
+  std::vector<int64_t> hcell;          // [n + 1] first basic cell, then [n + 1] first 8 x 8 tile, per instance
 };
 
 // ---- block -> instance ----------------------------------------------------------------
@@ -82,23 +140,34 @@ struct fvm_ensemble {
 // a block (row table `bi`) or an ordering tile to its instance, so a block
 // finds its instance with one load.
 
-// ---- kernels ------------------------------------------------------------------------------
-__global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ insts, int n,
-                                                 const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
-                                                 int32_t *__restrict__ key,
-                                                 int32_t *__restrict__ slot, int32_t *__restrict__ count,
-                                                 int32_t *__restrict__ counters) {
-  extern __shared__ double sgrid[];
-  const int ii = bi[blockIdx.x];
-  const InstDev &I = insts[ii];
-  const DGrid &g = I.g;
+// ---- per-block bodies ---------------------------------------------------------------------
+// Each body does the work of one block of one phase for instance I; `lb` is
+// the block's index within that instance's range.  The per-phase kernels map
+// blockIdx.x to (instance, lb); the persistent step (kb_persist) maps its work
+// items the same way, so both run the same code.
+
+// x[0..NX-1], y[0..NY-1], xc[0..NX], yc[0..NY] of the instance in LDS
+__device__ __forceinline__ DGrid ens_stage_grid(const DGrid &g, double *sgrid) {
   for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
   for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
   for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
   for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
   __syncthreads();
-  const double *gx = sgrid, *gy = sgrid + g.NX, *gxc = sgrid + g.NX + g.NY, *gyc = sgrid + 2 * g.NX + g.NY + 1;
-  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
+  DGrid gl = g;
+  gl.x = sgrid;
+  gl.y = sgrid + g.NX;
+  gl.xc = sgrid + g.NX + g.NY;
+  gl.yc = sgrid + 2 * g.NX + g.NY + 1;
+  return gl;
+}
+
+// gl: the instance's grid as ens_stage_grid left it in LDS
+__device__ __forceinline__ void ens_locate_block(const InstDev &I, int64_t lb, const DGrid &gl, int32_t *__restrict__ key,
+                                                 int32_t *__restrict__ slot, int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ counters) {
+  const DGrid &g = gl;
+  const double *gx = g.x, *gy = g.y, *gxc = g.xc, *gyc = g.yc;
+  const int64_t m = lb * (int64_t)blockDim.x + threadIdx.x;
   int out = 0, bad = 0, k = -1;
   if (m < I.nMark) {
     const double xm = I.X[m], ym = I.Y[m];
@@ -136,17 +205,127 @@ __global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ ins
   }
 }
 
+__device__ __forceinline__ void ens_scatter_block(const InstDev &I, int64_t lb, const int32_t *__restrict__ key,
+                                                  const int32_t *__restrict__ slot, const int32_t *__restrict__ start,
+                                                  int32_t *__restrict__ perm) {
+  const int64_t m = lb * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= I.nMark) return;
+  const int k = key[I.mark_off + m];
+  if (k < 0) return;
+  perm[start[k] + slot[I.mark_off + m]] = (int32_t)m;
+}
+
+// Bucket order of basic cell (cx, cy) of instance ii by the calling wavefront,
+// with the payload gathered from the instance's columns.
+__device__ __forceinline__ void ens_order_cell(const InstDev &I, int ii, int n, int64_t c,
+                                               const int32_t *__restrict__ start, int32_t *__restrict__ perm,
+                                               double *__restrict__ xs, double *__restrict__ ys,
+                                               int32_t *__restrict__ ms, int ncol,
+                                               const double *const *__restrict__ colsrc, double *__restrict__ pcols,
+                                               int64_t nmark) {
+  const double *__restrict__ X = I.X, *__restrict__ Y = I.Y;
+  const int32_t *__restrict__ Mat = I.Mat;
+  bucket_order_cell(c, start, perm, [&](int p, int v) {
+    xs[p] = X[v];
+    ys[p] = Y[v];
+    ms[p] = Mat[v];
+    for (int k = 0; k < ncol; k++) pcols[(int64_t)k * nmark + p] = colsrc[(int64_t)k * n + ii][v];
+  });
+}
+
+// One block of the universal lattice ((NX+1) x (NY+1) cell coordinates) of
+// instance ii for projection group gi.
+template <int METHOD, int NF, int G>
+__device__ __forceinline__ void ens_partials_block(const InstDev &I, int ii, int n, int64_t lb, int gi, int sx, int sy,
+                                                   const ProjGroup *__restrict__ all_groups,
+                                                   const int64_t *__restrict__ all_partoff,
+                                                   const int32_t *__restrict__ start, const double *__restrict__ xs,
+                                                   const double *__restrict__ ys, const int32_t *__restrict__ ms,
+                                                   double *__restrict__ part) {
+  const int64_t lt = lb * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = lt / G;
+  const int j = (int)(lt % G);
+  const int64_t nT = (int64_t)(I.g.NX + 1) * (I.g.NY + 1);
+  const bool inside = t < nT;
+  const int cx = inside ? (int)(t % (I.g.NX + 1)) - 1 : -2;
+  const int cy = inside ? (int)(t / (I.g.NX + 1)) - 1 : -2;
+  const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
+  const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
+  const int64_t nL = (int64_t)Lx.n * Ly.n;
+  const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
+  partials_cell_sorted<METHOD, NF, G>(I.g, all_groups[(int64_t)gi * n + ii], start + I.bucket_off, xs, ys, ms,
+                                      part + all_partoff[(int64_t)gi * n + ii], nL, tl, live ? cx : 0,
+                                      live ? cy : 0, j, live);
+}
+
+__device__ __forceinline__ void ens_nodes_block(const InstDev &I, int64_t lb, const NodeGroup &q) {
+  const int64_t i = lb * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= (int64_t)I.g.NX * I.g.NY) return;
+  node_group(q, (int)(i % I.g.NX), (int)(i / I.g.NX), i);
+}
+
+template <int G>
+__device__ __forceinline__ void ens_subt_block(const InstDev &I, int ii, int64_t lb,
+                                               const ProjGroup *__restrict__ groups,
+                                               const int64_t *__restrict__ partoff, const int32_t *__restrict__ start,
+                                               const int32_t *__restrict__ perm, const double *__restrict__ xs,
+                                               const double *__restrict__ ys, const double *__restrict__ mt,
+                                               double *__restrict__ part, double *__restrict__ dTsub) {
+  const int nLx = I.g.NX - 1, nLy = I.g.NY - 1;
+  const int64_t nL = (int64_t)nLx * nLy;
+  const int64_t lt = lb * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t t = lt / G;
+  const int j = (int)(lt % G);
+  const bool live = t < nL;
+  const int lx = live ? (int)(t % nLx) : 0;
+  const int ly = live ? (int)(t / nLx) : 0;
+  SubT sb;
+  sb.T = I.lastT;
+  sb.rho = I.rhon;
+  sb.Cp = I.Cpn;
+  sb.k = I.kn;
+  sb.MT = mt;                     // payload column of marker T (global positions)
+  sb.dTsub = dTsub + I.mark_off;  // written at perm[p], the instance's marker index
+  sb.rho0 = I.rho0;
+  sb.dt = I.dt;
+  sb.dsub = I.dsub;
+  partials_cell_subt_sorted<G>(I.g, groups[ii], start + I.bucket_off, perm, xs, ys, part + partoff[ii], nL, t, lx,
+                               ly, j, live, sb);
+}
+
+// gl: the instance's grid staged in LDS (ens_stage_grid), as in the location pass
+__device__ __forceinline__ void ens_advect_block(const InstDev &I, int64_t lb, const DGrid &gl,
+                                                 const double *__restrict__ deltaT, const double *__restrict__ dTsub) {
+  const int64_t m = lb * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= I.nMark) return;
+  advect_marker<false>(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.cellX, I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, I.T,
+                       I.Tdot, deltaT + I.node_off, dTsub + I.mark_off, I.dt);
+  // the rotation fused behind the RK4 stages: the batched launch keeps one kernel per phase
+  rotate_eii_marker(m, I.g.ncx(), I.g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii,
+                    I.dt, I.jaumann);
+}
+
+// ---- per-phase kernels ------------------------------------------------------------------
+// Every kind of work pads each instance's range to whole blocks; a table maps
+// a block (row table `bi`) or an ordering tile to its instance, so a block
+// finds its instance with one load.
+__global__ void __launch_bounds__(512) kb_locate(const InstDev *__restrict__ insts, int n,
+                                                 const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
+                                                 int32_t *__restrict__ key,
+                                                 int32_t *__restrict__ slot, int32_t *__restrict__ count,
+                                                 int32_t *__restrict__ counters) {
+  extern __shared__ double sgrid[];
+  const int ii = bi[blockIdx.x];
+  const DGrid gl = ens_stage_grid(insts[ii].g, sgrid);
+  ens_locate_block(insts[ii], blockIdx.x - blk[ii], gl, key, slot, count, counters);
+}
+
 __global__ void __launch_bounds__(512) kb_scatter(const InstDev *__restrict__ insts, int n,
                                                   const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
                                                   const int32_t *__restrict__ key, const int32_t *__restrict__ slot,
                                                   const int32_t *__restrict__ start, int32_t *__restrict__ perm) {
   const int ii = bi[blockIdx.x];
-  const InstDev &I = insts[ii];
-  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
-  if (m >= I.nMark) return;
-  const int k = key[I.mark_off + m];
-  if (k < 0) return;
-  perm[start[k] + slot[I.mark_off + m]] = (int32_t)m;
+  ens_scatter_block(insts[ii], blockIdx.x - blk[ii], key, slot, start, perm);
 }
 
 // Bucket order of every instance's cells (one wavefront per cell) with the
@@ -177,23 +356,9 @@ __global__ void __launch_bounds__(256) kb_order_gather(const InstDev *__restrict
   const int l = (int)(w & 63);
   const int cx = (int)(tl % ntx) * 8 + (l >> 3), cy = (int)(tl / ntx) * 8 + (l & 7);
   if (cx >= ncx || cy >= ncy) return;
-  const int64_t c = cell_off[ii] + (int64_t)cy * ncx + cx;
-  const double *__restrict__ X = I.X, *__restrict__ Y = I.Y;
-  const int32_t *__restrict__ Mat = I.Mat;
-  bucket_order_cell(c, start, perm, [&](int p, int v) {
-    xs[p] = X[v];
-    ys[p] = Y[v];
-    ms[p] = Mat[v];
-    for (int k = 0; k < ncol; k++) pcols[(int64_t)k * nmark + p] = colsrc[(int64_t)k * n + ii][v];
-  });
+  ens_order_cell(I, ii, n, cell_off[ii] + (int64_t)cy * ncx + cx, start, perm, xs, ys, ms, ncol, colsrc, pcols,
+                 nmark);
 }
-
-// Groups of one (method, field count) share a launch: launch row blockIdx.y
-// is group gi[blockIdx.y]; a row's blocks beyond that group's count exit.
-#define KB_MAX_MULTI 8
-struct KbGroups {
-  int gi[KB_MAX_MULTI], sx[KB_MAX_MULTI], sy[KB_MAX_MULTI];
-};
 
 // Every group of one (method, stored field count) in one launch over the
 // universal lattice ((NX+1) x (NY+1) cell coordinates per instance, block row
@@ -212,23 +377,9 @@ __global__ void __launch_bounds__(256) kb_partials(const InstDev *__restrict__ i
   const int q = (int)(b % nrow);
   const int64_t ub = b / nrow;
   if (ub >= ublk[n]) return;
-  const int gi = kg.gi[q], sx = kg.sx[q], sy = kg.sy[q];
   const int ii = ubi[ub];
-  const InstDev &I = insts[ii];
-  const int64_t lt = (ub - ublk[ii]) * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t t = lt / G;
-  const int j = (int)(lt % G);
-  const int64_t nT = (int64_t)(I.g.NX + 1) * (I.g.NY + 1);
-  const bool inside = t < nT;
-  const int cx = inside ? (int)(t % (I.g.NX + 1)) - 1 : -2;
-  const int cy = inside ? (int)(t / (I.g.NX + 1)) - 1 : -2;
-  const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
-  const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
-  const int64_t nL = (int64_t)Lx.n * Ly.n;
-  const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
-  partials_cell_sorted<METHOD, NF, G>(I.g, all_groups[(int64_t)gi * n + ii], start + I.bucket_off, xs, ys, ms,
-                                      part + all_partoff[(int64_t)gi * n + ii], nL, tl, live ? cx : 0,
-                                      live ? cy : 0, j, live);
+  ens_partials_block<METHOD, NF, G>(insts[ii], ii, n, ub - ublk[ii], kg.gi[q], kg.sx[q], kg.sy[q], all_groups,
+                                    all_partoff, start, xs, ys, ms, part);
 }
 
 // node pass: launch row blockIdx.y = group g0 + blockIdx.y, every field of it
@@ -236,11 +387,7 @@ __global__ void __launch_bounds__(256) kb_nodes(const InstDev *__restrict__ inst
                                                 const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
                                                 const NodeGroup *__restrict__ nodes, int g0) {
   const int ii = bi[blockIdx.x];
-  const InstDev &I = insts[ii];
-  const int64_t i = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= (int64_t)I.g.NX * I.g.NY) return;
-  const int ix = (int)(i % I.g.NX), jy = (int)(i / I.g.NX);
-  node_group(nodes[(int64_t)(g0 + blockIdx.y) * n + ii], ix, jy, i);
+  ens_nodes_block(insts[ii], blockIdx.x - blk[ii], nodes[(int64_t)(g0 + blockIdx.y) * n + ii]);
 }
 
 template <int G>
@@ -255,57 +402,261 @@ __global__ void __launch_bounds__(256) kb_partials_subt(const InstDev *__restric
                                                         const double *__restrict__ mt, double *__restrict__ part,
                                                         double *__restrict__ dTsub) {
   const int ii = bi[blockIdx.x];
-  const InstDev &I = insts[ii];
-  const int nLx = I.g.NX - 1, nLy = I.g.NY - 1;
-  const int64_t nL = (int64_t)nLx * nLy;
-  const int64_t lt = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t t = lt / G;
-  const int j = (int)(lt % G);
-  const bool live = t < nL;
-  const int lx = live ? (int)(t % nLx) : 0;
-  const int ly = live ? (int)(t / nLx) : 0;
-  SubT sb;
-  sb.T = I.lastT;
-  sb.rho = I.rhon;
-  sb.Cp = I.Cpn;
-  sb.k = I.kn;
-  sb.MT = mt;                     // payload column of marker T (global positions)
-  sb.dTsub = dTsub + I.mark_off;  // written at perm[p], the instance's marker index
-  sb.rho0 = I.rho0;
-  sb.dt = I.dt;
-  sb.dsub = I.dsub;
-  partials_cell_subt_sorted<G>(I.g, groups[ii], start + I.bucket_off, perm, xs, ys, part + partoff[ii], nL, t, lx,
-                               ly, j, live, sb);
+  ens_subt_block<G>(insts[ii], ii, blockIdx.x - blk[ii], groups, partoff, start, perm, xs, ys, mt, part, dTsub);
 }
 
 __global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ insts, int n,
                                                  const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
                                                  const double *__restrict__ deltaT, const double *__restrict__ dTsub) {
-  // the instance's grid coordinates are staged in LDS (as in kb_locate)
   extern __shared__ double sgrid[];
   const int ii = bi[blockIdx.x];
-  const InstDev &I = insts[ii];
-  const DGrid &g = I.g;
-  for (int i = threadIdx.x; i < g.NX; i += blockDim.x) sgrid[i] = g.x[i];
-  for (int i = threadIdx.x; i < g.NY; i += blockDim.x) sgrid[g.NX + i] = g.y[i];
-  for (int i = threadIdx.x; i <= g.NX; i += blockDim.x) sgrid[g.NX + g.NY + i] = g.xc[i];
-  for (int i = threadIdx.x; i <= g.NY; i += blockDim.x) sgrid[2 * g.NX + g.NY + 1 + i] = g.yc[i];
-  __syncthreads();
-  DGrid gl = g;
-  gl.x = sgrid;
-  gl.y = sgrid + g.NX;
-  gl.xc = sgrid + g.NX + g.NY;
-  gl.yc = sgrid + 2 * g.NX + g.NY + 1;
-  const int64_t m = (blockIdx.x - blk[ii]) * (int64_t)blockDim.x + threadIdx.x;
-  if (m >= I.nMark) return;
-  advect_marker<false>(m, gl, I.X, I.Y, I.VX, I.VY, I.wxy, I.cellX, I.cellY, I.vx, I.vy, I.wxyn, I.bv, I.dt, I.T, I.Tdot,
-                deltaT + I.node_off, dTsub + I.mark_off, I.dt);
-  // the rotation fused behind the RK4 stages: the batched launch keeps one kernel per phase
-  rotate_eii_marker(m, g.ncx(), g.NY, I.cellX, I.cellY, I.wxy, I.sxx, I.syy, I.sxy, I.exx, I.eyy, I.exy, I.Eii, I.dt,
-                    I.jaumann);
+  const DGrid gl = ens_stage_grid(insts[ii].g, sgrid);
+  ens_advect_block(insts[ii], blockIdx.x - blk[ii], gl, deltaT, dTsub);
+}
+
+// ---- persistent step ------------------------------------------------------------------------
+// The whole step as ONE launch: a grid of resident workgroups takes chunks of
+// consecutive work items from a ticket and runs each item's body (the same
+// bodies as the per-phase kernels).  A chunk is PS_CHUNK items, an eighth of
+// that for the ordering pass's 64-cell tiles, and one tile of the scan: its
+// tiles chain through the look-back, so consecutive tiles must run side by
+// side, not one after another in one workgroup.  Items are numbered phase by phase in
+// step order (zero, locate, scan, scatter, order, partials, nodes, dTsub
+// partials, dTsub nodes, advect), instances in order within a phase, so every
+// item depends only on items numbered before it: an item waits until the
+// previous phase of its own instance is complete (the scan: until the counts
+// of every bucket up to its tile are), and instances overlap freely across
+// phase boundaries.
+//
+// Completion is counted per (phase, instance).  A workgroup credits its items
+// when its chunk ends (one release per chunk) and counts its own finished but
+// uncredited items as done when it checks a dependency, so the lowest
+// uncredited item can always run: no wait can cycle, and every wait is also
+// bounded (PS_SPIN_MAX polls; on expiry the step stops and sets a sticky flag,
+// counters[2], which fvm_ensemble_counters reports).  The last workgroup to
+// leave resets the ticket and the completion counts for the next launch.
+struct PsPend {
+  int n;
+  int ph[PS_PEND], inst[PS_PEND], cnt[PS_PEND];
+  // (phase, instance) pairs this workgroup has seen complete and acquired:
+  // their data is final, so later items that need them wait and fence no more
+  int nsat;
+  int sat[PS_SAT];
+};
+
+__device__ __forceinline__ int ps_pending(const PsPend &p, int ph, int i) {
+  int c = 0;
+  for (int k = 0; k < p.n; k++) c += (p.ph[k] == ph && p.inst[k] == i) ? p.cnt[k] : 0;
+  return c;
+}
+
+// thread 0: credit the pending items (after a release, so that everything the
+// workgroup wrote is visible before any count that vouches for it)
+__device__ __forceinline__ void ps_flush(const PersistArgs &A, PsPend &p) {
+  if (p.n == 0) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  for (int k = 0; k < p.n; k++)
+    __hip_atomic_fetch_add(&A.done[p.ph[k] * A.n + p.inst[k]], p.cnt[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  p.n = 0;
+}
+
+__device__ __forceinline__ void ps_credit(const PersistArgs &A, PsPend &p, int ph, int lo, int hi, int cnt) {
+  for (int i = lo; i <= hi; i++) {
+    int k = 0;
+    while (k < p.n && !(p.ph[k] == ph && p.inst[k] == i)) k++;
+    if (k == p.n) {
+      if (p.n == PS_PEND) {
+        ps_flush(A, p);
+        k = 0;
+      }
+      p.ph[k] = ph, p.inst[k] = i, p.cnt[k] = 0;
+      p.n = k + 1 > p.n ? k + 1 : p.n;
+    }
+    p.cnt[k] += cnt;
+  }
+}
+
+// thread 0: wait until phase `ph` is complete for instances lo..hi; false if
+// the wait ran out or another workgroup gave up
+__device__ __forceinline__ bool ps_wait(const PersistArgs &A, PsPend &p, int ph, int lo, int hi) {
+  bool global = false;
+  for (int i = lo; i <= hi; i++) {
+    const int key = ph * A.n + i;
+    bool seen = false;
+    for (int k = 0; k < p.nsat; k++) seen |= p.sat[k] == key;
+    if (seen) continue;
+    const int want = A.need[ph * A.n + i] - ps_pending(p, ph, i);
+    if (want <= 0) continue;
+    global = true;
+    int spins = 0;
+    while (__hip_atomic_load(&A.done[ph * A.n + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+      if (++spins > PS_SPIN_MAX || __hip_atomic_load(&A.counters[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        atomicOr(&A.counters[2], 1);
+        return false;
+      }
+      __builtin_amdgcn_s_sleep(4);
+    }
+    // complete without this workgroup's pending items: remember it (the
+    // acquire below covers it)
+    p.sat[p.nsat < PS_SAT ? p.nsat : PS_SAT - 1] = key;
+    if (p.nsat < PS_SAT) p.nsat++;
+  }
+  if (global) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+// Three waves per SIMD (168 VGPRs, a few spills) ran the 64-instance step in
+// 18.6 ms against 20.5 ms at two (213 VGPRs, none); chunks of 8, 32 or 128
+// items were within 10% of each other.
+#ifndef PS_WAVES_PER_EU
+#define PS_WAVES_PER_EU 3
+#endif
+// Runs of items: consecutive items of one phase and one instance share one
+// dependency wait, one staging of the instance's grid and one credit, so the
+// per-item cost is the body alone.
+template <int NF, int G>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PS_WAVES_PER_EU)))
+kb_persist(const PersistArgs *__restrict__ Ap) {
+  const PersistArgs &A = *Ap;
+  extern __shared__ double sgrid[];
+  __shared__ int64_t s_item;
+  __shared__ int s_go;
+  __shared__ PsPend pend;
+  if (threadIdx.x == 0) pend.n = 0, pend.nsat = 0;
+  bool stop = false;
+  while (!stop) {
+    if (threadIdx.x == 0) {
+      const bool ab = __hip_atomic_load(&A.counters[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+      const int64_t c = ab ? A.nchunk : (int64_t)atomicAdd(&A.ticket[0], 1ull);
+      s_item = c < A.nchunk ? c : A.nchunk;
+    }
+    __syncthreads();
+    // the chunk number is the same in every lane: say so, so that everything
+    // derived from it (instance record, rows, column pointers) is scalar
+    const int64_t ch = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(s_item >> 32)) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readfirstlane((int)s_item));
+    if (ch >= A.nchunk) break;
+    const int64_t it1 = A.chunk[ch + 1];
+    int ph = 0;
+    for (int64_t it = A.chunk[ch]; it < it1;) {
+      while (it >= A.off[ph + 1]) ph++;
+      const int64_t b = it - A.off[ph];
+      // the run's instance, its end (the instance's last item of this phase, or
+      // the chunk's end), dependency (phase dph for instances dlo..dhi) and
+      // credit range
+      int ii = 0, dph = -1, dlo = 0, dhi = 0, clo = 0, chi = 0;
+      int64_t lb = 0, bend = A.off[ph + 1] - A.off[ph];
+      switch (ph) {
+        case PS_ZERO: break;
+        case PS_LOCATE: case PS_SCATTER: case PS_ADVECT:
+          ii = A.mbi[b], lb = b - A.mblk[ii], bend = A.mblk[ii + 1];
+          break;
+        case PS_SCAN:
+          dph = PS_LOCATE, dlo = 0, dhi = A.scan_upto[b], clo = A.scan_lo[b], chi = A.scan_hi[b], bend = b + 1;
+          break;
+        case PS_ORDER: ii = A.tile_inst[b], lb = b - A.tile_off[ii], bend = A.tile_off[ii + 1]; break;
+        case PS_PART: ii = A.ubi[b], lb = b - A.ublk[ii], bend = A.ublk[ii + 1]; break;
+        case PS_NODES: case PS_NODES2: ii = A.nbi[b], lb = b - A.nblk[ii], bend = A.nblk[ii + 1]; break;
+        case PS_SUBT: ii = A.sbi[b], lb = b - A.sblk[ii], bend = A.sblk[ii + 1]; break;
+      }
+      if (ph == PS_LOCATE) dph = PS_ZERO, dlo = dhi = 0;
+      else if (ph > PS_SCAN) dph = ph - 1, dlo = dhi = ii;
+      if (ph > PS_SCAN || ph == PS_LOCATE) clo = chi = ii;
+      const int64_t rend = A.off[ph] + bend < it1 ? A.off[ph] + bend : it1;
+      const int nrun = (int)(rend - it);
+      if (threadIdx.x == 0) s_go = dph < 0 ? 1 : (ps_wait(A, pend, dph, dlo, dhi) ? 1 : 0);
+      __syncthreads();
+      if (!__builtin_amdgcn_readfirstlane(s_go)) {
+        stop = true;
+        break;
+      }
+      const InstDev &I = A.insts[ii];
+      switch (ph) {
+        case PS_ZERO:
+          for (int64_t z = b; z < b + nrun; z++) {
+            const int64_t z0 = z * PS_ZERO_ITEM;
+            for (int64_t k = z0 + threadIdx.x; k < z0 + PS_ZERO_ITEM; k += blockDim.x) {
+              if (k <= A.nbucket) A.count[k] = 0;
+              if (k < A.nstate) A.scan_state[k] = 0ull;
+            }
+            if (z == 0 && threadIdx.x < 2) A.counters[threadIdx.x] = 0;
+          }
+          break;
+        case PS_LOCATE: {
+          const DGrid gl = ens_stage_grid(I.g, sgrid);
+          for (int r = 0; r < nrun; r++) ens_locate_block(I, lb + r, gl, A.key, A.slot, A.count, A.counters);
+          break;
+        }
+        case PS_SCAN: scan_tile((int32_t)b, A.count, A.start, A.scan_state, A.nbucket, A.scan_fail); break;
+        case PS_SCATTER:
+          for (int r = 0; r < nrun; r++) ens_scatter_block(I, lb + r, A.key, A.slot, A.start, A.perm);
+          break;
+        case PS_ORDER: {
+          // each tile's 64 cells, 16 per wavefront (cell l at (l >> 3, l & 7) in the tile)
+          const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+          const int ncx = I.g.NX - 1, ncy = I.g.NY - 1;
+          const uint32_t ntx = (uint32_t)(ncx + 7) >> 3;
+          for (int r = 0; r < nrun; r++) {
+            const uint32_t t = (uint32_t)(lb + r);
+            for (int l = wave; l < 64; l += 4) {
+              const int cx = (int)(t % ntx) * 8 + (l >> 3), cy = (int)(t / ntx) * 8 + (l & 7);
+              if (cx < ncx && cy < ncy)
+                ens_order_cell(I, ii, A.n, A.cell_off[ii] + (int64_t)cy * ncx + cx, A.start, A.perm, A.xs, A.ys,
+                               A.ms, A.ncol, A.colsrc, A.pcols, A.nmark);
+            }
+          }
+          break;
+        }
+        case PS_PART:
+          for (int r = 0; r < nrun; r++)
+            for (int q = 0; q < A.nrow; q++)
+              ens_partials_block<PM_ARITH, NF, G>(I, ii, A.n, lb + r, A.kg.gi[q], A.kg.sx[q], A.kg.sy[q],
+                                                  A.groups_sorted, A.partoff, A.start, A.xs, A.ys, A.ms, A.part);
+          break;
+        case PS_NODES:
+          for (int r = 0; r < nrun; r++)
+            for (int g = 0; g < A.ng; g++) ens_nodes_block(I, lb + r, A.nodes[(int64_t)g * A.n + ii]);
+          break;
+        case PS_SUBT:
+          for (int r = 0; r < nrun; r++)
+            ens_subt_block<G>(I, ii, lb + r, A.groups + (int64_t)A.ng * A.n, A.partoff + (int64_t)A.ng * A.n,
+                              A.start, A.perm, A.xs, A.ys, A.mt, A.part, A.dTsub);
+          break;
+        case PS_NODES2:
+          for (int r = 0; r < nrun; r++) ens_nodes_block(I, lb + r, A.nodes[(int64_t)A.ng * A.n + ii]);
+          break;
+        case PS_ADVECT: {
+          const DGrid gl = ens_stage_grid(I.g, sgrid);
+          for (int r = 0; r < nrun; r++) ens_advect_block(I, lb + r, gl, A.deltaT, A.dTsub);
+          break;
+        }
+      }
+      __syncthreads();  // the run's writes (and its use of the staged grid) are done
+      if (threadIdx.x == 0) ps_credit(A, pend, ph, clo, chi, nrun);
+      it = rend;
+    }
+    if (threadIdx.x == 0) ps_flush(A, pend);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    ps_flush(A, pend);
+    // acquire-release: the last one out sees every other workgroup's credits
+    // before it clears them
+    const unsigned long long prev =
+        __hip_atomic_fetch_add(&A.ticket[1], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == (unsigned long long)gridDim.x - 1) {  // last one out: reset for the next launch
+      for (int k = 0; k < PS_N * A.n; k++) A.done[k] = 0;
+      A.ticket[0] = 0ull;
+      A.ticket[1] = 0ull;
+    }
+  }
 }
 
 // ---- host side ----------------------------------------------------------------------------
+static void free_dev(void *p) {
+  if (p) (void)hipFree(p);
+}
+
 // block offsets rows: 0 markers (tune.block), 1 nodes (256), 2 + g lattice cells of
 // group g, then the universal lattice ((NX+1) x (NY+1) cells) of the shared
 // partials launches
@@ -313,6 +664,8 @@ static int64_t *blk_row(fvm_ensemble *E, int r) { return E->dblk + (int64_t)r * 
 static const int32_t *bi_row(fvm_ensemble *E, int r) { return E->dbi + E->hbi_off[r]; }
 
 static int urow(const fvm_ensemble *E) { return 2 + E->ng + 1; }
+
+static int persist_setup(fvm_ctx *ctx, fvm_ensemble *E);
 
 static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
   const int n = E->n, rows = urow(E) + 1;
@@ -355,6 +708,161 @@ static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
   FVM_HIP(ctx, hipMemcpyAsync(E->dbi_off, E->hbi_off.data(), sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice,
                               ctx->stream));
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return E->tune.persistent ? persist_setup(ctx, E) : FVM_OK;
+}
+
+// ---- persistent schedule (host) -------------------------------------------------------------
+#define PS_DISPATCH(NF_, G_, CALL)                                 \
+  switch ((NF_) * 100 + (G_)) {                                    \
+    case 108: CALL(1, 8); break;                                   \
+    case 116: CALL(1, 16); break;                                  \
+    case 132: CALL(1, 32); break;                                  \
+    case 208: CALL(2, 8); break;                                   \
+    case 216: CALL(2, 16); break;                                  \
+    case 232: CALL(2, 32); break;                                  \
+    case 308: CALL(3, 8); break;                                   \
+    case 316: CALL(3, 16); break;                                  \
+    case 332: CALL(3, 32); break;                                  \
+    case 408: CALL(4, 8); break;                                   \
+    case 416: CALL(4, 16); break;                                  \
+    case 432: CALL(4, 32); break;                                  \
+    case 508: CALL(5, 8); break;                                   \
+    case 516: CALL(5, 16); break;                                  \
+    case 532: CALL(5, 32); break;                                  \
+    default: return fvm_set_err(ctx, FVM_ERR_ARG, "persistent step: no kernel for %d fields, %d lanes", NF_, G_); \
+  }
+
+// Item ranges of every phase, the completion counts each (phase, instance)
+// needs, the scan tiles' instance ranges and the launch arguments; the
+// completion counts and the ticket start at zero (every launch leaves them so).
+static int persist_setup(fvm_ctx *ctx, fvm_ensemble *E) {
+  const int n = E->n, u = urow(E);
+  auto rowcnt = [&](int r, int i) { return E->hblk[(size_t)r * (n + 1) + i + 1] - E->hblk[(size_t)r * (n + 1) + i]; };
+  auto rowtot = [&](int r) { return E->hblk[(size_t)r * (n + 1) + n]; };
+  const int64_t T = SCAN_BS * SCAN_IPT;
+  const int64_t ntiles = fvm_scan_tiles(E->nbucket), nstate = ntiles + 2;
+  const int64_t nzero = (std::max(E->nbucket + 1, nstate) + PS_ZERO_ITEM - 1) / PS_ZERO_ITEM;
+  std::vector<int32_t> need((size_t)PS_N * n, 0), upto((size_t)ntiles), lo((size_t)ntiles, n), hi((size_t)ntiles, -1);
+  auto at = [&](int ph, int i) -> int32_t & { return need[(size_t)ph * n + i]; };
+  at(PS_ZERO, 0) = (int32_t)nzero;
+  for (int i = 0; i < n; i++) {
+    const InstDev &I = E->hinst[i];
+    at(PS_LOCATE, i) = at(PS_SCATTER, i) = at(PS_ADVECT, i) = (int32_t)rowcnt(0, i);
+    at(PS_ORDER, i) = (int32_t)(E->hcell[n + 1 + i + 1] - E->hcell[n + 1 + i]);
+    at(PS_PART, i) = (int32_t)rowcnt(u, i);
+    at(PS_NODES, i) = at(PS_NODES2, i) = (int32_t)rowcnt(1, i);
+    at(PS_SUBT, i) = (int32_t)rowcnt(2 + E->ng, i);
+    // scan tiles whose output the instance reads: its buckets and the start of
+    // the next instance's (or the total, written by the last tile)
+    const int64_t b0 = I.bucket_off, b1 = I.bucket_off + 4LL * (I.g.NX - 1) * (I.g.NY - 1);
+    const int64_t t0 = b0 / T, t1 = std::min(b1, E->nbucket - 1) / T;
+    at(PS_SCAN, i) = (int32_t)(t1 - t0 + 1);
+    for (int64_t t = t0; t <= t1; t++) {
+      lo[(size_t)t] = std::min(lo[(size_t)t], i);
+      hi[(size_t)t] = std::max(hi[(size_t)t], i);
+    }
+  }
+  // tile t sums the counts of every bucket before its end: instances whose
+  // buckets start before it
+  for (int64_t t = 0, i = 0; t < ntiles; t++) {
+    const int64_t end = std::min((t + 1) * T, E->nbucket);
+    while (i + 1 < n && E->hinst[i + 1].bucket_off < end) i++;
+    upto[(size_t)t] = (int32_t)i;
+  }
+  PersistArgs &A = E->pa;
+  memset(&A, 0, sizeof(A));
+  const int64_t cnt[PS_N] = {nzero, rowtot(0), ntiles, rowtot(0), E->ntile, rowtot(u), rowtot(1), rowtot(2 + E->ng),
+                             rowtot(1), rowtot(0)};
+  A.off[0] = 0;
+  for (int ph = 0; ph < PS_N; ph++) A.off[ph + 1] = A.off[ph] + cnt[ph];
+  if ((size_t)ntiles > E->scan_cap) {
+    FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_dev(E->d_scan_upto), free_dev(E->d_scan_lo), free_dev(E->d_scan_hi);
+    E->d_scan_upto = E->d_scan_lo = E->d_scan_hi = nullptr;
+    E->scan_cap = 0;
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_scan_upto, sizeof(int32_t) * (ntiles + 1)));
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_scan_lo, sizeof(int32_t) * (ntiles + 1)));
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_scan_hi, sizeof(int32_t) * (ntiles + 1)));
+    E->scan_cap = (size_t)ntiles;
+  }
+  // chunk boundaries: never across a phase boundary
+  std::vector<int64_t> chunk;
+  for (int ph = 0; ph < PS_N; ph++) {
+    const int64_t cs = ph == PS_SCAN ? 1 : (ph == PS_ORDER ? std::max(1, PS_CHUNK / 8) : PS_CHUNK);
+    for (int64_t it = A.off[ph]; it < A.off[ph + 1]; it += cs) chunk.push_back(it);
+  }
+  chunk.push_back(A.off[PS_N]);
+  A.nchunk = (int64_t)chunk.size() - 1;
+  if ((size_t)chunk.size() > E->chunk_cap) {
+    FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    free_dev(E->d_chunk);
+    E->d_chunk = nullptr;
+    E->chunk_cap = 0;
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_chunk, sizeof(int64_t) * chunk.size()));
+    E->chunk_cap = chunk.size();
+  }
+  FVM_HIP(ctx, hipMemcpyAsync(E->d_chunk, chunk.data(), sizeof(int64_t) * chunk.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
+  A.chunk = E->d_chunk;
+  if (!E->d_done) {
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_pa, sizeof(PersistArgs)));
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_done, sizeof(int32_t) * PS_N * n));
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_need, sizeof(int32_t) * PS_N * n));
+    FVM_HIP(ctx, hipMalloc((void **)&E->d_ticket, sizeof(unsigned long long) * 2));
+  }
+  FVM_HIP(ctx, hipMemcpyAsync(E->d_need, need.data(), sizeof(int32_t) * PS_N * n, hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->d_scan_upto, upto.data(), sizeof(int32_t) * ntiles, hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->d_scan_lo, lo.data(), sizeof(int32_t) * ntiles, hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemcpyAsync(E->d_scan_hi, hi.data(), sizeof(int32_t) * ntiles, hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipMemsetAsync(E->d_done, 0, sizeof(int32_t) * PS_N * n, ctx->stream));
+  FVM_HIP(ctx, hipMemsetAsync(E->d_ticket, 0, sizeof(unsigned long long) * 2, ctx->stream));
+  A.insts = E->dinst;
+  A.n = n;
+  A.ng = E->ng;
+  A.nrow = E->ng;
+  A.ncol = E->ncol;
+  A.mblk = blk_row(E, 0), A.mbi = bi_row(E, 0);
+  A.nblk = blk_row(E, 1), A.nbi = bi_row(E, 1);
+  A.sblk = blk_row(E, 2 + E->ng), A.sbi = bi_row(E, 2 + E->ng);
+  A.ublk = blk_row(E, u), A.ubi = bi_row(E, u);
+  A.cell_off = E->dcell_off;
+  A.tile_off = E->dcell_off + n + 1;
+  A.tile_inst = E->dtile_inst;
+  A.scan_upto = E->d_scan_upto, A.scan_lo = E->d_scan_lo, A.scan_hi = E->d_scan_hi;
+  A.done = E->d_done, A.need = E->d_need, A.ticket = E->d_ticket;
+  A.key = E->key, A.slot = E->slot, A.perm = E->perm, A.count = E->count, A.start = E->start;
+  A.counters = E->counters;
+  A.scan_fail = (int32_t *)ctx->red.ptr + FVM_FLAG_SCAN_FAIL;
+  A.scan_state = (unsigned long long *)E->sums;
+  A.nbucket = E->nbucket, A.nstate = nstate, A.nmark = E->nmark;
+  A.xs = E->xs, A.ys = E->ys, A.pcols = E->pcols, A.part = E->part, A.dTsub = E->dTsub, A.deltaT = E->deltaT;
+  A.mt = E->pcols + (int64_t)E->mt_role * E->nmark;
+  A.ms = E->ms;
+  A.colsrc = E->dcolsrc;
+  A.groups_sorted = E->dgroups_sorted, A.groups = E->dgroups;
+  A.partoff = E->dpartoff;
+  A.nodes = E->dnodes;
+  for (int g = 0; g < E->ng; g++) A.kg.gi[g] = g, A.kg.sx[g] = E->gstag_x[g], A.kg.sy[g] = E->gstag_y[g];
+  // resident workgroups: what the kernel's registers and LDS allow on every CU
+  int per_cu = 0, cus = 0;
+#define PS_OCC(NF_, G_) \
+  FVM_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kb_persist<NF_, G_>, 256, E->lds_locate))
+  PS_DISPATCH(E->persist_nf, E->tune.lanes, PS_OCC);
+#undef PS_OCC
+  FVM_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+  E->persist_grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)std::max(per_cu, 1) * cus, A.nchunk));
+  FVM_HIP(ctx, hipMemcpyAsync(E->d_pa, &A, sizeof(PersistArgs), hipMemcpyHostToDevice, ctx->stream));
+  FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FVM_OK;
+}
+
+static int launch_persist(fvm_ctx *ctx, fvm_ensemble *E) {
+  FVM_PHASE(ctx, FVM_PHASE_PERSISTENT);
+#define PS_LAUNCH(NF_, G_) \
+  kb_persist<NF_, G_><<<(unsigned)E->persist_grid, 256, E->lds_locate, ctx->stream>>>(E->d_pa)
+  PS_DISPATCH(E->persist_nf, E->tune.lanes, PS_LAUNCH);
+#undef PS_LAUNCH
+  FVM_LAUNCH_CHECK(ctx);
   return FVM_OK;
 }
 
@@ -461,6 +969,7 @@ static int launch_kb_nodes(fvm_ctx *ctx, fvm_ensemble *E, int g0, int g1) {
 }
 
 static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
+  if (E->tune.persistent) return launch_persist(ctx, E);
   const int n = E->n;
   const int64_t mblk = E->hblk[0 * (n + 1) + n];
   const int B = E->tune.block;
@@ -505,10 +1014,6 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   return FVM_OK;
 }
 
-static void free_dev(void *p) {
-  if (p) (void)hipFree(p);
-}
-
 extern "C" int fvm_ensemble_destroy(fvm_ensemble *E) {
   if (!E) return FVM_OK;
   if (E->exec) (void)hipGraphExecDestroy(E->exec);
@@ -516,7 +1021,8 @@ extern "C" int fvm_ensemble_destroy(fvm_ensemble *E) {
   void *bufs[] = {E->dinst, E->dgroups, E->dnodes, E->dblk,    E->dpartoff, E->key,       E->slot,
                   E->perm,  E->count,   E->start,  E->sums,    E->counters, E->part,      E->dTsub,
                   E->deltaT, E->xs,     E->ys,     E->pcols,   E->ms,       (void *)E->dcolsrc, E->dcell_off,
-                  E->dgroups_sorted, E->dbi, E->dtile_inst, E->dbi_off};
+                  E->dgroups_sorted, E->dbi, E->dtile_inst, E->dbi_off, E->d_done, E->d_need, E->d_ticket,
+                  E->d_scan_upto, E->d_scan_lo, E->d_scan_hi, E->d_pa, E->d_chunk};
   for (void *b : bufs) free_dev(b);
   delete E;
   return FVM_OK;
@@ -590,6 +1096,11 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
     for (int gi = 0; gi < E->ng; gi++)
       if (E->gmethod[gi] == PM_ARITH && E->gnf[gi] == nmax - 1) E->gsnf[gi] = nmax;
   }
+  // the persistent step runs one partials kind: every group ARITHMETIC with
+  // the same stored field count (at most 5)
+  E->persist_nf = E->ng <= KB_MAX_MULTI ? E->gsnf[0] : 0;
+  for (int gi = 0; gi < E->ng; gi++)
+    if (E->gmethod[gi] != PM_ARITH || E->gsnf[gi] != E->gsnf[0] || E->gsnf[gi] > 5) E->persist_nf = 0;
 
   // per-instance device views and offsets
   E->hinst.resize(n);
@@ -651,6 +1162,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
   ENS_ALLOC(ctx, E->start, sizeof(int32_t) * (E->nbucket + 1));
   ENS_ALLOC(ctx, E->sums, sizeof(unsigned long long) * (ntiles + 2));  // one-pass scan state
   ENS_ALLOC(ctx, E->counters, sizeof(int32_t) * 4);
+  FVM_HIP(ctx, hipMemsetAsync(E->counters, 0, sizeof(int32_t) * 4, ctx->stream));  // [2]: sticky stall flag
   ENS_ALLOC(ctx, E->part, sizeof(double) * parttot);
   ENS_ALLOC(ctx, E->dTsub, sizeof(double) * E->nmark);
   ENS_ALLOC(ctx, E->deltaT, sizeof(double) * E->nnode);
@@ -753,6 +1265,7 @@ extern "C" int fvm_ensemble_create(fvm_ctx *ctx, int32_t n, const fvm_instance *
     hcell[n + 1 + i + 1] = hcell[n + 1 + i] + ((ncx + 7) / 8) * ((ncy + 7) / 8);
   }
   E->ntile = hcell[2 * n + 1];
+  E->hcell = hcell;
   std::vector<int32_t> htile((size_t)E->ntile);
   for (int i = 0; i < n; i++)
     for (int64_t t = hcell[n + 1 + i]; t < hcell[n + 1 + i + 1]; t++) htile[(size_t)t] = i;
@@ -788,9 +1301,13 @@ extern "C" int fvm_ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
 
 extern "C" int fvm_ensemble_set_tuning(fvm_ctx *ctx, fvm_ensemble *E, const fvm_tuning *t) {
   if (!ctx || !E || !t) return FVM_ERR_ARG;
-  if ((t->block != 128 && t->block != 256 && t->block != 512) || (t->lanes != 8 && t->lanes != 16 && t->lanes != 32))
-    return fvm_set_err(ctx, FVM_ERR_ARG, "tuning: block in {128,256,512}, lanes in {8,16,32}");
-  if (E->tune.block == t->block && E->tune.lanes == t->lanes) return FVM_OK;
+  if ((t->block != 128 && t->block != 256 && t->block != 512) || (t->lanes != 8 && t->lanes != 16 && t->lanes != 32) ||
+      (t->persistent != 0 && t->persistent != 1))
+    return fvm_set_err(ctx, FVM_ERR_ARG, "tuning: block in {128,256,512}, lanes in {8,16,32}, persistent 0 or 1");
+  if (t->persistent && (t->block != 256 || E->persist_nf == 0))
+    return fvm_set_err(ctx, FVM_ERR_ARG,
+                       "tuning: the persistent step needs block 256 and one ARITHMETIC projection kind of <= 5 fields");
+  if (E->tune.block == t->block && E->tune.lanes == t->lanes && E->tune.persistent == t->persistent) return FVM_OK;
   E->tune = *t;
   if (E->exec) {  // a captured graph holds the old launch shapes
     (void)hipGraphExecDestroy(E->exec);
@@ -850,12 +1367,14 @@ extern "C" int fvm_ensemble_launch(fvm_ctx *ctx, fvm_ensemble *E) {
 extern "C" int fvm_ensemble_autotune(fvm_ctx *ctx, fvm_ensemble *E, int32_t reps, fvm_tuning *best,
                                      float *ms_per_step) {
   if (!ctx || !E || reps < 1) return FVM_ERR_ARG;
-  const int blocks[3] = {128, 256, 512}, lanes[3] = {8, 16, 32};
+  const int blocks[4] = {128, 256, 512, 256}, lanes[3] = {8, 16, 32};
   fvm_tuning bt = E->tune;
   float bms = 1e30f;
-  for (int bi = 0; bi < 3; bi++)
+  // per-phase launches with every block size, then the persistent step (if the
+  // projection list allows it)
+  for (int bi = 0; bi < (E->persist_nf ? 4 : 3); bi++)
     for (int li = 0; li < 3; li++) {
-      fvm_tuning t = {blocks[bi], lanes[li]};
+      fvm_tuning t = {blocks[bi], lanes[li], bi == 3 ? 1 : 0};
       FVM_TRY(fvm_ensemble_set_tuning(ctx, E, &t));
       FVM_TRY(fvm_ensemble_capture(ctx, E));
       FVM_TRY(fvm_ensemble_launch(ctx, E));  // warm
@@ -878,10 +1397,11 @@ extern "C" int fvm_ensemble_autotune(fvm_ctx *ctx, fvm_ensemble *E, int32_t reps
 
 extern "C" int fvm_ensemble_counters(fvm_ctx *ctx, fvm_ensemble *E, int32_t *nOut, int32_t *nBad) {
   if (!ctx || !E) return FVM_ERR_ARG;
-  int32_t h[2] = {0, 0};
-  FVM_HIP(ctx, hipMemcpyAsync(h, E->counters, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  int32_t h[3] = {0, 0, 0};
+  FVM_HIP(ctx, hipMemcpyAsync(h, E->counters, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
   FVM_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (nOut) *nOut = h[0];
   if (nBad) *nBad = h[1];
+  if (h[2]) return fvm_set_err(ctx, FVM_ERR_STALLED, "a persistent ensemble step gave up waiting; its results are incomplete");
   return FVM_OK;
 }

@@ -259,8 +259,11 @@ __device__ __forceinline__ void gather_entry(const PayloadCols &pc, int with_xy,
 // Put the four buckets of one basic cell (global bucket index 4c..4c+3) in
 // ascending marker order and call emit(p, v) for every final position p and
 // its marker v.  One wavefront per cell: the (usually <= 64) entries are ranked
-// across lanes; a fuller cell falls back to an insertion sort per bucket whose
-// lane then visits its own bucket (reading back its own writes).
+// across lanes.  A fuller cell is ordered bucket by bucket: a bucket of up to
+// 256 entries is held in registers (four per lane) and every entry is ranked
+// against all of them (marker indices are distinct); only a bucket above 256
+// entries falls back to an insertion sort by one lane (reading back its own
+// writes), which population control keeps from happening.
 template <class Emit>
 __device__ __forceinline__ void bucket_order_cell(int64_t c, const int32_t *__restrict__ start,
                                                   int32_t *__restrict__ perm, Emit &&emit) {
@@ -304,19 +307,50 @@ __device__ __forceinline__ void bucket_order_cell(int64_t c, const int32_t *__re
       perm[p] = v;
       emit(p, v);
     }
-  } else if (lane < 4) {
-    const int s0 = lane == 0 ? b0 : (lane == 1 ? b1 : (lane == 2 ? b2 : b3));
-    const int s1 = lane == 0 ? b1 : (lane == 1 ? b2 : (lane == 2 ? b3 : b4));
-    for (int i = s0 + 1; i < s1; i++) {
-      const int32_t v = perm[i];
-      int j = i - 1;
-      while (j >= s0 && perm[j] > v) {
-        perm[j + 1] = perm[j];
-        j--;
+  } else {
+#pragma unroll 1
+    for (int q = 0; q < 4; q++) {
+      const int s0 = q == 0 ? b0 : (q == 1 ? b1 : (q == 2 ? b2 : b3));
+      const int s1 = q == 0 ? b1 : (q == 1 ? b2 : (q == 2 ? b3 : b4));
+      const int len = s1 - s0;
+      if (len <= 256) {
+        int v[4], r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int pos = s0 + lane + 64 * u;
+          v[u] = pos < s1 ? perm[pos] : 0x7fffffff;
+          r[u] = 0;
+        }
+#pragma unroll
+        for (int u2 = 0; u2 < 4; u2++) {
+          if (64 * u2 >= len) break;
+          for (int j = 0; j < 64; j++) {
+            const int w = read_lane(v[u2], j);
+#pragma unroll
+            for (int u = 0; u < 4; u++) r[u] += w < v[u] ? 1 : 0;
+          }
+        }
+        // every lane's loads completed before the ranks above: writing back
+        // into the bucket's own range is safe
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (lane + 64 * u < len) {
+            perm[s0 + r[u]] = v[u];
+            emit(s0 + r[u], v[u]);
+          }
+      } else if (lane == 0) {
+        for (int i = s0 + 1; i < s1; i++) {
+          const int32_t v = perm[i];
+          int j = i - 1;
+          while (j >= s0 && perm[j] > v) {
+            perm[j + 1] = perm[j];
+            j--;
+          }
+          perm[j + 1] = v;
+        }
+        for (int p = s0; p < s1; p++) emit(p, perm[p]);
       }
-      perm[j + 1] = v;
     }
-    for (int p = s0; p < s1; p++) emit(p, perm[p]);
   }
 }
 

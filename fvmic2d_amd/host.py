@@ -294,19 +294,20 @@ class Ensemble:
     def launch(self):
         self.ctx.call("fvm_ensemble_launch", self.h)
 
-    def set_tuning(self, block, lanes):
-        t = fvm_tuning(block, lanes)
+    def set_tuning(self, block, lanes, persistent=0):
+        t = fvm_tuning(block, lanes, persistent)
         self.ctx.call("fvm_ensemble_set_tuning", self.h, C.byref(t))
 
     def get_tuning(self):
         t = fvm_tuning()
         self.ctx._check(self.ctx.lib.fvm_ensemble_get_tuning(self.h, C.byref(t)))
-        return t.block, t.lanes
+        return t.block, t.lanes, t.persistent
 
     def autotune(self, reps=5):
+        """((block, lanes, persistent), ms per step) of the fastest launch shape."""
         t, ms = fvm_tuning(), C.c_float()
         self.ctx.call("fvm_ensemble_autotune", self.h, I32(reps), C.byref(t), C.byref(ms))
-        return (t.block, t.lanes), ms.value
+        return (t.block, t.lanes, t.persistent), ms.value
 
     def counters(self):
         a, b = I32(), I32()

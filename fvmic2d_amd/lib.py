@@ -18,7 +18,8 @@ COMPAT_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libf
 ERRORS = {0: "FVM_OK", 1: "FVM_ERR_PERIODIC", 2: "FVM_ERR_CELL_OUT_OF_BOUNDS", 3: "FVM_ERR_BAD_STAG",
           4: "FVM_ERR_BAD_METHOD", 5: "FVM_ERR_NAN_RESIDUAL", 6: "FVM_ERR_VISCOSITY_LAW",
           7: "FVM_ERR_NOMEM", 8: "FVM_ERR_HIP", 9: "FVM_ERR_ARG", 10: "FVM_ERR_TOO_MANY_FIELDS",
-          11: "FVM_ERR_NOT_CONVERGED"}
+          11: "FVM_ERR_NOT_CONVERGED", 12: "FVM_ERR_BC", 13: "FVM_ERR_IO", 14: "FVM_ERR_CAPACITY",
+          15: "FVM_ERR_NEGATIVE_ADD", 16: "FVM_ERR_STALLED"}
 
 
 class FvmError(RuntimeError):
@@ -52,7 +53,7 @@ class fvm_instance(C.Structure):
 
 
 class fvm_tuning(C.Structure):
-    _fields_ = [("block", C.c_int32), ("lanes", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("lanes", C.c_int32), ("persistent", C.c_int32)]
 
 
 P = C.c_void_p

@@ -62,7 +62,8 @@ enum {
   FVM_ERR_BC = 12,               /* a boundary type whose ghost value the reference leaves undefined */
   FVM_ERR_IO = 13,               /* a marker file could not be opened, read or written, or has another layout */
   FVM_ERR_CAPACITY = 14,         /* markers.c:772-775: more markers to add than free slots (reference aborts) */
-  FVM_ERR_NEGATIVE_ADD = 15      /* markers.c:548-551: an under-full quarter's target is not above its count (aborts) */
+  FVM_ERR_NEGATIVE_ADD = 15,     /* markers.c:548-551: an under-full quarter's target is not above its count (aborts) */
+  FVM_ERR_STALLED = 16           /* a bounded wait of the ensemble's persistent step ran out; that step is incomplete */
 };
 
 /* ---- device marker store: one device column per Marker field --------------- */
@@ -380,7 +381,8 @@ enum {
   FVM_PHASE_SUBGRID_S1 = 7,
   FVM_PHASE_MARKER_UPDATE = 8, /* stress, strain/pressure, yield, viscosity, density */
   FVM_PHASE_OTHER = 9,
-  FVM_NUM_PHASES = 10
+  FVM_PHASE_PERSISTENT = 10,   /* the batched ensemble's one-launch step (every phase) */
+  FVM_NUM_PHASES = 11
 };
 int fvm_profile_enable(fvm_ctx *ctx, int32_t on);
 int fvm_profile_read(fvm_ctx *ctx, double *ms, int32_t *launches);
@@ -436,10 +438,16 @@ typedef struct {
 } fvm_instance;
 
 /* Launch shape: threads per block {128, 256, 512}; lanes per lattice cell in
- * the projection pass {8, 16, 32}. */
+ * the projection pass {8, 16, 32}; persistent 0: one launch per phase,
+ * 1: the whole step as one launch of resident workgroups that take work items
+ * (blocks of every phase of every instance) from a ticket, an item waiting
+ * only for the previous phase of its own instance (needs block 256 and a
+ * projection list whose groups are all ARITHMETIC with the same stored field
+ * count, at most 5). */
 typedef struct {
   int32_t block;
   int32_t lanes;
+  int32_t persistent;
 } fvm_tuning;
 
 typedef struct fvm_ensemble fvm_ensemble;
@@ -460,7 +468,8 @@ int fvm_ensemble_launch(fvm_ctx *ctx, fvm_ensemble *e);
 int fvm_ensemble_autotune(fvm_ctx *ctx, fvm_ensemble *e, int32_t reps, fvm_tuning *best,
                           float *ms_per_step);
 /* Markers outside the domain / with no valid cell, summed over instances, in
- * the last step (synchronises). */
+ * the last step (synchronises).  Returns FVM_ERR_STALLED if a persistent step
+ * since the ensemble's creation gave up waiting (its results are incomplete). */
 int fvm_ensemble_counters(fvm_ctx *ctx, fvm_ensemble *e, int32_t *nOut, int32_t *nBad);
 
 #ifdef __cplusplus

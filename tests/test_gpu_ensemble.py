@@ -1,5 +1,6 @@
 """Batched ensemble (fvm_ensemble_*): many independent thermal-marker problems of
-different sizes stepped by one launch per phase.
+different sizes stepped by one launch per phase, or by one persistent launch
+for the whole step.
 
 Bars: with the single-instance path's lanes per lattice cell (8) an
 ensemble member is bit-identical to the same problem stepped alone with
@@ -67,7 +68,7 @@ def test_ensemble_equals_single_instance():
     ws = _instances()
     alone, batch = _paths(ws, ctx), _paths(ws, ctx)
     ens = Ensemble(batch)
-    assert ens.get_tuning() == (256, 8)
+    assert ens.get_tuning() == (256, 8, 0)
     assert ens.n_markers == sum(w.n_markers for w in ws)
     for s in range(2):
         for p in alone:
@@ -178,15 +179,17 @@ def test_ensemble_autotune_keeps_results_valid():
     ws = _instances(3, seed=5, empty=False)
     paths = _paths(ws, ctx)
     ens = Ensemble(paths)
-    (block, lanes), ms = ens.autotune(reps=1)
-    assert block in (128, 256, 512) and lanes in (8, 16, 32) and ms > 0.0
-    assert ens.get_tuning() == (block, lanes)
-    # 9 shapes x 2 launches + 1 more: positions stay bit-identical to the oracle
+    (block, lanes, pers), ms = ens.autotune(reps=1)
+    assert block in (128, 256, 512) and lanes in (8, 16, 32) and pers in (0, 1) and ms > 0.0
+    assert ens.get_tuning() == (block, lanes, pers)
+    # 9 per-phase shapes and 3 persistent ones x 2 launches + 1 more: positions
+    # stay bit-identical to the oracle
     ens.launch()
     ctx.sync()
+    assert ens.counters()[1] == 0
     for i, (w, mp) in enumerate(zip(ws, paths)):
         st = oracle.State(w, extra=_outs(w))
-        for _ in range(19):
+        for _ in range(25):
             st.step_thermal()
         g = mp.get_markers()
         for f in ("X", "Y", "cellX", "cellY"):
@@ -203,8 +206,58 @@ def test_ensemble_rejects_bad_input():
         ens.set_tuning(96, 16)
     with pytest.raises(FvmError):
         ens.set_tuning(256, 12)
+    with pytest.raises(FvmError):  # the persistent step runs 256-thread workgroups
+        ens.set_tuning(128, 8, 1)
+    with pytest.raises(FvmError):
+        ens.set_tuning(256, 8, 2)
     ens.close()
     # projection patterns must agree
     paths[1].proj_list[0].stagx = 1
     with pytest.raises(FvmError):
         Ensemble(paths)
+
+
+@pytest.mark.parametrize("lanes", [8, 32])
+def test_ensemble_persistent_equals_per_phase(lanes):
+    """The one-launch persistent step (work items of every phase of every
+    instance from one ticket, an item waiting only on its own instance's
+    previous phase) gives bitwise the per-phase launches' results, eager and
+    graph-replayed, over three steps; 12 instances of 17^2..61^2 nodes put
+    several instances in one scan tile, and one of them has no markers."""
+    print("CASE ensemble persistent lanes=%d" % lanes, flush=True)
+    ctx = Context()
+    ws = workloads.ensemble(12, seed=31, nmin=17, nmax=61)
+    ws[4].markers = ws[4].markers[:0]
+    ref_p, per_p, gr_p = _paths(ws, ctx), _paths(ws, ctx), _paths(ws, ctx)
+    ref, per, gr = Ensemble(ref_p), Ensemble(per_p), Ensemble(gr_p)
+    ref.set_tuning(256, lanes, 0)
+    per.set_tuning(256, lanes, 1)
+    gr.set_tuning(256, lanes, 1)
+    assert per.get_tuning() == (256, lanes, 1)
+    gr.capture()
+    for s in range(3):
+        ref.step()
+        per.step()
+        gr.launch()
+        ctx.sync()
+        for i, (a, b, c) in enumerate(zip(ref_p, per_p, gr_p)):
+            _assert_same(a, b, "persistent step %d instance %d" % (s, i))
+            _assert_same(a, c, "persistent graph step %d instance %d" % (s, i))
+    assert per.counters() == ref.counters() and gr.counters() == ref.counters()
+    for e in (ref, per, gr):
+        e.close()
+
+
+def test_ensemble_persistent_needs_one_projection_kind():
+    """A projection list with two kinds (ARITHMETIC and ARITHMETIC_LOCAL
+    groups) runs per phase only."""
+    ctx = Context()
+    ws = _instances(2, seed=3, empty=False)
+    for w in ws:
+        w.projections = list(w.projections) + [("sxy", 0, 0, 1, "soxy")]
+    paths = _paths(ws, ctx)
+    ens = Ensemble(paths)
+    with pytest.raises(FvmError):
+        ens.set_tuning(256, 8, 1)
+    ens.step()
+    ens.close()

@@ -339,6 +339,32 @@ def test_plastic_yielding(vep):
     _cmp_markers(st, mp, fields=["eta"])
 
 
+def test_plastic_yielding_viscosity_laws():
+    """checkPlasticYielding with every viscosity law of viscosity.c:12-67 on the
+    GPU: materials 0-4 use laws 2 (linearised Arrhenius about Tref), 3 (eta T),
+    5 (exp(-QonR T)), 6 (the slab decoupling zone, entered by part of the
+    markers) and 0, with Drucker-Prager yielding (plasticity 1) and the
+    strain-weakening law 2 on material 2."""
+    w = _vep_small()
+    m, o = w.materials, w.options
+    for i, law in enumerate([2, 3, 5, 6, 0]):
+        m.hasEtaT[i] = law
+    m.QonR[2] = 1e-3
+    o.slabAngle, o.decoupleThickness = 0.5, 3.0e4
+    o.minDecoupleDepth, o.maxDecoupleDepth, o.decoupleEtaScaler = 1.0e4, 7.0e4, 0.01
+    st, mp = _pair(w)
+    for s in (st, mp):
+        s.find_marker_cells()
+    mk = w.markers
+    X, Y = mk["X"], mk["Y"]
+    zone = (mk["Mat"] == 3) & (X > Y / np.tan(0.5)) & (X < Y / np.tan(0.5) + 3.0e4) & (Y > 1e4) & (Y < 7e4)
+    assert zone.sum() > 100 and all((mk["Mat"] == i).sum() > 100 for i in range(5))
+    y_ref = st.check_plastic_yielding(w.dt)
+    y_gpu = mp.check_plastic_yielding(w.dt)
+    assert y_ref == y_gpu
+    _cmp_markers(st, mp, fields=["eta", "mu", "sxx", "syy", "sxy"])
+
+
 def test_glue_loops(vep):
     st, mp = _pair(vep)
     for s in (st, mp):
@@ -374,7 +400,8 @@ def test_step_convection_tdep_full(lanes):
 
 def test_step_crowded_cells():
     """Cells with more than 64 markers take the per-bucket ordering path (and
-    gather their payload there); results still match the oracle."""
+    gather their payload there): buckets of up to 256 ranked in registers, a
+    bucket of 300 by the insertion-sort fallback; results still match the oracle."""
     w = workloads.convection_tdep(N=41, NM=4)
     r = workloads.Rng(5, 3)
     m, x, y = w.markers, w.x, w.y
@@ -384,6 +411,9 @@ def test_step_crowded_cells():
     b = np.arange(2801, 2951)  # 150 markers inside one quadrant of cell (20, 5)
     m["X"][b] = x[20] + 0.49 * (x[21] - x[20]) * r.uniform(len(b))
     m["Y"][b] = y[6] - 0.49 * (y[6] - y[5]) * r.uniform(len(b))
+    c = np.arange(3001, 3301 * 3, 3)[:300]  # 300 markers inside one quadrant of cell (30, 30)
+    m["X"][c] = x[31] - 0.49 * (x[31] - x[30]) * r.uniform(len(c))
+    m["Y"][c] = y[30] + 0.49 * (y[31] - y[30]) * r.uniform(len(c))
     st, mp = _pair(w)
     sc = S.StepScales(st, vep=False)
     mp.step_thermal()
