@@ -409,8 +409,9 @@ def run_learned_guess(a, N, ntrain, ntest, epochs):
     workers = a.lg_workers or cpu_workers()[0]
     t0 = time.perf_counter()
     # spawned workers: in the default run this process has used the GPU already
-    raw_tr = lg.generate(N, list(range(ntrain)), workers=workers, start="spawn")
-    raw_te = lg.generate(N, list(range(10 ** 6, 10 ** 6 + ntest)), workers=workers, start="spawn")
+    prog = lambda d, t: log("learned guess: %d of %d host solves done (%.0f s)" % (d, t, time.perf_counter() - t0))
+    raw_tr = lg.generate(N, list(range(ntrain)), workers=workers, start="spawn", progress=prog)
+    raw_te = lg.generate(N, list(range(10 ** 6, 10 ** 6 + ntest)), workers=workers, start="spawn", progress=prog)
     t_gen = time.perf_counter() - t0
     log("learned guess: %d + %d states at %dx%d solved on the host in %.0f s" % (ntrain, ntest, N, N, t_gen))
     ctx = Context()
@@ -429,7 +430,7 @@ def run_learned_guess(a, N, ntrain, ntest, epochs):
         model.net.forward(dX)
     fwd_ms = ctx.timer_stop() / 10
     t0 = time.perf_counter()
-    rows = lg.evaluate(te, model)
+    rows = lg.evaluate(te, model, progress=lambda k, t: log("learned guess: %d of %d held-out states solved" % (k, t)))
     t_eval = time.perf_counter() - t0
     summ = lg.summarize(rows)
     flops_fwd = model_flops(model.net, N - 1) * 8

@@ -116,16 +116,26 @@ def enforce_box(x, N):
     return np.concatenate([vx.ravel(), vy.ravel(), p.ravel()])
 
 
-def generate(N, seeds, shift=0.03, workers=0, start="fork"):
+def generate(N, seeds, shift=0.03, workers=0, start="fork", progress=None):
     """The plain method's direct solves for every seed (host only).  With
     workers > 0 they run in worker processes: forked (start="fork", only before
     anything in this process has touched the GPU) or spawned (start="spawn",
-    fresh interpreters that inherit no GPU state)."""
+    fresh interpreters that inherit no GPU state).  progress(done, total) is
+    called after every 16 solves."""
     jobs = [(N, s, shift) for s in seeds]
+    out = []
     if workers > 0:
         with get_context(start).Pool(workers) as pool:
-            return pool.map(_worker, jobs, chunksize=1)
-    return [_worker(j) for j in jobs]
+            for r in pool.imap(_worker, jobs, chunksize=1):
+                out.append(r)
+                if progress and len(out) % 16 == 0:
+                    progress(len(out), len(jobs))
+        return out
+    for j in jobs:
+        out.append(_worker(j))
+        if progress and len(out) % 16 == 0:
+            progress(len(out), len(jobs))
+    return out
 
 
 class Dataset:
@@ -270,9 +280,10 @@ def balanced(A, b, M, p, N):
     return enforce_box(np.concatenate([v, x[2 * n:]]), N)
 
 
-def evaluate(ds, model, rtol=1e-8, restart=100):
+def evaluate(ds, model, rtol=1e-8, restart=100, progress=None):
     """FGMRES iterations from the plain guess, the previous solution and the
-    learned guess (with fallback) for every sample of ds."""
+    learned guess (with fallback) for every sample of ds; progress(k, total)
+    after every sample."""
     N, n = ds.N, ds.N * ds.N
     X, _ = ds.arrays()
     pred = model.predict(X)
@@ -296,6 +307,8 @@ def evaluate(ds, model, rtol=1e-8, restart=100):
             _, its, rel = stokes.fgmres(A, b, v, M, rtol=rtol, restart=restart, maxit=3000)
             res[nm] = its
         rows.append(dict(seed=s["seed"], r0=r0, its=res, fallback=fallback))
+        if progress:
+            progress(k + 1, len(ds.samples))
     return rows
 
 
