@@ -379,6 +379,39 @@ def run_thermal(a):
     }
 
 
+def run_store_order(a, steps=10):
+    """convection_tdep from a randomly permuted marker store (the spatial order a
+    long run drifts towards): the step before and after one fvm_markers_reorder,
+    and the reorder's own time (cells current; its bucket sort included)."""
+    from fvmic2d_amd import workloads
+    from fvmic2d_amd.host import MarkerPath
+    w = workloads.convection_tdep()
+    w.markers = w.markers[np.random.default_rng(7).permutation(w.n_markers)].copy()
+    mp = MarkerPath(w)
+
+    def timed():
+        for _ in range(2):
+            mp.step_thermal()
+        mp.ctx.sync()
+        mp.ctx.timer_start()
+        for _ in range(steps):
+            mp.step_thermal()
+        return mp.ctx.timer_stop() / steps
+    scrambled = timed()
+    reorder = []
+    for _ in range(3):
+        mp.find_marker_cells()
+        mp.ctx.timer_start()
+        mp.call("fvm_markers_reorder", mp.grid.ref, mp.markers.ref, None)
+        reorder.append(mp.ctx.timer_stop())
+    ordered = timed()
+    log("store order: scrambled %.3f ms/step, reordered %.3f ms/step, reorder %.3f ms" % (
+        scrambled, ordered, reorder[0]))
+    return {"workload": "convection_tdep, markers randomly permuted", "steps": steps,
+            "scrambled_ms_per_step": scrambled, "reordered_ms_per_step": ordered,
+            "reorder_ms": reorder[0], "reorder_ms_ordered_store": min(reorder[1:])}
+
+
 def copy_calibration(nbytes=512 << 20, reps=5):
     """A device-to-device copy of known size, for scaling the FETCH_SIZE and
     WRITE_SIZE counters (profile this run and divide nbytes by the counter)."""
@@ -789,6 +822,8 @@ def main():
         cpu_all = None if a.no_cpu or a.vep else cpu_all_cores("tdep", a.cpu_budget)
         out = run_thermal(a) if a.thermal else run_vep(a)
         line = dict(common, **out)
+        if a.thermal:
+            line["store_order"] = run_store_order(a)
         if a.thermal and not a.no_cpu:
             line["cpu_baseline"] = cpu_baseline_tdep(a.cpu_budget)
             line["cpu_baseline_all_cores"] = cpu_all
@@ -808,6 +843,7 @@ def main():
     if not (a.no_extras or a.ensemble):
         line["convection_tdep"] = run_thermal(a)
         log("single instance convection_tdep done")
+        line["convection_tdep"]["store_order"] = run_store_order(a)
         line["vep_subduction"] = run_vep(a, extras=False)
         log("vep_subduction done")
         try:

@@ -81,7 +81,7 @@ extern "C" int fvm_ctx_destroy(fvm_ctx *ctx) {
   fvm_buf *bufs[] = {&ctx->key,      &ctx->slot,     &ctx->perm,    &ctx->count,   &ctx->start,
                      &ctx->fill,     &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
                      &ctx->red,      &ctx->payload,   &ctx->payload_xy, &ctx->xfer,    &ctx->popc,
-                     &ctx->popc2};
+                     &ctx->popc2,    &ctx->reord};
   for (fvm_buf *b : bufs)
     if (b->ptr) (void)hipFree(b->ptr);
   if (ctx->pinned.ptr) (void)hipHostFree(ctx->pinned.ptr);
@@ -105,7 +105,8 @@ extern "C" size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx) {
   if (!ctx) return 0;
   const fvm_buf *bufs[] = {&ctx->key,  &ctx->slot,    &ctx->perm,     &ctx->count,   &ctx->start,
                            &ctx->fill, &ctx->blocksums, &ctx->partials, &ctx->scratch, &ctx->staging,
-                           &ctx->red,  &ctx->payload,   &ctx->payload_xy, &ctx->popc, &ctx->popc2};
+                           &ctx->red,  &ctx->payload,   &ctx->payload_xy, &ctx->popc, &ctx->popc2,
+                           &ctx->reord};
   size_t s = 0;
   for (const fvm_buf *b : bufs) s += b->bytes;
   return s;
@@ -233,6 +234,12 @@ __host__ __device__ constexpr int dcol_offset(int c) { return c < 4 ? c * 8 : (c
 __host__ __device__ constexpr int icol_offset(int c) { return c == 0 ? 32 : (c < 4 ? 184 + (c - 1) * 4 : 208); }
 static_assert(dcol_offset(4) == 40 && dcol_offset(21) == 176 && dcol_offset(22) == 200, "T, wxy, residual");
 static_assert(icol_offset(1) == 184 && icol_offset(3) == 192 && icol_offset(4) == 208, "cellX, cpu, isYielding");
+
+static_assert(kNumD == FVM_NUM_DCOLS && kNumI == FVM_NUM_ICOLS, "column counts");
+void fvm_marker_columns(fvm_markers *mk, double **d, int32_t **i) {
+  for (int c = 0; c < kNumD; c++) d[c] = *dcol(mk, c);
+  for (int c = 0; c < kNumI; c++) i[c] = *icol(mk, c);
+}
 
 static size_t col_stride(int32_t n) { return ((size_t)n * 8 + 255) / 256 * 256; }
 

@@ -28,6 +28,7 @@ struct fvm_ctx {
   fvm_buf popc, popc2;  // population control: per-quarter plan and out list; added markers
   fvm_buf payload;     // bucket-ordered copies of the marker columns a projection reads
   fvm_buf payload_xy;  // bucket-ordered X, Y, Mat: valid while xy_serial == sort_serial
+  fvm_buf reord;       // fvm_markers_reorder: flags, prefix, scan state, order, column staging
   uint64_t sort_serial = 0, xy_serial = 0;
   // columns held in `payload` (slot k = payload + k * stride doubles) for sort
   // payload_serial; a fused subgrid pass may take its T column from there only
@@ -375,6 +376,12 @@ int fvm_project_groups(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, i
                        const SubT *sub = nullptr);
 // Invalidate the cached bucket order (positions or cells changed).
 static inline void fvm_markers_moved(fvm_ctx *ctx) { ctx->epoch++; }
+
+// The columns of a marker store in a fixed order (context.hip): FVM_NUM_DCOLS
+// FP64 columns (X first) and FVM_NUM_ICOLS int32 columns (Mat first).
+#define FVM_NUM_DCOLS 23
+#define FVM_NUM_ICOLS 5
+void fvm_marker_columns(fvm_markers *mk, double **d, int32_t **i);
 
 // ---- per-phase timing --------------------------------------------------------
 struct ProfScope {

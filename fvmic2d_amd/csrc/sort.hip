@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) k_zero_counts(int32_t *__restrict__ count
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (count && i < n) count[i] = 0;
   if (state && i < nstate) state[i] = 0ull;
-  if (i < 2) counters[i] = 0;
+  if (counters && i < 2) counters[i] = 0;
 }
 
 __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__restrict__ key,
@@ -335,6 +335,109 @@ int fvm_sort_markers(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, con
   }
   FVM_TRY(finish_sort(ctx, g, mk, sg));
   if (gathered) *gathered = sg != nullptr && n > 0;
+  return FVM_OK;
+}
+
+// ---- store reorder (fvm_markers_reorder) ---------------------------------------------
+// Markers outside the bucket order (key < 0: outside the domain or not located)
+// are flagged, and an exclusive scan of the flags (k_scan_onepass) gives their
+// rank among themselves, so they follow the ordered markers in index order.
+__global__ void __launch_bounds__(256) k_flag_unordered(int32_t n, const int32_t *__restrict__ key,
+                                                        int32_t *__restrict__ flag) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < n) flag[m] = key[m] < 0 ? 1 : 0;
+}
+
+// order[p] = index of the marker that moves to slot p: the bucket order for
+// p < nact = *total (the scan's total of the bucket counts), then the
+// unordered markers by index.
+__global__ void __launch_bounds__(256) k_reorder_index(int32_t n, const int32_t *__restrict__ key,
+                                                       const int32_t *__restrict__ prefix,
+                                                       const int32_t *__restrict__ total,
+                                                       const int32_t *__restrict__ perm, int32_t *__restrict__ order) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const int32_t nact = *total;
+  if (m < nact) order[m] = perm[m];
+  if (key[m] < 0) {
+    const int32_t p = nact + prefix[m];
+    if ((uint32_t)p < (uint32_t)n) order[p] = (int32_t)m;  // always true for a valid scan
+  }
+}
+
+#define REORDER_COLS 8
+template <class T>
+struct PermTab {
+  int nc;
+  const T *src[REORDER_COLS];
+  T *dst[REORDER_COLS];
+};
+// dst[k][p] = src[k][order[p]]: the staging copy of up to REORDER_COLS columns
+template <class T>
+__global__ void __launch_bounds__(256) k_permute_cols(int32_t n, const int32_t *__restrict__ order, PermTab<T> t) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int v = order[p];
+#pragma unroll
+  for (int k = 0; k < REORDER_COLS; k++)
+    if (k < t.nc) t.dst[k][p] = t.src[k][v];
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+template <class T>
+static int permute_columns(fvm_ctx *ctx, int32_t n, const int32_t *order, T **cols, int ncols, char *stage) {
+  for (int c0 = 0; c0 < ncols; c0 += REORDER_COLS) {
+    PermTab<T> t;
+    t.nc = std::min(REORDER_COLS, ncols - c0);
+    for (int k = 0; k < t.nc; k++) {
+      t.src[k] = cols[c0 + k];
+      t.dst[k] = (T *)(stage + align256((size_t)n * sizeof(T)) * k);
+    }
+    k_permute_cols<T><<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, order, t);
+    FVM_LAUNCH_CHECK(ctx);
+    for (int k = 0; k < t.nc; k++)
+      FVM_HIP(ctx, hipMemcpyAsync(cols[c0 + k], t.dst[k], (size_t)n * sizeof(T), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
+  }
+  return FVM_OK;
+}
+
+extern "C" int fvm_markers_reorder(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t *order) {
+  if (!ctx || !mk) return FVM_ERR_ARG;
+  FVM_TRY(fvm_grid_check(ctx, g));
+  const int32_t n = mk->nMark;
+  if (n <= 0) return FVM_OK;
+  FVM_TRY(fvm_sort_markers(ctx, g, mk));
+  FVM_PHASE(ctx, FVM_PHASE_SORT);
+  const int64_t nk = 4LL * fvm_ncx(g) * (g->NY - 1);
+  const int64_t tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+  const size_t b_flag = align256((size_t)n * 4), b_prefix = align256((size_t)(n + 1) * 4);
+  const size_t b_state = align256((size_t)(tiles + 2) * 8), b_order = align256((size_t)n * 4);
+  const size_t b_stage = align256((size_t)n * 8) * REORDER_COLS;
+  FVM_TRY(fvm_ws(ctx, ctx->reord, b_flag + b_prefix + b_state + b_order + b_stage));
+  char *w = (char *)ctx->reord.ptr;
+  int32_t *flag = (int32_t *)w, *prefix = (int32_t *)(w + b_flag);
+  unsigned long long *state = (unsigned long long *)(w + b_flag + b_prefix);
+  int32_t *ord = order ? order : (int32_t *)(w + b_flag + b_prefix + b_state);
+  char *stage = w + b_flag + b_prefix + b_state + b_order;
+  const int32_t *key = (const int32_t *)ctx->key.ptr, *perm = (const int32_t *)ctx->perm.ptr;
+  const int32_t *total = (const int32_t *)ctx->start.ptr + nk;
+  k_zero_counts<<<fvm_blocks(tiles + 2, 256), 256, 0, ctx->stream>>>(nullptr, 0, nullptr, state, tiles + 2);
+  FVM_LAUNCH_CHECK(ctx);
+  k_flag_unordered<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, flag);
+  FVM_LAUNCH_CHECK(ctx);
+  k_scan_onepass<<<(unsigned)tiles, SCAN_BS, 0, ctx->stream>>>(flag, prefix, state, n,
+                                                               (int32_t *)ctx->red.ptr + FVM_FLAG_SCAN_FAIL);
+  FVM_LAUNCH_CHECK(ctx);
+  k_reorder_index<<<fvm_blocks(n, 256), 256, 0, ctx->stream>>>(n, key, prefix, total, perm, ord);
+  FVM_LAUNCH_CHECK(ctx);
+  double *dc[FVM_NUM_DCOLS];
+  int32_t *ic[FVM_NUM_ICOLS];
+  fvm_marker_columns(mk, dc, ic);
+  FVM_TRY(permute_columns<double>(ctx, n, ord, dc, FVM_NUM_DCOLS, stage));
+  FVM_TRY(permute_columns<int32_t>(ctx, n, ord, ic, FVM_NUM_ICOLS, stage));
+  fvm_markers_moved(ctx);
   return FVM_OK;
 }
 

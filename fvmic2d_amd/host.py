@@ -81,6 +81,19 @@ class MarkerPath:
         self.call("fvm_find_marker_cells", self.grid.ref, self.markers.ref, I32(rank), C.byref(nout))
         return nout.value
 
+    def reorder_markers(self):
+        """fvm_markers_reorder: move the markers into cell order (the cells must
+        be current); returns the old index of the marker now in each slot."""
+        n = self.markers.n
+        if n == 0:
+            return np.zeros(0, np.int32)
+        d = self.ctx.alloc(4 * n)
+        try:
+            self.call("fvm_markers_reorder", self.grid.ref, self.markers.ref, C.c_void_p(d))
+            return self.ctx.dtoh(d, (n,), np.int32)
+        finally:
+            self.ctx.free(d)
+
     def project_marker_fields(self, entries):
         arr = self.projection_list(entries)
         self.call("fvm_project_marker_fields", self.grid.ref, self.markers.ref, I32(len(entries)),

@@ -82,6 +82,7 @@ _SIGS = {
     "fvm_markers_upload_aos": [P, P, P, I32],
     "fvm_markers_download_aos": [P, P, P],
     "fvm_find_marker_cells": [P, P, P, I32, C.POINTER(I32)],
+    "fvm_markers_reorder": [P, P, P, P],
     "fvm_project_marker_fields": [P, P, P, I32, P],
     "fvm_project_markers_nodes_all2": [P, P, P, P, P],
     "fvm_project_markers_nodes_from_scalar": [P, P, P, P, P],

@@ -144,6 +144,27 @@ int fvm_markers_download_aos(fvm_ctx *ctx, const fvm_markers *mk, Marker *aos);
 int fvm_find_marker_cells(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t rank,
                           int32_t *nOut);
 
+/* Store reorder.  No counterpart in the reference: a layout call for callers
+ * that do not need a marker to keep its index between steps (the reference's
+ * own marker exchange moves markers between slots: outgoing markers are
+ * flagged -1 in place and incoming ones take the first flagged slot or are
+ * appended, markers.c:160-170,218-226).  Advection scrambles the index order
+ * of the markers in space over many steps; every marker kernel then gathers
+ * from scattered lines (a randomly ordered store runs the thermal step about
+ * twice as long).  This call moves the markers back into cell order: located
+ * markers first, by (basic cell, quadrant) bucket -- the projections' bucket
+ * order, (cellY*ncx + cellX, quadrant), ascending index inside a bucket --
+ * then the unlocated ones (cellX == -1) by index.  Every column moves; nMark
+ * is unchanged.  order (device, nMark int32, may be NULL) receives the old
+ * index of the marker now in each slot.  The cells must be current (a
+ * fvm_find_marker_cells or whole step since the markers last moved), as for a
+ * projection.  Per-marker results are unchanged (each marker's arithmetic is
+ * its own); nodal sums stay within rounding of the unordered store's (their
+ * in-bucket order follows the new indices); population control, which picks
+ * by index, then picks among the new indices.  Costs about 0.8 KB of HBM
+ * traffic per marker. */
+int fvm_markers_reorder(fvm_ctx *ctx, const fvm_grid *g, fvm_markers *mk, int32_t *order);
+
 /* ---- A3/A4/A5: projection -------------------------------------------------- */
 /* One projection of markerProjection.c:71 projectMarkerFieldToNodes: a marker
  * column (markerField, device) or a material property table (materialProperty,
