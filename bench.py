@@ -207,6 +207,22 @@ def phase_profile(ctx, step, nsteps):
     return ms, la
 
 
+STORE_ORDER = "cell order: fvm_markers_reorder once before the timed steps (a driver reorders every few tens of steps)"
+
+
+def reorder_stores(paths):
+    """Put every path's marker store in cell order (fvm_markers_reorder after
+    locating the cells).  Returns the GPU time of the reorders in ms."""
+    ctx = paths[0].ctx
+    total = 0.0
+    for p in paths:
+        p.find_marker_cells()
+        ctx.timer_start()
+        p.call("fvm_markers_reorder", p.grid.ref, p.markers.ref, None)
+        total += ctx.timer_stop()
+    return total
+
+
 def run_ensemble(a):
     """The batched ensemble: 64 convection_tdep-style instances, grids 65^2..257^2
     (--ens-nmin/--ens-nmax), 16-36 markers per cell, viscosity contrast 1e2-1e6,
@@ -228,6 +244,21 @@ def run_ensemble(a):
     n0 = sum(p.markers.n for p in paths)
     nodes = sum(nx * ny for nx, ny in shapes)
 
+    # the batched step on the workloads' own marker order, then the stores in
+    # cell order for everything below
+    ens = Ensemble(paths)
+    ens.capture()
+    for _ in range(a.warmup):
+        ens.launch()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(max(3, a.steps // 2)):
+        ens.launch()
+    init_ms = ctx.timer_stop() / max(3, a.steps // 2)
+    reorder_ms = reorder_stores(paths)
+    log("ensemble: batched %.3f ms/step on the initial order; reorder of the 64 stores %.2f ms" % (
+        init_ms, reorder_ms))
+
     def seq_step():
         for p in paths:
             p.step_thermal()
@@ -240,7 +271,6 @@ def run_ensemble(a):
     seq_ms = ctx.timer_stop() / a.steps
     log("ensemble: sequential %.3f ms/step" % seq_ms)
 
-    ens = Ensemble(paths)
     (block, lanes, persistent), tuned_ms = ens.autotune(reps=3)
     log("ensemble: autotuned block=%d lanes=%d persistent=%d (%.3f ms)" % (block, lanes, persistent, tuned_ms))
 
@@ -282,7 +312,10 @@ def run_ensemble(a):
                    "grids": "%dx%d..%dx%d" % (min(s[0] for s in shapes), min(s[0] for s in shapes),
                                               max(s[0] for s in shapes), max(s[0] for s in shapes)),
                    "markers": n0, "markers_per_cell": "16-36", "viscosity_contrast": "1e2-1e6",
-                   "tuning": {"block": block, "lanes": lanes, "persistent": persistent}, "graph": True},
+                   "tuning": {"block": block, "lanes": lanes, "persistent": persistent}, "graph": True,
+                   "store": STORE_ORDER},
+        "initial_order": {"ms_per_step": init_ms, "markers_per_s": n0 / (init_ms / 1e3),
+                          "reorder_ms": reorder_ms, "reorder_ms_per_step_every_50_steps": reorder_ms / 50},
         "sequential": {"ms_per_step": seq_ms, "markers_per_s": active / (seq_ms / 1e3)},
         "speedup_vs_sequential": seq_ms / ens_ms,
         "launch_modes": modes,
@@ -308,6 +341,14 @@ def run_thermal(a):
     if a.lanes != 8:
         mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(a.lanes))
     n0 = w.n_markers
+    for _ in range(a.warmup):
+        mp.step_thermal()
+    mp.ctx.sync()
+    mp.ctx.timer_start()
+    for _ in range(max(5, a.steps // 2)):
+        mp.step_thermal()
+    init_ms = mp.ctx.timer_stop() / max(5, a.steps // 2)
+    reorder_ms = reorder_stores([mp])
     for _ in range(a.warmup):
         mp.step_thermal()
     mp.ctx.sync()
@@ -367,7 +408,8 @@ def run_thermal(a):
         "value": value, "unit": "markers/s", "steps": a.steps, "ms_per_step": ms / a.steps,
         "config": {"workload": "convection_tdep", "grid": "201x201 uniform", "markers": n0,
                    "markers_per_cell": 36, "active_markers_after": active, "projections": 17,
-                   "viscosity_contrast": 1e5},
+                   "viscosity_contrast": 1e5, "store": STORE_ORDER},
+        "initial_order": {"ms_per_step": init_ms, "markers_per_s": n0 / (init_ms / 1e3), "reorder_ms": reorder_ms},
         "roofline": roofline(dom, dom, ab[dom], per_launch_ms),
         "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
                  "wall_ms_per_step": wall / a.steps * 1e3,
@@ -601,6 +643,14 @@ def run_vep(a, extras=True):
     for _ in range(a.warmup):
         mp.step_vep()
     mp.ctx.sync()
+    mp.ctx.timer_start()
+    for _ in range(max(3, a.steps // 2)):
+        mp.step_vep()
+    init_ms = mp.ctx.timer_stop() / max(3, a.steps // 2)
+    reorder_ms = reorder_stores([mp])
+    for _ in range(a.warmup):
+        mp.step_vep()
+    mp.ctx.sync()
     t0 = time.perf_counter()
     mp.ctx.timer_start()
     for _ in range(a.steps):
@@ -620,7 +670,9 @@ def run_vep(a, extras=True):
         "value": value, "unit": "markers/s", "steps": a.steps, "ms_per_step": ms / a.steps,
         "config": {"workload": "vep_subduction", "grid": "801x401 tanh-stretched", "markers": n0,
                    "markers_per_cell": 25, "active_markers_after": active, "materials": 5,
-                   "plasticity": "Drucker-Prager", "yielding_last_step": y, "residual_last_step": r},
+                   "plasticity": "Drucker-Prager", "yielding_last_step": y, "residual_last_step": r,
+                   "store": STORE_ORDER},
+        "initial_order": {"ms_per_step": init_ms, "markers_per_s": n0 / (init_ms / 1e3), "reorder_ms": reorder_ms},
         "roofline": roofline(dom, dom + "_vep", ab[dom], cands[dom]),
         "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
                  "wall_ms_per_step": wall / a.steps * 1e3,
