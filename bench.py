@@ -223,6 +223,26 @@ def reorder_stores(paths):
     return total
 
 
+def time_steps(ctx, step, warmup, steps):
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    ctx.timer_start()
+    for _ in range(steps):
+        step()
+    return ctx.timer_stop() / steps
+
+
+def initial_order(a, ctx, step, n0, steps, paths):
+    """The step on the workloads' own marker order (skipped with --no-extras),
+    then the stores put in cell order for the timed steps."""
+    init = None if a.no_extras else time_steps(ctx, step, a.warmup, steps)
+    out = {"reorder_ms": reorder_stores(paths)}
+    if init is not None:
+        out.update(ms_per_step=init, markers_per_s=n0 / (init / 1e3))
+    return out
+
+
 def run_ensemble(a):
     """The batched ensemble: 64 convection_tdep-style instances, grids 65^2..257^2
     (--ens-nmin/--ens-nmax), 16-36 markers per cell, viscosity contrast 1e2-1e6,
@@ -248,16 +268,9 @@ def run_ensemble(a):
     # cell order for everything below
     ens = Ensemble(paths)
     ens.capture()
-    for _ in range(a.warmup):
-        ens.launch()
-    ctx.sync()
-    ctx.timer_start()
-    for _ in range(max(3, a.steps // 2)):
-        ens.launch()
-    init_ms = ctx.timer_stop() / max(3, a.steps // 2)
-    reorder_ms = reorder_stores(paths)
-    log("ensemble: batched %.3f ms/step on the initial order; reorder of the 64 stores %.2f ms" % (
-        init_ms, reorder_ms))
+    init = initial_order(a, ctx, ens.launch, n0, max(3, a.steps // 2), paths)
+    init["reorder_ms_per_step_every_50_steps"] = init["reorder_ms"] / 50
+    log("ensemble: initial order %s" % init)
 
     def seq_step():
         for p in paths:
@@ -314,8 +327,7 @@ def run_ensemble(a):
                    "markers": n0, "markers_per_cell": "16-36", "viscosity_contrast": "1e2-1e6",
                    "tuning": {"block": block, "lanes": lanes, "persistent": persistent}, "graph": True,
                    "store": STORE_ORDER},
-        "initial_order": {"ms_per_step": init_ms, "markers_per_s": n0 / (init_ms / 1e3),
-                          "reorder_ms": reorder_ms, "reorder_ms_per_step_every_50_steps": reorder_ms / 50},
+        "initial_order": init,
         "sequential": {"ms_per_step": seq_ms, "markers_per_s": active / (seq_ms / 1e3)},
         "speedup_vs_sequential": seq_ms / ens_ms,
         "launch_modes": modes,
@@ -343,12 +355,7 @@ def run_thermal(a):
     n0 = w.n_markers
     for _ in range(a.warmup):
         mp.step_thermal()
-    mp.ctx.sync()
-    mp.ctx.timer_start()
-    for _ in range(max(5, a.steps // 2)):
-        mp.step_thermal()
-    init_ms = mp.ctx.timer_stop() / max(5, a.steps // 2)
-    reorder_ms = reorder_stores([mp])
+    init = initial_order(a, mp.ctx, mp.step_thermal, n0, max(5, a.steps // 2), [mp])
     for _ in range(a.warmup):
         mp.step_thermal()
     mp.ctx.sync()
@@ -409,7 +416,7 @@ def run_thermal(a):
         "config": {"workload": "convection_tdep", "grid": "201x201 uniform", "markers": n0,
                    "markers_per_cell": 36, "active_markers_after": active, "projections": 17,
                    "viscosity_contrast": 1e5, "store": STORE_ORDER},
-        "initial_order": {"ms_per_step": init_ms, "markers_per_s": n0 / (init_ms / 1e3), "reorder_ms": reorder_ms},
+        "initial_order": init,
         "roofline": roofline(dom, dom, ab[dom], per_launch_ms),
         "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
                  "wall_ms_per_step": wall / a.steps * 1e3,
@@ -642,12 +649,7 @@ def run_vep(a, extras=True):
     n0 = w.n_markers
     for _ in range(a.warmup):
         mp.step_vep()
-    mp.ctx.sync()
-    mp.ctx.timer_start()
-    for _ in range(max(3, a.steps // 2)):
-        mp.step_vep()
-    init_ms = mp.ctx.timer_stop() / max(3, a.steps // 2)
-    reorder_ms = reorder_stores([mp])
+    init = initial_order(a, mp.ctx, mp.step_vep, n0, max(3, a.steps // 2), [mp])
     for _ in range(a.warmup):
         mp.step_vep()
     mp.ctx.sync()
@@ -672,7 +674,7 @@ def run_vep(a, extras=True):
                    "markers_per_cell": 25, "active_markers_after": active, "materials": 5,
                    "plasticity": "Drucker-Prager", "yielding_last_step": y, "residual_last_step": r,
                    "store": STORE_ORDER},
-        "initial_order": {"ms_per_step": init_ms, "markers_per_s": n0 / (init_ms / 1e3), "reorder_ms": reorder_ms},
+        "initial_order": init,
         "roofline": roofline(dom, dom + "_vep", ab[dom], cands[dom]),
         "step": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (ms / a.steps / 1e3) / 1e9,
                  "wall_ms_per_step": wall / a.steps * 1e3,
@@ -834,7 +836,8 @@ def main():
     ap.add_argument("--ens-nmin", type=int, default=65)
     ap.add_argument("--ens-nmax", type=int, default=257)
     ap.add_argument("--no-extras", action="store_true",
-                    help="default line: the ensemble only (no single-instance or learned-guess keys)")
+                    help="default line: the ensemble only (no single-instance or learned-guess keys); every "
+                         "line: no initial-order timing and no scrambled-store run (for profiling)")
     ap.add_argument("--thermal", action="store_true",
                     help="the single-instance convection_tdep step as the line (201x201, 1.44M markers)")
     ap.add_argument("--ensemble", action="store_true", help="the ensemble line only (no extra keys)")
@@ -874,7 +877,7 @@ def main():
         cpu_all = None if a.no_cpu or a.vep else cpu_all_cores("tdep", a.cpu_budget)
         out = run_thermal(a) if a.thermal else run_vep(a)
         line = dict(common, **out)
-        if a.thermal:
+        if a.thermal and not a.no_extras:
             line["store_order"] = run_store_order(a)
         if a.thermal and not a.no_cpu:
             line["cpu_baseline"] = cpu_baseline_tdep(a.cpu_budget)

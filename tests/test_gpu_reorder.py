@@ -114,3 +114,30 @@ def test_steps_with_reorder_follow_oracle():
     assert_close(g["T"][act], r["T"][act], "T after 20 steps", rtol=1e-10)
     for n in ("rho", "etaN", "kThermal", "lastT", "Cp"):
         assert_close(mp.get_nodal(n), st.nodal[n], n, rtol=1e-10)
+
+
+def test_ensemble_members_reordered_between_steps():
+    """Ensemble members are the callers' stores: reordered in place between
+    batched steps (cells located first), each member stays bitwise equal to the
+    same problem stepped alone and reordered the same way."""
+    from fvmic2d_amd.host import Ensemble
+    from fvmic2d_amd.lib import Context
+    print("CASE ensemble with reorder, 4 instances", flush=True)
+    ctx = Context()
+    ws = [_scrambled(w, seed=20 + i) for i, w in enumerate(workloads.ensemble(4, seed=9, nmin=17, nmax=61))]
+    alone, batch = [MarkerPath(w, ctx=ctx) for w in ws], [MarkerPath(w, ctx=ctx) for w in ws]
+    ens = Ensemble(batch)
+    for s in range(3):
+        for a, b in zip(alone, batch):
+            for p in (a, b):
+                p.find_marker_cells()
+            assert_bitwise(b.reorder_markers(), a.reorder_markers(), "order %d" % s)
+            a.step_thermal()
+        ens.step()
+        for i, (a, b) in enumerate(zip(alone, batch)):
+            ma, mb = a.get_markers(), b.get_markers()
+            for f in ("X", "Y", "VX", "VY", "T", "Tdot", "rho", "wxy", "cellX", "cellY"):
+                assert_bitwise(mb[f], ma[f], "step %d instance %d %s" % (s, i, f))
+            for o in [p[4] for p in a.w.projections]:
+                assert_bitwise(b.get_nodal(o), a.get_nodal(o), "step %d instance %d %s" % (s, i, o))
+    ens.close()
