@@ -157,3 +157,34 @@ def test_hundred_steps_with_population_control():
         assert_bitwise(g[f], r[f], f)
     act = r["cellX"] != -1
     assert_close(g["T"][act], r["T"][act], "T after 100 steps", rtol=1e-10)
+
+
+@pytest.mark.timeout(300)
+def test_population_control_with_reorder():
+    """60 thermal steps with population control every step and the store
+    reordered every 10 steps (fvm_markers_reorder); the oracle's markers are
+    permuted by the same order, so every later choice by index (removal, hole
+    recycling, inheritance) must still be the oracle's."""
+    print("CASE population control + reorder, 60 thermal steps", flush=True)
+    w = workloads.convection_small()
+    _options(w)
+    st, mp = _pair(w, 2 * w.n_markers)
+    for step in range(60):
+        for s in (st, mp):
+            s.find_marker_cells()
+        if step % 10 == 5:
+            order = mp.reorder_markers()
+            n = st.ms.nMark
+            assert len(order) == n
+            st.markers[:n] = st.markers[:n][order]
+        ra = st.check_marker_density(seed=500 + step)
+        ga = mp.check_marker_density(seed=500 + step)
+        assert ra == ga, (step, ra, ga)
+        st.step_thermal()
+        mp.step_thermal()
+    g, r = mp.get_markers(), st.markers[:st.ms.nMark]
+    assert len(g) == len(r)
+    for f in ("cellX", "cellY", "Mat", "X", "Y", "VX", "VY"):
+        assert_bitwise(g[f], r[f], f)
+    act = r["cellX"] != -1
+    assert_close(g["T"][act], r["T"][act], "T after 60 steps", rtol=1e-10)
