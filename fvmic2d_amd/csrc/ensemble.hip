@@ -217,7 +217,8 @@ __device__ __forceinline__ void ens_scatter_block(const InstDev &I, int64_t lb, 
 
 // Bucket order of basic cell (cx, cy) of instance ii by the calling wavefront,
 // with the payload gathered from the instance's columns.
-__device__ __forceinline__ void ens_order_cell(const InstDev &I, int ii, int n, int64_t c,
+// Two cells of one instance (c1 < 0: none), as bucket_order_pair.
+__device__ __forceinline__ void ens_order_pair(const InstDev &I, int ii, int n, int64_t c0, int64_t c1,
                                                const int32_t *__restrict__ start, int32_t *__restrict__ perm,
                                                double *__restrict__ xs, double *__restrict__ ys,
                                                int32_t *__restrict__ ms, int ncol,
@@ -225,7 +226,7 @@ __device__ __forceinline__ void ens_order_cell(const InstDev &I, int ii, int n, 
                                                int64_t nmark) {
   const double *__restrict__ X = I.X, *__restrict__ Y = I.Y;
   const int32_t *__restrict__ Mat = I.Mat;
-  bucket_order_cell(c, start, perm, [&](int p, int v) {
+  bucket_order_pair(c0, c1, start, perm, [&](int p, int v) {
     xs[p] = X[v];
     ys[p] = Y[v];
     ms[p] = Mat[v];
@@ -346,18 +347,19 @@ __global__ void __launch_bounds__(256) kb_order_gather(const InstDev *__restrict
   // its column pointers are read with scalar loads
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wave;
-  const int64_t tile = w >> 6;
+  const int64_t tile = w >> 5;
   if (tile >= tile_off[n]) return;  // whole wavefront exits together
   const int ii = tile_inst[tile];
   const InstDev &I = insts[ii];
   const int ncx = I.g.NX - 1, ncy = I.g.NY - 1;
   const uint32_t ntx = (uint32_t)(ncx + 7) >> 3;
   const uint32_t tl = (uint32_t)(tile - tile_off[ii]);
-  const int l = (int)(w & 63);
+  // cells l and l + 1 of the tile: one above the other
+  const int l = (int)(w & 31) * 2;
   const int cx = (int)(tl % ntx) * 8 + (l >> 3), cy = (int)(tl / ntx) * 8 + (l & 7);
   if (cx >= ncx || cy >= ncy) return;
-  ens_order_cell(I, ii, n, cell_off[ii] + (int64_t)cy * ncx + cx, start, perm, xs, ys, ms, ncol, colsrc, pcols,
-                 nmark);
+  const int64_t c0 = cell_off[ii] + (int64_t)cy * ncx + cx;
+  ens_order_pair(I, ii, n, c0, cy + 1 < ncy ? c0 + ncx : -1, start, perm, xs, ys, ms, ncol, colsrc, pcols, nmark);
 }
 
 // Every group of one (method, stored field count) in one launch over the
@@ -592,17 +594,20 @@ kb_persist(const PersistArgs *__restrict__ Ap) {
           for (int r = 0; r < nrun; r++) ens_scatter_block(I, lb + r, A.key, A.slot, A.start, A.perm);
           break;
         case PS_ORDER: {
-          // each tile's 64 cells, 16 per wavefront (cell l at (l >> 3, l & 7) in the tile)
+          // each tile's 64 cells, 16 per wavefront in pairs (cell l at (l >> 3, l & 7)
+          // in the tile, with cell l + 1 above it)
           const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
           const int ncx = I.g.NX - 1, ncy = I.g.NY - 1;
           const uint32_t ntx = (uint32_t)(ncx + 7) >> 3;
           for (int r = 0; r < nrun; r++) {
             const uint32_t t = (uint32_t)(lb + r);
-            for (int l = wave; l < 64; l += 4) {
+            for (int l = 2 * wave; l < 64; l += 8) {
               const int cx = (int)(t % ntx) * 8 + (l >> 3), cy = (int)(t / ntx) * 8 + (l & 7);
-              if (cx < ncx && cy < ncy)
-                ens_order_cell(I, ii, A.n, A.cell_off[ii] + (int64_t)cy * ncx + cx, A.start, A.perm, A.xs, A.ys,
-                               A.ms, A.ncol, A.colsrc, A.pcols, A.nmark);
+              if (cx < ncx && cy < ncy) {
+                const int64_t c0 = A.cell_off[ii] + (int64_t)cy * ncx + cx;
+                ens_order_pair(I, ii, A.n, c0, cy + 1 < ncy ? c0 + ncx : -1, A.start, A.perm, A.xs, A.ys, A.ms,
+                               A.ncol, A.colsrc, A.pcols, A.nmark);
+              }
             }
           }
           break;
@@ -992,7 +997,7 @@ static int ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
   }
   {
     FVM_PHASE(ctx, FVM_PHASE_SORT);
-    kb_order_gather<<<fvm_blocks(E->ntile * 64 * 64, 256), 256, 0, ctx->stream>>>(
+    kb_order_gather<<<fvm_blocks(E->ntile * 32 * 64, 256), 256, 0, ctx->stream>>>(
         E->dinst, n, E->dcell_off, E->dcell_off + n + 1, E->dtile_inst, E->start, E->perm, E->xs, E->ys, E->ms, E->ncol,
         E->dcolsrc, E->pcols, E->nmark);
     FVM_LAUNCH_CHECK(ctx);

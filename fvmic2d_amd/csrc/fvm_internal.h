@@ -355,6 +355,70 @@ __device__ __forceinline__ void bucket_order_cell(int64_t c, const int32_t *__re
   }
 }
 
+// Two basic cells per wavefront: c0 in lanes 0-31, c1 in lanes 32-63 (c1 < 0:
+// none).  When both cells hold at most 32 entries (16-25 markers per cell are
+// common) each half ranks its cell's packed (bucket, marker) keys against its
+// own half, so one wavefront does the work of two; otherwise (or for marker
+// indices >= 2^30) the cells go through bucket_order_cell one after the other.
+// The order is the same either way: the cell's entries by (bucket, marker).
+template <class Emit>
+__device__ __forceinline__ void bucket_order_pair(int64_t c0, int64_t c1, const int32_t *__restrict__ start,
+                                                  int32_t *__restrict__ perm, Emit &&emit) {
+  c0 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(c0 >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)c0));
+  c1 = (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(c1 >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)c1));
+  if (c1 < 0) {
+    if (c0 >= 0) bucket_order_cell(c0, start, perm, emit);
+    return;
+  }
+  if (c0 < 0) {
+    bucket_order_cell(c1, start, perm, emit);
+    return;
+  }
+  const int a0 = start[4 * c0], a4 = start[4 * c0 + 4];
+  const int d0 = start[4 * c1], d4 = start[4 * c1 + 4];
+  const int n0 = __builtin_amdgcn_readfirstlane(a4 - a0), n1 = __builtin_amdgcn_readfirstlane(d4 - d0);
+  if (n0 <= 32 && n1 <= 32) {
+    const int lane = threadIdx.x & 63;
+    const bool hi = lane >= 32;
+    const int hl = lane & 31;
+    const int64_t c = hi ? c1 : c0;
+    const int b0 = start[4 * c], b1 = start[4 * c + 1], b2 = start[4 * c + 2], b3 = start[4 * c + 3];
+    const int n = hi ? n1 : n0;
+    const int pos = b0 + hl;
+    const bool have = hl < n;
+    const int v = have ? perm[pos] : 0x7fffffff;
+    const int bk = (pos >= b1) + (pos >= b2) + (pos >= b3);
+    if (__ballot(have && v >= (1 << 30)) == 0) {
+      const uint32_t key = have ? ((uint32_t)bk << 30) | (uint32_t)v : 0xffffffffu;
+      // each half counts the smaller keys of its own half: lanes j and 32 + j
+      // are read together (j + 3 <= 31, so 32 + j + 3 stays a lane index)
+      const int nmax = n0 > n1 ? n0 : n1;
+      int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+      for (int j = 0; j < nmax; j += 4) {
+        const uint32_t l0 = (uint32_t)read_lane((int)key, j), h0 = (uint32_t)read_lane((int)key, 32 + j);
+        const uint32_t l1 = (uint32_t)read_lane((int)key, j + 1), h1 = (uint32_t)read_lane((int)key, 33 + j);
+        const uint32_t l2 = (uint32_t)read_lane((int)key, j + 2), h2 = (uint32_t)read_lane((int)key, 34 + j);
+        const uint32_t l3 = (uint32_t)read_lane((int)key, j + 3), h3 = (uint32_t)read_lane((int)key, 35 + j);
+        r0 += (hi ? h0 : l0) < key ? 1 : 0;
+        r1 += (hi ? h1 : l1) < key ? 1 : 0;
+        r2 += (hi ? h2 : l2) < key ? 1 : 0;
+        r3 += (hi ? h3 : l3) < key ? 1 : 0;
+      }
+      const int p = b0 + ((r0 + r1) + (r2 + r3));
+      if (have) {
+        perm[p] = v;
+        emit(p, v);
+      }
+      return;
+    }
+    // a marker index >= 2^30: nothing written yet, order the cells one by one
+  }
+  bucket_order_cell(c0, start, perm, emit);
+  bucket_order_cell(c1, start, perm, emit);
+}
+
 struct SortGather {
   int with_xy;
   const double *X, *Y;

@@ -155,29 +155,33 @@ __global__ void __launch_bounds__(256) k_scatter(int32_t n, const int32_t *__res
   if ((uint32_t)p < (uint32_t)n) perm[p] = (int32_t)m;  // always true for a valid scan
 }
 
-// Put every bucket in ascending marker order (bucket_order_cell, one
-// wavefront per basic cell).  With GATHER the payload columns of each entry
-// are copied to its final position as well, and the cells are visited in
-// 8 x 8 tiles (y fastest inside a tile), tiles spread over the XCDs in
-// contiguous runs, so that the marker rows a tile's wavefronts read share L2
-// lines.
+// Put every bucket in ascending marker order (bucket_order_pair: two basic
+// cells per wavefront).  With GATHER the payload columns of each entry are
+// copied to its final position as well, and the cells are visited in 8 x 8
+// tiles (y fastest inside a tile; a wavefront takes two cells one above the
+// other), tiles spread over the XCDs in contiguous runs, so that the marker
+// rows a tile's wavefronts read share L2 lines.
 template <bool GATHER>
 __global__ void __launch_bounds__(256) k_bucket_order(int32_t ncell, int32_t ncx, int32_t ncy,
                                                       const int32_t *__restrict__ start, int32_t *__restrict__ perm,
                                                       SortGather sg) {
-  int64_t c;
+  int64_t c0, c1;
   if (GATHER) {
     const int64_t w = xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const uint32_t ntx = (uint32_t)(ncx + 7) >> 3;
-    const uint32_t tile = (uint32_t)(w >> 6);
-    const int l = (int)(w & 63);
+    const uint32_t tile = (uint32_t)(w >> 5);
+    const int l = (int)(w & 31) * 2;
     const int cx = (int)(tile % ntx) * 8 + (l >> 3), cy = (int)(tile / ntx) * 8 + (l & 7);
-    c = (cx < ncx && cy < ncy) ? (int64_t)cy * ncx + cx : ncell;
+    const bool in = cx < ncx && tile / ntx < (uint32_t)((ncy + 7) >> 3);
+    c0 = (in && cy < ncy) ? (int64_t)cy * ncx + cx : -1;
+    c1 = (in && cy + 1 < ncy) ? (int64_t)(cy + 1) * ncx + cx : -1;
   } else {
-    c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    c0 = 2 * w < ncell ? 2 * w : -1;
+    c1 = 2 * w + 1 < ncell ? 2 * w + 1 : -1;
   }
-  if (c >= ncell) return;  // whole wavefront exits together
-  bucket_order_cell(c, start, perm, [&](int p, int v) {
+  if (c0 < 0 && c1 < 0) return;  // whole wavefront exits together
+  bucket_order_pair(c0, c1, start, perm, [&](int p, int v) {
     if (GATHER) gather_entry(sg.pc, sg.with_xy, sg.X, sg.Y, sg.Mat, sg.xs, sg.ys, sg.ms, p, v);
   });
 }
@@ -230,13 +234,13 @@ static int finish_sort(fvm_ctx *ctx, const fvm_grid *g, const fvm_markers *mk, c
   if (sg && n > 0) {
     const int32_t ncx = fvm_ncx(g), ncy = g->NY - 1;
     const int64_t ntiles = (int64_t)((ncx + 7) / 8) * ((ncy + 7) / 8);
-    k_bucket_order<true><<<fvm_blocks(ntiles * 64 * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, ncx, ncy, start,
+    k_bucket_order<true><<<fvm_blocks(ntiles * 32 * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, ncx, ncy, start,
                                                                                    perm, *sg);
   } else {
     SortGather none;
     memset(&none, 0, sizeof(none));
-    k_bucket_order<false><<<fvm_blocks(ncell * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, 0, 0, start, perm,
-                                                                                none);
+    k_bucket_order<false><<<fvm_blocks((ncell + 1) / 2 * 64, 256), 256, 0, ctx->stream>>>((int32_t)ncell, 0, 0, start,
+                                                                                          perm, none);
   }
   FVM_LAUNCH_CHECK(ctx);
   ctx->sort_epoch = ctx->epoch;
