@@ -303,7 +303,7 @@ def run_ensemble(a):
     # both launch modes at the tuned lane count, for the record; the per-phase
     # mode last, for the phase profile
     modes = {}
-    ens.set_tuning(256, lanes, 1)
+    ens.set_tuning(256, max(lanes, 8), 1)  # the persistent step has no 4-lane form
     ens.capture()
     modes["persistent_ms_per_step"] = timed(max(3, a.steps // 4))
     ens.set_tuning(256 if persistent else block, lanes, 0)

@@ -121,7 +121,8 @@ extern "C" int fvm_ctx_set_jaumann(fvm_ctx *ctx, int32_t jaumann) {
 
 extern "C" int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes) {
   if (!ctx) return FVM_ERR_ARG;
-  if (lanes != 8 && lanes != 16 && lanes != 32) return fvm_set_err(ctx, FVM_ERR_ARG, "lanes must be 8, 16 or 32");
+  if (lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32)
+    return fvm_set_err(ctx, FVM_ERR_ARG, "lanes must be 4, 8, 16 or 32");
   ctx->lanes = lanes;
   return FVM_OK;
 }

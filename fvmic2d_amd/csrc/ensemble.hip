@@ -881,6 +881,10 @@ static int launch_kb_partials(fvm_ctx *ctx, fvm_ensemble *E, const KbGroups &kg,
   const int64_t *ub = blk_row(E, u);
   const int32_t *ubi = bi_row(E, u);
   switch (E->tune.lanes) {
+    case 4:
+      kb_partials<METHOD, NF, 4><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, ub, ubi, E->dgroups_sorted, E->dpartoff,
+                                                                E->start, E->xs, E->ys, E->ms, E->part, kg, nrow);
+      break;
     case 8:
       kb_partials<METHOD, NF, 8><<<grid, 256, 0, ctx->stream>>>(E->dinst, n, ub, ubi, E->dgroups_sorted, E->dpartoff,
                                                                 E->start, E->xs, E->ys, E->ms, E->part, kg, nrow);
@@ -946,6 +950,10 @@ static int launch_kb_partials_subt(fvm_ctx *ctx, fvm_ensemble *E) {
   const int32_t *bi = bi_row(E, 2 + gi);
   FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
   switch (E->tune.lanes) {
+    case 4:
+      kb_partials_subt<4><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
+                                                                   E->xs, E->ys, mt, E->part, E->dTsub);
+      break;
     case 8:
       kb_partials_subt<8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
                                                                    E->xs, E->ys, mt, E->part, E->dTsub);
@@ -1306,9 +1314,11 @@ extern "C" int fvm_ensemble_step(fvm_ctx *ctx, fvm_ensemble *E) {
 
 extern "C" int fvm_ensemble_set_tuning(fvm_ctx *ctx, fvm_ensemble *E, const fvm_tuning *t) {
   if (!ctx || !E || !t) return FVM_ERR_ARG;
-  if ((t->block != 128 && t->block != 256 && t->block != 512) || (t->lanes != 8 && t->lanes != 16 && t->lanes != 32) ||
-      (t->persistent != 0 && t->persistent != 1))
-    return fvm_set_err(ctx, FVM_ERR_ARG, "tuning: block in {128,256,512}, lanes in {8,16,32}, persistent 0 or 1");
+  if ((t->block != 128 && t->block != 256 && t->block != 512) ||
+      (t->lanes != 4 && t->lanes != 8 && t->lanes != 16 && t->lanes != 32) || (t->persistent != 0 && t->persistent != 1))
+    return fvm_set_err(ctx, FVM_ERR_ARG, "tuning: block in {128,256,512}, lanes in {4,8,16,32}, persistent 0 or 1");
+  if (t->persistent && t->lanes == 4)
+    return fvm_set_err(ctx, FVM_ERR_ARG, "tuning: the persistent step takes lanes 8, 16 or 32");
   if (t->persistent && (t->block != 256 || E->persist_nf == 0))
     return fvm_set_err(ctx, FVM_ERR_ARG,
                        "tuning: the persistent step needs block 256 and one ARITHMETIC projection kind of <= 5 fields");
@@ -1372,13 +1382,13 @@ extern "C" int fvm_ensemble_launch(fvm_ctx *ctx, fvm_ensemble *E) {
 extern "C" int fvm_ensemble_autotune(fvm_ctx *ctx, fvm_ensemble *E, int32_t reps, fvm_tuning *best,
                                      float *ms_per_step) {
   if (!ctx || !E || reps < 1) return FVM_ERR_ARG;
-  const int blocks[4] = {128, 256, 512, 256}, lanes[3] = {8, 16, 32};
+  const int blocks[4] = {128, 256, 512, 256}, lanes[4] = {8, 16, 32, 4};
   fvm_tuning bt = E->tune;
   float bms = 1e30f;
-  // per-phase launches with every block size, then the persistent step (if the
-  // projection list allows it)
+  // per-phase launches with every block size and lane count, then the
+  // persistent step (if the projection list allows it; not with 4 lanes)
   for (int bi = 0; bi < (E->persist_nf ? 4 : 3); bi++)
-    for (int li = 0; li < 3; li++) {
+    for (int li = 0; li < (bi == 3 ? 3 : 4); li++) {
       fvm_tuning t = {blocks[bi], lanes[li], bi == 3 ? 1 : 0};
       FVM_TRY(fvm_ensemble_set_tuning(ctx, E, &t));
       FVM_TRY(fvm_ensemble_capture(ctx, E));

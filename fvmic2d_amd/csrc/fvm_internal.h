@@ -38,7 +38,7 @@ struct fvm_ctx {
   size_t payload_stride = 0;
   uint64_t payload_serial = 0;
   bool reuse_payload = false;
-  int lanes = 8;  // lanes per lattice cell in the projection partials (8, 16 or 32)
+  int lanes = 8;  // lanes per lattice cell in the projection partials (4, 8, 16 or 32)
   int jaumann = 0;  // stress rotation of the whole-step entry points
   fvm_buf pinned;  // small pinned host buffer for scalar results
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -298,12 +298,17 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
     }
   }
   // fixed butterfly over the G lanes of the group (group_step)
-  static_assert(G == 8 || G == 16 || G == 32, "lane groups of 8, 16 or 32");
+  static_assert(G == 4 || G == 8 || G == 16 || G == 32, "lane groups of 4, 8, 16 or 32");
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    sw[c] = group_step<2>(group_step<1>(group_step<0>(sw[c])));
+    sw[c] = group_step<1>(group_step<0>(sw[c]));
 #pragma unroll
-    for (int k = 0; k < NF; k++) sf[c][k] = group_step<2>(group_step<1>(group_step<0>(sf[c][k])));
+    for (int k = 0; k < NF; k++) sf[c][k] = group_step<1>(group_step<0>(sf[c][k]));
+    if constexpr (G >= 8) {
+      sw[c] = group_step<2>(sw[c]);
+#pragma unroll
+      for (int k = 0; k < NF; k++) sf[c][k] = group_step<2>(sf[c][k]);
+    }
     if constexpr (G >= 16) {
       sw[c] = group_step<3>(sw[c]);
 #pragma unroll

@@ -18,7 +18,7 @@
 #include "proj_common.h"
 
 // Lanes per lattice cell (G) of the single-instance launches: 8 by default,
-// 16 or 32 through fvm_ctx_set_lanes.
+// 4, 16 or 32 through fvm_ctx_set_lanes.
 
 template <int METHOD, int NF, int G>
 __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const int32_t *__restrict__ start,
@@ -124,7 +124,8 @@ static int launch_partials(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, c
 #define LAUNCH_P(GG)                                                                                     \
   k_partials<METHOD, NF, GG><<<fvm_blocks(nL * GG, 256), 256, 0, ctx->stream>>>(                          \
       dgrid(g), pg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms, part, Lx.n, Ly.n, Lx.lo, Ly.lo)
-  if (G == 8) LAUNCH_P(8);
+  if (G == 4) LAUNCH_P(4);
+  else if (G == 8) LAUNCH_P(8);
   else if (G == 32) LAUNCH_P(32);
   else LAUNCH_P(16);
 #undef LAUNCH_P
@@ -141,7 +142,8 @@ static int launch_multi(fvm_ctx *ctx, const fvm_grid *g, const Payload &pl, cons
   k_partials_multi<METHOD, NF, GG><<<(unsigned)fvm_blocks(nT * GG, 256) * (unsigned)mg.ng, 256, 0,              \
                                      ctx->stream>>>(                                                       \
       dgrid(g), mg, (const int32_t *)ctx->start.ptr, pl.xs, pl.ys, pl.ms)
-  if (G == 8) LAUNCH_M(8);
+  if (G == 4) LAUNCH_M(4);
+  else if (G == 8) LAUNCH_M(8);
   else if (G == 32) LAUNCH_M(32);
   else LAUNCH_M(16);
 #undef LAUNCH_M
@@ -199,7 +201,8 @@ static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const Payload &mk, c
   k_partials_subt<GG><<<fvm_blocks(nL * GG, 256), 256, 0, ctx->stream>>>(                                 \
       dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk.xs, mk.ys, part, \
       Lx.n, Ly.n)
-    if (G == 8) LAUNCH_S(8);
+    if (G == 4) LAUNCH_S(4);
+  else if (G == 8) LAUNCH_S(8);
     else if (G == 32) LAUNCH_S(32);
     else LAUNCH_S(16);
 #undef LAUNCH_S

@@ -100,7 +100,7 @@ void *fvm_ctx_stream(fvm_ctx *ctx);
 int fvm_ctx_sync(fvm_ctx *ctx);
 /* Bytes of device workspace the context currently holds (sort buffers, partials). */
 size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
-/* Lanes per lattice cell in the projection passes: 8 (default), 16 or 32.  The
+/* Lanes per lattice cell in the projection passes: 8 (default), 4, 16 or 32.  The
  * lane count fixes the summation tree, so results are bitwise reproducible for
  * a given setting and agree across settings to the projection tolerance. */
 int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes);
@@ -459,7 +459,8 @@ typedef struct {
 } fvm_instance;
 
 /* Launch shape: threads per block {128, 256, 512}; lanes per lattice cell in
- * the projection pass {8, 16, 32}; persistent 0: one launch per phase,
+ * the projection pass {4, 8, 16, 32} (4: per-phase launches only); persistent
+ * 0: one launch per phase,
  * 1: the whole step as one launch of resident workgroups that take work items
  * (blocks of every phase of every instance) from a ticket, an item waiting
  * only for the previous phase of its own instance (needs block 256 and a

@@ -137,7 +137,7 @@ def test_ensemble_full_size_equals_single_instance():
     ens.close()
 
 
-@pytest.mark.parametrize("block,lanes", [(128, 8), (256, 16), (512, 32), (128, 32), (512, 8)])
+@pytest.mark.parametrize("block,lanes", [(128, 8), (256, 16), (512, 32), (128, 32), (512, 8), (256, 4)])
 def test_ensemble_vs_oracle(block, lanes):
     print("CASE ensemble vs oracle block=%d lanes=%d" % (block, lanes), flush=True)
     ctx = Context()
@@ -180,16 +180,16 @@ def test_ensemble_autotune_keeps_results_valid():
     paths = _paths(ws, ctx)
     ens = Ensemble(paths)
     (block, lanes, pers), ms = ens.autotune(reps=1)
-    assert block in (128, 256, 512) and lanes in (8, 16, 32) and pers in (0, 1) and ms > 0.0
+    assert block in (128, 256, 512) and lanes in (4, 8, 16, 32) and pers in (0, 1) and ms > 0.0
     assert ens.get_tuning() == (block, lanes, pers)
-    # 9 per-phase shapes and 3 persistent ones x 2 launches + 1 more: positions
+    # 12 per-phase shapes and 3 persistent ones x 2 launches + 1 more: positions
     # stay bit-identical to the oracle
     ens.launch()
     ctx.sync()
     assert ens.counters()[1] == 0
     for i, (w, mp) in enumerate(zip(ws, paths)):
         st = oracle.State(w, extra=_outs(w))
-        for _ in range(25):
+        for _ in range(31):
             st.step_thermal()
         g = mp.get_markers()
         for f in ("X", "Y", "cellX", "cellY"):
@@ -210,6 +210,8 @@ def test_ensemble_rejects_bad_input():
         ens.set_tuning(128, 8, 1)
     with pytest.raises(FvmError):
         ens.set_tuning(256, 8, 2)
+    with pytest.raises(FvmError):  # the persistent step has no 4-lane form
+        ens.set_tuning(256, 4, 1)
     ens.close()
     # projection patterns must agree
     paths[1].proj_list[0].stagx = 1

@@ -385,7 +385,7 @@ def test_step_convection_small():
         assert_close(mp.get_nodal(n), st.nodal[n], n, scale=sc.nodal[n])
 
 
-@pytest.mark.parametrize("lanes", [8, 16, 32])
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32])
 def test_step_convection_tdep_full(lanes):
     w = workloads.convection_tdep()
     st, mp = _pair(w)
@@ -431,8 +431,10 @@ def _cmp_vep(st, mp, sc, y_r, r_r, y_g, r_g):
         assert_close(mp.get_nodal(n), st.nodal[n], n, scale=sc.nodal.get(n))
 
 
-def test_step_vep(vep):
+@pytest.mark.parametrize("lanes", [8, 4])
+def test_step_vep(vep, lanes):
     st, mp = _pair(vep)
+    mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(lanes))
     sc = S.StepScales(st, vep=True)
     y_r, r_r = sc.result
     y_g, r_g = mp.step_vep()
