@@ -683,7 +683,8 @@ static int build_blocks(fvm_ctx *ctx, fvm_ensemble *E) {
     E->hblk[1 * (n + 1) + i + 1] = E->hblk[1 * (n + 1) + i] + (nn + 255) / 256;
     for (int gi = 0; gi <= E->ng; gi++) {
       const Lat Lx = lattice(E->gstag_x[gi], I.g.NX), Ly = lattice(E->gstag_y[gi], I.g.NY);
-      const int64_t th = (int64_t)Lx.n * Ly.n * G;
+      // row 2 + ng: the dTsub pass, which takes at least 8 lanes per cell
+      const int64_t th = (int64_t)Lx.n * Ly.n * (gi == E->ng && G < 8 ? 8 : G);
       const int r = 2 + gi;
       E->hblk[r * (n + 1) + i + 1] = E->hblk[r * (n + 1) + i] + (th + 255) / 256;
     }
@@ -949,11 +950,11 @@ static int launch_kb_partials_subt(fvm_ctx *ctx, fvm_ensemble *E) {
   const double *mt = E->pcols + (int64_t)E->mt_role * E->nmark;
   const int32_t *bi = bi_row(E, 2 + gi);
   FVM_PHASE(ctx, FVM_PHASE_SUBGRID_T1);
+  // the dTsub pass writes one value per marker at its store index: with fewer
+  // than 8 lanes per cell those writes split into more partial lines, so it
+  // takes 8 lanes when the projections take 4 (as the single-instance pass)
   switch (E->tune.lanes) {
     case 4:
-      kb_partials_subt<4><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
-                                                                   E->xs, E->ys, mt, E->part, E->dTsub);
-      break;
     case 8:
       kb_partials_subt<8><<<(unsigned)nblk, 256, 0, ctx->stream>>>(E->dinst, n, blk, bi, groups, po, E->start, E->perm,
                                                                    E->xs, E->ys, mt, E->part, E->dTsub);

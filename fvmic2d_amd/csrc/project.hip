@@ -201,8 +201,9 @@ static int dispatch_method(fvm_ctx *ctx, const fvm_grid *g, const Payload &mk, c
   k_partials_subt<GG><<<fvm_blocks(nL * GG, 256), 256, 0, ctx->stream>>>(                                 \
       dgrid(g), pg, *sb, (const int32_t *)ctx->start.ptr, (const int32_t *)ctx->perm.ptr, mk.xs, mk.ys, part, \
       Lx.n, Ly.n)
-    if (G == 4) LAUNCH_S(4);
-  else if (G == 8) LAUNCH_S(8);
+    // 4 lanes per cell split the per-marker dTsub writes into more partial
+    // lines: the pass takes 8 then
+    if (G <= 8) LAUNCH_S(8);
     else if (G == 32) LAUNCH_S(32);
     else LAUNCH_S(16);
 #undef LAUNCH_S

@@ -100,7 +100,8 @@ void *fvm_ctx_stream(fvm_ctx *ctx);
 int fvm_ctx_sync(fvm_ctx *ctx);
 /* Bytes of device workspace the context currently holds (sort buffers, partials). */
 size_t fvm_ctx_workspace_bytes(const fvm_ctx *ctx);
-/* Lanes per lattice cell in the projection passes: 8 (default), 4, 16 or 32.  The
+/* Lanes per lattice cell in the projection passes: 8 (default), 4, 16 or 32 (the
+ * fused dTsub pass of subgridTemperatureChanges takes 8 when this is 4).  The
  * lane count fixes the summation tree, so results are bitwise reproducible for
  * a given setting and agree across settings to the projection tolerance. */
 int fvm_ctx_set_lanes(fvm_ctx *ctx, int32_t lanes);
