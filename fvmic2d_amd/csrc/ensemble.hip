@@ -329,8 +329,9 @@ __global__ void __launch_bounds__(512) kb_scatter(const InstDev *__restrict__ in
   ens_scatter_block(insts[ii], blockIdx.x - blk[ii], key, slot, start, perm);
 }
 
-// Bucket order of every instance's cells (one wavefront per cell) with the
-// payload gathered from the owning instance's columns.  Cells are visited in
+// Bucket order of every instance's cells (two cells per wavefront, one above
+// the other, bucket_order_pair) with the payload gathered from the owning
+// instance's columns.  Cells are visited in
 // 8 x 8 tiles per instance (y fastest inside a tile; tile_off = first tile of
 // each instance), tiles spread over the XCDs in contiguous runs, as in the
 // single-instance ordering pass.
