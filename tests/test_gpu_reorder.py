@@ -148,3 +148,26 @@ def test_ensemble_members_reordered_between_steps():
             for o in [p[4] for p in a.w.projections]:
                 assert_bitwise(b.get_nodal(o), a.get_nodal(o), "step %d instance %d %s" % (s, i, o))
     ens.close()
+
+
+def test_reorder_edge_cases():
+    """No markers: nothing to do; every marker outside the domain: the order is
+    the identity (unlocated markers keep their index order); one cell holding a
+    crowded bucket of 300 markers (the one-wavefront ordering path)."""
+    w = workloads.convection_small()
+    w.markers = w.markers[:0]
+    mp = MarkerPath(w)
+    mp.find_marker_cells()
+    assert len(mp.reorder_markers()) == 0
+    w = workloads.convection_small()
+    w.markers = w.markers[:500].copy()
+    w.markers["X"] = -1.0
+    mp = MarkerPath(w)
+    assert mp.find_marker_cells() == 500
+    assert_bitwise(mp.reorder_markers(), np.arange(500, dtype=np.int32), "all outside")
+    w = _scrambled(workloads.convection_small(), seed=4)
+    m = w.markers
+    a = np.arange(0, 300 * 11, 11)
+    m["X"][a] = w.x[7] + 0.2 * (w.x[8] - w.x[7]) * np.linspace(0.1, 0.9, len(a))
+    m["Y"][a] = w.y[3] + 0.2 * (w.y[4] - w.y[3]) * np.linspace(0.9, 0.1, len(a))
+    _check_reorder(w)
