@@ -267,11 +267,12 @@ def run_ensemble(a):
     # the batched step on the workloads' own marker order, then the stores in
     # cell order for everything below
     ens = Ensemble(paths)
-    ens.capture()
+    if not a.no_extras:  # the initial order at its own autotuned shape
+        ens.autotune(reps=3)
     b0, l0, p0 = ens.get_tuning()
     init = initial_order(a, ctx, ens.launch, n0, max(3, a.steps // 2), paths)
     init["reorder_ms_per_step_every_50_steps"] = init["reorder_ms"] / 50
-    init["tuning"] = {"block": b0, "lanes": l0, "persistent": p0}  # the default shape, before the autotuner
+    init["tuning"] = {"block": b0, "lanes": l0, "persistent": p0}
     log("ensemble: initial order %s" % init)
 
     def seq_step():
