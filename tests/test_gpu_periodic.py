@@ -156,8 +156,10 @@ def test_strain_pressure_periodic(vep):
     assert abs(r_gpu - r_ref) <= RTOL * abs(r_ref), (r_gpu, r_ref)
 
 
-def test_step_thermal_periodic(plain):
+@pytest.mark.parametrize("lanes", [8, 4])
+def test_step_thermal_periodic(plain, lanes):
     st, mp = _pair(plain)
+    mp.ctx.call("fvm_ctx_set_lanes", C.c_int32(lanes))
     sc = S.StepScales(st, vep=False)
     mp.step_thermal()
     _cmp_markers(st, mp, exact=("X", "Y", "VX", "VY", "wxy"), scales=sc.marker)
