@@ -4,7 +4,7 @@ split, unpreconditioned relative residual 1e-8) started from the plain, previous
 and learned guesses.
 
 The small case checks the pipeline and that the learned path is taken; the
-config-5 case runs the configured network (257 x 257 nodes, 4 levels, 32 base
+full-size case runs the configured network (257 x 257 nodes, 4 levels, 32 base
 channels) on a small training set and reports the iterations it saves against
 the plain guess (the 40% target is recorded, not asserted: the measured gain is
 what this test reports)."""
@@ -47,7 +47,7 @@ def test_learned_guess_pipeline_small():
 
 
 @pytest.mark.timeout(600)
-def test_learned_guess_config5_size():
+def test_learned_guess_257_box():
     """257 x 257 nodes (the network sees 256 x 256), 4 levels, 32 base channels,
     viscosity contrast 1e4; 24 training states, 3 held out."""
     import time
