@@ -415,7 +415,9 @@ __global__ void __launch_bounds__(256) kb_partials_subt(const InstDev *__restric
   ens_subt_block<G>(insts[ii], ii, blockIdx.x - blk[ii], groups, partoff, start, perm, xs, ys, mt, part, dTsub);
 }
 
-__global__ void __launch_bounds__(512) kb_advect(const InstDev *__restrict__ insts, int n,
+// five waves per SIMD: the tile loop took the body from 90 to 100 VGPRs (four
+// waves); held to 94 without spilling, the phase runs 2.95 ms instead of 3.06
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(5))) kb_advect(const InstDev *__restrict__ insts, int n,
                                                  const int64_t *__restrict__ blk, const int32_t *__restrict__ bi,
                                                  const double *__restrict__ deltaT, const double *__restrict__ dTsub) {
   extern __shared__ double sgrid[];
