@@ -253,8 +253,10 @@ __device__ __forceinline__ void ens_partials_block(const InstDev &I, int ii, int
   const int j = (int)(lt % G);
   const int64_t nT = (int64_t)(I.g.NX + 1) * (I.g.NY + 1);
   const bool inside = t < nT;
-  const int cx = inside ? (int)(t % (I.g.NX + 1)) - 1 : -2;
-  const int cy = inside ? (int)(t / (I.g.NX + 1)) - 1 : -2;
+  // 32-bit division (an instance's lattice has far fewer than 2^31 cells)
+  const uint32_t t32 = (uint32_t)t, w32 = (uint32_t)(I.g.NX + 1);
+  const int cx = inside ? (int)(t32 % w32) - 1 : -2;
+  const int cy = inside ? (int)(t32 / w32) - 1 : -2;
   const Lat Lx = lattice(sx, I.g.NX), Ly = lattice(sy, I.g.NY);
   const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
   const int64_t nL = (int64_t)Lx.n * Ly.n;

@@ -235,6 +235,12 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
     y1 = sy == -1 ? g.yc[ly + 1] : (sy == 0 ? g.y[ly + 1] : g.yc[ly + 2]);
   }
   const double xden = x1 - x0, yden = y1 - y0;
+  // ARITHMETIC weights are continuous in the fractional position, so they take
+  // it from one reciprocal per lattice cell (within an ulp of the quotient: the
+  // sums stay within rounding); the nearest-corner and tie rules compare it
+  // with 0.5 and the dTsub pass follows subgridTemperatureChanges' expression,
+  // so those keep the division
+  const double xinv = 1.0 / xden, yinv = 1.0 / yden;
   // Lane j takes entries j, j+G, j+2G, ... of the concatenated ranges and adds
   // them in that order.  U entries are fetched together (their loads in
   // flight at once) before they are added; with more than three fields one at
@@ -273,8 +279,8 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
     for (int u = 0; u < U; u++) {
       if (mm[u] < 0) break;
       const int m = mm[u];
-      const double mdx = (xs[u] - x0) / xden;
-      const double mdy = (ys[u] - y0) / yden;
+      const double mdx = METHOD == PM_ARITH && !SUBT ? (xs[u] - x0) * xinv : (xs[u] - x0) / xden;
+      const double mdy = METHOD == PM_ARITH && !SUBT ? (ys[u] - y0) * yinv : (ys[u] - y0) / yden;
       double f[NF];
       if constexpr (SUBT) {
         // subgridTemperatureChanges.c:53-67, same operations and order
