@@ -261,7 +261,10 @@ __device__ __forceinline__ void ens_partials_block(const InstDev &I, int ii, int
   const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
   const int64_t nL = (int64_t)Lx.n * Ly.n;
   const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
-  partials_cell_sorted<METHOD, NF, G>(I.g, all_groups[(int64_t)gi * n + ii], start + I.bucket_off, xs, ys, ms,
+  const ProjGroup &pg = all_groups[(int64_t)gi * n + ii];
+  __shared__ double smat[NF * FVM_MAXMAT];
+  const double *mtab = stage_mat_tables<NF>(pg, smat);
+  partials_cell_sorted<METHOD, NF, G>(I.g, pg, start + I.bucket_off, xs, ys, ms, mtab,
                                       part + all_partoff[(int64_t)gi * n + ii], nL, tl, live ? cx : 0,
                                       live ? cy : 0, j, live);
 }

@@ -119,6 +119,14 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t nb) {
   return x * q + (x < r ? x : r) + k;
 }
 
+// A device pointer read from memory is generic to the compiler, and a load
+// through it is a flat load, which also counts on the scalar/LDS counter: a
+// wait for any scalar load then waits for it too.  Marker columns (device
+// global memory) read through a pointer held in a struct are read through this
+// cast, as global loads.
+typedef __attribute__((address_space(1))) const double gdouble;
+__device__ __forceinline__ const gdouble *global_ptr(const double *p) { return (const gdouble *)p; }
+
 // Wavefront-uniform lane broadcast (v_readlane): cheaper than ds_bpermute.
 __device__ __forceinline__ int read_lane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 

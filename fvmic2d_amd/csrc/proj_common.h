@@ -150,14 +150,15 @@ struct SubT {
 };
 
 // SORTED: MX, MY, Mat and the source columns are payload arrays in bucket
-// order (indexed by the position in perm) instead of marker columns.
+// order (indexed by the position in perm) instead of marker columns.  mtab:
+// the group's material tables staged in LDS (stage_mat_tables; unused by SUBT).
 template <int METHOD, int NF, int G, bool SUBT, bool SORTED>
 __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGroup &pg,
                                                    const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
                                                    const double *__restrict__ MX, const double *__restrict__ MY,
-                                                   const int32_t *__restrict__ Mat, double *__restrict__ part,
-                                                   int64_t nL, int64_t t, int lx, int ly, int j, bool live,
-                                                   const SubT &sb) {
+                                                   const int32_t *__restrict__ Mat, const double *__restrict__ mtab,
+                                                   double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly,
+                                                   int j, bool live, const SubT &sb) {
   const int sx = pg.stagx, sy = pg.stagy;
   bool any_mat = false;
 #pragma unroll
@@ -264,20 +265,25 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int m = mm[u] >= 0 ? mm[u] : mm[0];  // mm[0] >= 0: e0 < tot
-      xs[u] = MX[m];
-      ys[u] = MY[m];
       if constexpr (SUBT) {
+        xs[u] = MX[m];
+        ys[u] = MY[m];
         vs[u][0] = sb.MT[m];
         mats[u] = 0;
       } else {
+        // the material index first: the table reads wait for it alone
         mats[u] = any_mat ? Mat[m] : 0;
 #pragma unroll
-        for (int k = 0; k < NF; k++) vs[u][k] = pg.src[k].use_mat ? 0.0 : pg.src[k].col[m];
+        for (int k = 0; k < NF; k++) vs[u][k] = pg.src[k].use_mat ? 0.0 : global_ptr(pg.src[k].col)[m];
+        xs[u] = MX[m];
+        ys[u] = MY[m];
       }
     }
+    // every load of the round is issued before the first use of any of them
+    if constexpr (!SUBT) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      if (mm[u] < 0) break;
+      if (u > 0 && mm[u] < 0) break;  // mm[0] >= 0: e0 < tot
       const int m = mm[u];
       const double mdx = METHOD == PM_ARITH && !SUBT ? (xs[u] - x0) * xinv : (xs[u] - x0) / xden;
       const double mdy = METHOD == PM_ARITH && !SUBT ? (ys[u] - y0) * yinv : (ys[u] - y0) / yden;
@@ -298,11 +304,15 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
 #pragma unroll
         for (int k = 0; k < NF; k++) f[k] = d;
       } else {
+        // the table read is unconditional (index masked into the table); a
+        // material index outside it gives NaN
         const int mat = mats[u];
+        const bool mok = (unsigned)mat < FVM_MAXMAT;
 #pragma unroll
-        for (int k = 0; k < NF; k++)
-          f[k] = pg.src[k].use_mat ? (((unsigned)mat < FVM_MAXMAT) ? pg.src[k].mat[mat] : __builtin_nan(""))
-                                   : vs[u][k];
+        for (int k = 0; k < NF; k++) {
+          const double tv = mtab[k * FVM_MAXMAT + (mat & (FVM_MAXMAT - 1))];
+          f[k] = pg.src[k].use_mat ? (mok ? tv : __builtin_nan("")) : vs[u][k];
+        }
       }
       accumulate<METHOD, NF>(mdx, mdy, f, sw, sf);
     }
@@ -340,40 +350,31 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
   }
 }
 
-template <int METHOD, int NF, int G>
-__device__ __forceinline__ void partials_cell(const DGrid &g, const ProjGroup &pg, const int32_t *__restrict__ start,
-                                              const int32_t *__restrict__ perm, const double *__restrict__ MX,
-                                              const double *__restrict__ MY, const int32_t *__restrict__ Mat,
-                                              double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly, int j,
-                                              bool live) {
-  const SubT none = {};
-  partials_cell_impl<METHOD, NF, G, false, false>(g, pg, start, perm, MX, MY, Mat, part, nL, t, lx, ly, j, live,
-                                                  none);
+// The group's material tables in LDS, smat[k * FVM_MAXMAT + mat] for its NF
+// fields: a marker's lookup is then an LDS read, which does not wait behind
+// the marker's own global loads.  Every thread of the block calls it with the
+// same group; the first barrier lets a previous table go out of use.
+template <int NF>
+__device__ __forceinline__ const double *stage_mat_tables(const ProjGroup &pg, double *smat) {
+  __syncthreads();
+  for (int i = threadIdx.x; i < NF * FVM_MAXMAT; i += blockDim.x)
+    smat[i] = pg.src[i / FVM_MAXMAT].mat[i % FVM_MAXMAT];
+  __syncthreads();
+  return smat;
 }
 
-// The basic-node projection of subgridTemperatureChanges' dTsub with pass 1
-// computed on the fly (method ARITHMETIC, one field).
-template <int G>
-__device__ __forceinline__ void partials_cell_subt(const DGrid &g, const ProjGroup &pg,
-                                                   const int32_t *__restrict__ start, const int32_t *__restrict__ perm,
-                                                   const double *__restrict__ MX, const double *__restrict__ MY,
-                                                   double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly,
-                                                   int j, bool live, const SubT &sb) {
-  partials_cell_impl<PM_ARITH, 1, G, true, false>(g, pg, start, perm, MX, MY, nullptr, part, nL, t, lx, ly, j, live,
-                                                  sb);
-}
-
-// The same over bucket-ordered payload (xs, ys, ms and the columns named in pg
+// Pass 1 over the bucket-ordered payload (xs, ys, ms and the columns named in pg
 // or sb.MT hold position-ordered copies; perm maps a position to its marker).
 template <int METHOD, int NF, int G>
 __device__ __forceinline__ void partials_cell_sorted(const DGrid &g, const ProjGroup &pg,
                                                      const int32_t *__restrict__ start,
                                                      const double *__restrict__ xs, const double *__restrict__ ys,
-                                                     const int32_t *__restrict__ ms, double *__restrict__ part,
-                                                     int64_t nL, int64_t t, int lx, int ly, int j, bool live) {
+                                                     const int32_t *__restrict__ ms, const double *__restrict__ mtab,
+                                                     double *__restrict__ part, int64_t nL, int64_t t, int lx, int ly,
+                                                     int j, bool live) {
   const SubT none = {};
-  partials_cell_impl<METHOD, NF, G, false, true>(g, pg, start, nullptr, xs, ys, ms, part, nL, t, lx, ly, j, live,
-                                                 none);
+  partials_cell_impl<METHOD, NF, G, false, true>(g, pg, start, nullptr, xs, ys, ms, mtab, part, nL, t, lx, ly, j,
+                                                 live, none);
 }
 
 template <int G>
@@ -384,8 +385,8 @@ __device__ __forceinline__ void partials_cell_subt_sorted(const DGrid &g, const 
                                                           const double *__restrict__ ys, double *__restrict__ part,
                                                           int64_t nL, int64_t t, int lx, int ly, int j, bool live,
                                                           const SubT &sb) {
-  partials_cell_impl<PM_ARITH, 1, G, true, true>(g, pg, start, perm, xs, ys, nullptr, part, nL, t, lx, ly, j, live,
-                                                 sb);
+  partials_cell_impl<PM_ARITH, 1, G, true, true>(g, pg, start, perm, xs, ys, nullptr, nullptr, part, nL, t, lx, ly,
+                                                 j, live, sb);
 }
 
 // ---- pass 2: one thread per (node, field) ------------------------------------------

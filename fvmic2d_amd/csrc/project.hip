@@ -32,7 +32,9 @@ __global__ void __launch_bounds__(256) k_partials(DGrid g, ProjGroup pg, const i
   const bool live = t < nL;
   const int lx = live ? (int)(t % nLx) + lox : 0;
   const int ly = live ? (int)(t / nLx) + loy : 0;
-  partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, part, nL, t, lx, ly, j, live);
+  __shared__ double smat[NF * FVM_MAXMAT];
+  const double *mtab = stage_mat_tables<NF>(pg, smat);
+  partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, mtab, part, nL, t, lx, ly, j, live);
 }
 
 // Several groups of one kind (method, field count) in one launch: the launch
@@ -70,8 +72,10 @@ __global__ void __launch_bounds__(256) k_partials_multi(DGrid g, MultiGroups mg,
   const bool live = cx >= Lx.lo && cx < Lx.lo + Lx.n && cy >= Ly.lo && cy < Ly.lo + Ly.n;
   const int64_t nL = (int64_t)Lx.n * Ly.n;
   const int64_t tl = live ? (int64_t)(cy - Ly.lo) * Lx.n + (cx - Lx.lo) : 0;
-  partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, mg.part[q], nL, tl, live ? cx : 0, live ? cy : 0, j,
-                                      live);
+  __shared__ double smat[NF * FVM_MAXMAT];
+  const double *mtab = stage_mat_tables<NF>(pg, smat);
+  partials_cell_sorted<METHOD, NF, G>(g, pg, start, xs, ys, ms, mtab, mg.part[q], nL, tl, live ? cx : 0,
+                                      live ? cy : 0, j, live);
 }
 
 template <int G>
