@@ -280,7 +280,7 @@ __device__ __forceinline__ void partials_cell_impl(const DGrid &g, const ProjGro
       }
     }
     // every load of the round is issued before the first use of any of them
-    if constexpr (!SUBT) __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int u = 0; u < U; u++) {
       if (u > 0 && mm[u] < 0) break;  // mm[0] >= 0: e0 < tot
