@@ -193,10 +193,14 @@ __device__ __forceinline__ int wrap_x(int ix, int NX, int per) {
 
 // Read of a ghosted local array: indices outside the grid read 0 (a fresh,
 // zeroed DMDA local vector; DMGlobalToLocal fills only owned nodes); on a
-// periodic grid the x ghosts read the wrapped column.
+// periodic grid the x ghosts read the wrapped column.  The load itself is
+// unconditional (an index outside the grid reads element 0 and the value is
+// dropped), so a kernel's gathers are issued together, without a branch each.
 __device__ __forceinline__ double gat(const double *__restrict__ f, int NX, int NY, int per, int jy, int ix) {
   ix = wrap_x(ix, NX, per);
-  return (ix >= 0 && ix < NX && jy >= 0 && jy < NY) ? f[(size_t)jy * NX + ix] : 0.0;
+  const bool in = ix >= 0 && ix < NX && jy >= 0 && jy < NY;
+  const double v = f[in ? (size_t)jy * NX + ix : 0];
+  return in ? v : 0.0;
 }
 
 // The reference's bilinear gather written as four accumulations
