@@ -148,6 +148,47 @@ def test_projection_families(small, method):
                 assert_close(got, ref, what, scale=np.abs(sa.nodal["out"]))
 
 
+@pytest.mark.parametrize("sx,sy", [(0, 0), (-1, -1)])
+def test_projection_material_index_outside_table(sx, sy):
+    """A marker whose material index lies outside the table (here 9 and -3)
+    makes every node it touches NaN, as an out-of-bounds table read has no
+    value; every other node is bitwise the projection with those markers given
+    a valid index (the tables are staged per block in LDS: a wrong slot or a
+    stale table shows up here too)."""
+    import copy
+    w = workloads.convection_small()
+    bad = np.array([5, 1000, 20000])
+    w.markers["Mat"][bad] = [9, -3, 9]
+    print("CASE mat index outside the table, family (%d,%d)" % (sx, sy), flush=True)
+    mp = MarkerPath(w, extra=["out"])
+    mp.find_marker_cells()
+    mp.project_marker_field("mat:kThermal", "out", sx, sy, abi.ARITHMETIC)
+    got = mp.get_nodal("out")
+    wv = copy.deepcopy(w)
+    wv.markers["Mat"][bad] = 0
+    mv = MarkerPath(wv, extra=["out"])
+    mv.find_marker_cells()
+    mv.project_marker_field("mat:kThermal", "out", sx, sy, abi.ARITHMETIC)
+    ref = mv.get_nodal("out")
+    m = mp.get_markers()
+    touched = np.zeros(got.shape, bool)
+    for i in bad:
+        x, y = m["X"][i], m["Y"][i]
+        # the lattice cell of family (sx, sy): basic nodes (0) or centres (-1)
+        xs = w.x if sx == 0 else w.xc
+        ys = w.y if sy == 0 else w.yc
+        ix = int(np.searchsorted(xs, x, side="right")) - 1
+        iy = int(np.searchsorted(ys, y, side="right")) - 1
+        for dy in (0, 1):
+            for dx in (0, 1):
+                jy, jx = iy + dy, ix + dx
+                if 0 <= jy < got.shape[0] and 0 <= jx < got.shape[1]:
+                    touched[jy, jx] = True
+    assert touched.any()
+    assert np.isnan(got[touched]).all(), "touched nodes"
+    assert_bitwise(got[~touched], ref[~touched], "untouched nodes")
+
+
 def test_projection_is_deterministic(small):
     _, mp = _pair(small, extra=["a", "b"])
     mp.find_marker_cells()
